@@ -1,0 +1,148 @@
+/*
+ * mans.h -- C ABI of the MI355X multi-byte ANS codec (libmans.so).
+ *
+ * Plain C, plain pointers and sizes.  Every data pointer is a *device*
+ * pointer unless its name starts with h_.  Streams are hipStream_t passed as
+ * void* (NULL = the legacy default stream).  All calls are stream-ordered and
+ * return a status; none allocates device memory except the workspace calls
+ * (mans_workspace_create / mans_workspace_reserve, and the first call that
+ * needs more workspace than reserved).
+ *
+ * Two wire formats share one container layout (reference
+ * include/ans/GpuANSCodec.h:68-209):
+ *   lanes = 32 -> the reference format: 32 interleaved rANS states per
+ *                 4 KiB block, version 1; byte-identical to the reference's
+ *                 ansEncode output (its uninitialised bytes defined as 0).
+ *   lanes = 64 -> the native format: 64 states per 8 KiB block (one gfx950
+ *                 wavefront per block), version 2.
+ */
+#ifndef MANS_H
+#define MANS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MANS_ABI_VERSION 1
+
+typedef enum {
+  MANS_OK = 0,
+  MANS_ERR_INVALID_ARG = 1,  /* null pointer, bad enum value */
+  MANS_ERR_PRECISION = 2,    /* prob_bits outside {9, 10, 11} */
+  MANS_ERR_ALIGNMENT = 3,    /* pointer not aligned as required */
+  MANS_ERR_TOO_LARGE = 4,    /* size beyond the 32-bit fields of the format */
+  MANS_ERR_CAPACITY = 5,     /* output buffer smaller than required */
+  MANS_ERR_CORRUPT = 6,      /* malformed / truncated / inconsistent stream */
+  MANS_ERR_HIP = 7,          /* HIP runtime error */
+  MANS_ERR_NO_DEVICE = 8,    /* no usable gfx950 device */
+  MANS_ERR_DOMAIN = 9,       /* ADM mapping requested for data outside its domain */
+  MANS_ERR_BUDGET = 10,      /* request exceeds the configured device-memory budget */
+  MANS_ERR_BUSY = 11         /* serving queue full */
+} mans_status;
+
+typedef enum {
+  MANS_MODE_PLANES = 0,      /* element byte k -> plane k (k < elem_bytes) */
+  MANS_MODE_ADM = 1,         /* uint16 ADM mapping: code plane + bit-signal plane */
+  MANS_MODE_AUTO = 2         /* ADM when in its lossless domain, else planes */
+} mans_mode;
+
+typedef struct {
+  int32_t prob_bits;   /* 9, 10 or 11 (reference GpuANSEncode.h:348-360) */
+  int32_t lanes;       /* 32 = reference format, 64 = native format */
+  int32_t elem_bytes;  /* frame API: 1, 2 or 4 */
+  int32_t mode;        /* frame API: mans_mode */
+} mans_params;
+
+typedef struct mans_workspace mans_workspace;
+
+/* ---- library / workspace ---------------------------------------------- */
+const char* mans_status_string(mans_status s);
+int32_t mans_abi_version(void);
+
+/* Creates a workspace on `device` (-1 = current device).  A workspace holds
+ * the histogram / table / look-back scratch of one call at a time: calls
+ * that overlap on the same workspace race; use one workspace per stream. */
+mans_status mans_workspace_create(mans_workspace** ws, int32_t device);
+mans_status mans_workspace_destroy(mans_workspace* ws);
+/* Pre-sizes the workspace for inputs of up to max_bytes (no allocation in
+ * later calls up to that size). */
+mans_status mans_workspace_reserve(mans_workspace* ws, uint64_t max_bytes, const mans_params* p);
+/* Waits for the workspace's last call and returns MANS_ERR_CORRUPT if a
+ * kernel flagged a malformed stream (and clears the flags). */
+mans_status mans_workspace_check(mans_workspace* ws, void* stream, uint32_t* h_flags);
+
+/* ---- single stream: drop-in for the reference GpuANS API -------------- */
+/* Upper bound of the container size for n input bytes (replaces
+ * getMaxCompressedSize, GpuANSCodec.h:237-246; the reference's formula
+ * passes 4096 as the block count and can undershoot, this one cannot). */
+uint64_t mans_max_compressed_size(uint64_t n, const mans_params* p);
+
+/* Encode n bytes at d_in into the container at d_out (16-byte aligned).
+ * Replaces multibyte_ans::ansEncodeBatch (src/GpuANS.cu:16-31) /
+ * ansEncode (include/ans/GpuANSEncode.h:286-401).  The container size is
+ * written by the device to *d_out_size (uint32, as the reference's outSize)
+ * when non-NULL, and to *h_out_size (host; the call then synchronises the
+ * stream) when non-NULL. */
+mans_status mans_encode(const mans_params* p, const void* d_in, uint64_t n, void* d_out,
+                        uint64_t out_capacity, uint32_t* d_out_size, uint64_t* h_out_size,
+                        mans_workspace* ws, void* stream);
+
+/* Decode the container at d_in (c bytes; 0 = unknown, as the reference)
+ * into d_out.  Replaces multibyte_ans::ansDecodeBatch (src/GpuANS.cu:33-44)
+ * / ansDecode (include/ans/GpuANSDecode.h:379-451).  No host synchronisation:
+ * the block count is read on the device.  out_capacity bounds the bytes the
+ * device may write (0 = unchecked, as the reference). */
+mans_status mans_decode(const mans_params* p, const void* d_in, uint64_t c, void* d_out,
+                        uint64_t out_capacity, mans_workspace* ws, void* stream);
+
+/* ---- stage entry points (each reference kernel on its own) ------------- */
+/* 256-bin histogram of the n bytes d_in[i*stride] (ansHistogramBatch,
+ * GpuANSStatistics.h:409-446).  d_hist is overwritten. */
+mans_status mans_histogram(const void* d_in, uint64_t n, uint32_t stride, uint32_t* d_hist,
+                           mans_workspace* ws, void* stream);
+/* Normalised table {pdf, cdf, magic, shift} x 256 from a histogram
+ * (ansCalcWeights / quantizeWeights, GpuANSStatistics.h:186-406,449-468). */
+mans_status mans_normalize(const uint32_t* d_hist, uint32_t total, int32_t prob_bits,
+                           uint32_t* d_table, mans_workspace* ws, void* stream);
+/* ADM mapping of n uint16 values (mapping/cpu/mappingcpuV1.cpp:33-193,
+ * mapping/nv/mappingV15_fused_uint16.cu:400-730).  Writes offsets[gsize+1]
+ * (exclusive prefix of group byte lengths, last = total), centers[gsize],
+ * codes[n] and the bit plane (32*offsets[gsize] bytes, bits_capacity bound).
+ * gsize = ceil(n/512).  *h_bits_len / *h_in_domain (host, synchronising)
+ * when non-NULL. */
+mans_status mans_adm_map(const uint16_t* d_in, uint64_t n, int32_t* d_offsets, uint16_t* d_centers,
+                         uint8_t* d_codes, uint8_t* d_bits, uint64_t bits_capacity,
+                         uint64_t* h_bits_len, int32_t* h_in_domain, mans_workspace* ws, void* stream);
+/* Inverse mapping (mappingcpuV1.cpp:195-276, mappingV15_fused_uint16.cu:24-121). */
+mans_status mans_adm_unmap(const int32_t* d_offsets, const uint16_t* d_centers, const uint8_t* d_codes,
+                           const uint8_t* d_bits, uint64_t n, uint16_t* d_out, mans_workspace* ws,
+                           void* stream);
+
+/* ---- multi-byte frames (byte planes or ADM planes, one container) ------ */
+uint64_t mans_frame_max_size(uint64_t n_elems, const mans_params* p);
+/* Encode n_elems elements of p->elem_bytes bytes.  Frame size to *d_size
+ * (device, uint64) and/or *h_size (host, synchronising). */
+mans_status mans_frame_encode(const mans_params* p, const void* d_in, uint64_t n_elems, void* d_out,
+                              uint64_t out_capacity, uint64_t* d_size, uint64_t* h_size,
+                              mans_workspace* ws, void* stream);
+/* Reads the 128-byte frame header (synchronising) and decodes into d_out. */
+mans_status mans_frame_decode(const void* d_in, uint64_t c, void* d_out, uint64_t out_capacity,
+                              uint64_t* h_n_elems, mans_workspace* ws, void* stream);
+/* Parses a frame header already on the host. */
+mans_status mans_frame_info(const void* h_header, uint64_t header_bytes, uint64_t* n_elems,
+                            int32_t* elem_bytes, int32_t* mode, uint64_t* total_size);
+
+/* ---- profiling --------------------------------------------------------- */
+/* When enabled, each call records HIP events around its kernels; the
+ * durations (ms) of the last call are returned by mans_last_kernel_times in
+ * launch order with their kernel names. */
+mans_status mans_set_profiling(mans_workspace* ws, int32_t enable);
+int32_t mans_last_kernel_times(mans_workspace* ws, double* h_ms, const char** h_names, int32_t max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MANS_H */

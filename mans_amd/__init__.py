@@ -1,0 +1,255 @@
+"""MI355X multi-byte ANS codec: Python host bindings over libmans.so.
+
+The product is the C ABI in ``include/mans.h`` (hand-written HIP kernels for
+gfx950 behind it); this module binds it with ctypes for tests, benchmarks and
+Python callers.  Device buffers are torch tensors on ``cuda`` (torch is only
+used for device memory and streams).  There is no CPU fallback: every entry
+point raises if the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmans.so")
+CLI_PATH = os.path.join(HERE, "bin", "compress")
+
+MODE_PLANES = 0
+MODE_ADM = 1
+MODE_AUTO = 2
+
+STATUS = {
+    0: "MANS_OK", 1: "MANS_ERR_INVALID_ARG", 2: "MANS_ERR_PRECISION", 3: "MANS_ERR_ALIGNMENT",
+    4: "MANS_ERR_TOO_LARGE", 5: "MANS_ERR_CAPACITY", 6: "MANS_ERR_CORRUPT", 7: "MANS_ERR_HIP",
+    8: "MANS_ERR_NO_DEVICE", 9: "MANS_ERR_DOMAIN", 10: "MANS_ERR_BUDGET", 11: "MANS_ERR_BUSY",
+}
+
+# Every function declared in include/mans.h, with its ctypes signature.
+_u64, _u32, _i32, _vp = C.c_uint64, C.c_uint32, C.c_int32, C.c_void_p
+
+
+class Params(C.Structure):
+    _fields_ = [("prob_bits", C.c_int32), ("lanes", C.c_int32), ("elem_bytes", C.c_int32), ("mode", C.c_int32)]
+
+
+_P = C.POINTER(Params)
+SIGNATURES = {
+    "mans_status_string": (C.c_char_p, [C.c_int]),
+    "mans_abi_version": (_i32, []),
+    "mans_workspace_create": (C.c_int, [C.POINTER(_vp), _i32]),
+    "mans_workspace_destroy": (C.c_int, [_vp]),
+    "mans_workspace_reserve": (C.c_int, [_vp, _u64, _P]),
+    "mans_workspace_check": (C.c_int, [_vp, _vp, C.POINTER(_u32)]),
+    "mans_max_compressed_size": (_u64, [_u64, _P]),
+    "mans_encode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, C.POINTER(_u64), _vp, _vp]),
+    "mans_decode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, _vp]),
+    "mans_histogram": (C.c_int, [_vp, _u64, _u32, _vp, _vp, _vp]),
+    "mans_normalize": (C.c_int, [_vp, _u32, _i32, _vp, _vp, _vp]),
+    "mans_adm_map": (C.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _u64, C.POINTER(_u64), C.POINTER(_i32), _vp, _vp]),
+    "mans_adm_unmap": (C.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp]),
+    "mans_frame_max_size": (_u64, [_u64, _P]),
+    "mans_frame_encode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, C.POINTER(_u64), _vp, _vp]),
+    "mans_frame_decode": (C.c_int, [_vp, _u64, _vp, _u64, C.POINTER(_u64), _vp, _vp]),
+    "mans_frame_info": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64)]),
+    "mans_set_profiling": (C.c_int, [_vp, _i32]),
+    "mans_last_kernel_times": (_i32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_char_p), _i32]),
+}
+
+
+class MansError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__(f"{what}: {STATUS.get(status, status)}")
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libmans.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m mans_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise MansError(rc, what)
+
+
+def params(prob_bits: int = 10, lanes: int = 32, elem_bytes: int = 1, mode: int = MODE_PLANES) -> Params:
+    return Params(prob_bits, lanes, elem_bytes, mode)
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("mans_amd needs a GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _stream(stream=None) -> int:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+class Workspace:
+    """Device scratch for one stream (see mans_workspace_create)."""
+
+    def __init__(self, device: int = 0):
+        _torch()
+        self.device = device
+        h = C.c_void_p()
+        _check(lib().mans_workspace_create(C.byref(h), device), "mans_workspace_create")
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            lib().mans_workspace_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reserve(self, max_bytes: int, p: Params | None = None) -> None:
+        _check(lib().mans_workspace_reserve(self.handle, max_bytes, C.byref(p) if p else None), "reserve")
+
+    def check(self, stream=None) -> int:
+        flags = C.c_uint32(0)
+        rc = lib().mans_workspace_check(self.handle, _stream(stream), C.byref(flags))
+        if rc != 0:
+            raise MansError(rc, f"device flags 0x{flags.value:x}")
+        return flags.value
+
+    def set_profiling(self, on: bool = True) -> None:
+        _check(lib().mans_set_profiling(self.handle, 1 if on else 0), "set_profiling")
+
+    def kernel_times(self) -> list[tuple[str, float]]:
+        ms = (C.c_double * 16)()
+        names = (C.c_char_p * 16)()
+        n = lib().mans_last_kernel_times(self.handle, ms, names, 16)
+        return [(names[i].decode(), ms[i]) for i in range(n)]
+
+
+def max_compressed_size(n: int, p: Params) -> int:
+    return lib().mans_max_compressed_size(n, C.byref(p))
+
+
+def frame_max_size(n_elems: int, p: Params) -> int:
+    return lib().mans_frame_max_size(n_elems, C.byref(p))
+
+
+def _ptr(t) -> int:
+    return t.data_ptr() if t is not None and t.numel() > 0 else 0
+
+
+def encode(data, p: Params, ws: Workspace, out=None, stream=None, sync_size: bool = True):
+    """Encode a uint8 CUDA tensor into one GpuANS container.
+
+    Returns (out_buffer, size) when sync_size, else (out_buffer, d_size) with
+    the size as a device uint32 tensor."""
+    torch = _torch()
+    n = data.numel() * data.element_size()
+    cap = max_compressed_size(n, p)
+    if out is None:
+        out = torch.empty(cap, dtype=torch.uint8, device=data.device)
+    d_size = torch.zeros(1, dtype=torch.int32, device=data.device)
+    h_size = C.c_uint64(0)
+    rc = lib().mans_encode(C.byref(p), _ptr(data), n, out.data_ptr(), out.numel(), d_size.data_ptr(),
+                           C.byref(h_size) if sync_size else None, ws.handle, _stream(stream))
+    _check(rc, "mans_encode")
+    return (out, h_size.value) if sync_size else (out, d_size)
+
+
+def decode(comp, n: int, p: Params, ws: Workspace, out=None, stream=None, c: int | None = None):
+    torch = _torch()
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=torch.uint8, device=comp.device)
+    rc = lib().mans_decode(C.byref(p), comp.data_ptr(), comp.numel() if c is None else c, out.data_ptr(),
+                           out.numel(), ws.handle, _stream(stream))
+    _check(rc, "mans_decode")
+    return out[:n]
+
+
+def histogram(data, ws: Workspace, stride: int = 1, stream=None):
+    torch = _torch()
+    nbytes = data.numel() * data.element_size()
+    h = torch.empty(256, dtype=torch.int32, device=data.device)
+    _check(lib().mans_histogram(_ptr(data), nbytes // stride, stride, h.data_ptr(), ws.handle, _stream(stream)),
+           "mans_histogram")
+    return h
+
+
+def normalize(hist, total: int, prob_bits: int, ws: Workspace, stream=None):
+    torch = _torch()
+    t = torch.empty((256, 4), dtype=torch.int32, device=hist.device)
+    _check(lib().mans_normalize(hist.data_ptr(), total, prob_bits, t.data_ptr(), ws.handle, _stream(stream)),
+           "mans_normalize")
+    return t
+
+
+def adm_map(x, ws: Workspace, stream=None):
+    """ADM mapping of a uint16 (int16-typed) CUDA tensor.
+
+    Returns (in_domain, offsets[gsize+1], centers[gsize], codes[n], bits)."""
+    torch = _torch()
+    n = x.numel()
+    g = (n + 511) // 512
+    offs = torch.empty(g + 1, dtype=torch.int32, device=x.device)
+    cen = torch.empty(max(g, 1), dtype=torch.int16, device=x.device)
+    codes = torch.empty(max(n, 16), dtype=torch.uint8, device=x.device)
+    cap = g * 1024
+    bits = torch.empty(max(cap, 16), dtype=torch.uint8, device=x.device)
+    blen = C.c_uint64(0)
+    dom = C.c_int32(0)
+    _check(lib().mans_adm_map(_ptr(x), n, offs.data_ptr(), cen.data_ptr(), codes.data_ptr(), bits.data_ptr(), cap,
+                              C.byref(blen), C.byref(dom), ws.handle, _stream(stream)), "mans_adm_map")
+    return bool(dom.value), offs, cen[:g], codes[:n], bits[: blen.value]
+
+
+def adm_unmap(offsets, centers, codes, bits, n: int, ws: Workspace, stream=None):
+    torch = _torch()
+    out = torch.empty(max(n, 8), dtype=torch.int16, device=offsets.device)
+    _check(lib().mans_adm_unmap(offsets.data_ptr(), _ptr(centers), _ptr(codes), _ptr(bits), n, out.data_ptr(),
+                                ws.handle, _stream(stream)), "mans_adm_unmap")
+    return out[:n]
+
+
+def frame_encode(x, p: Params, ws: Workspace, out=None, stream=None, sync_size: bool = True):
+    """Encode a CUDA tensor of 1/2/4-byte elements into a MANS frame."""
+    torch = _torch()
+    n = x.numel()
+    p = Params(p.prob_bits, p.lanes, x.element_size(), p.mode)
+    cap = frame_max_size(n, p)
+    if out is None:
+        out = torch.empty(cap, dtype=torch.uint8, device=x.device)
+    d_size = torch.zeros(1, dtype=torch.int64, device=x.device)
+    h_size = C.c_uint64(0)
+    rc = lib().mans_frame_encode(C.byref(p), _ptr(x), n, out.data_ptr(), out.numel(), d_size.data_ptr(),
+                                 C.byref(h_size) if sync_size else None, ws.handle, _stream(stream))
+    _check(rc, "mans_frame_encode")
+    return (out, h_size.value) if sync_size else (out, d_size)
+
+
+def frame_decode(comp, size: int, dtype, n: int, ws: Workspace, out=None, stream=None):
+    torch = _torch()
+    if out is None:
+        out = torch.empty(max(n, 8), dtype=dtype, device=comp.device)
+    nout = C.c_uint64(0)
+    rc = lib().mans_frame_decode(comp.data_ptr(), size, out.data_ptr(), out.numel() * out.element_size(),
+                                 C.byref(nout), ws.handle, _stream(stream))
+    _check(rc, "mans_frame_decode")
+    return out[:n]

@@ -1,0 +1,267 @@
+// adm.hip -- ADM byte mapping of uint16 data (code plane + bit-signal plane).
+//
+// Reference: mapping/cpu/mappingcpuV1.cpp:33-276 (semantics) and
+// mapping/nv/mappingV15_fused_uint16.cu:24-121, 124-398, 400-730 (GPU
+// kernels).  Group g = elements [512g, 512g + 512) = 32 lanes x 16 elements.
+// center = floor(sum / count); per element d = |x - c|, len = 1 if x == c else
+// ceil(d / 126), code = 1 if x == c else 2(d - 126(len - 1)) (+1 if x < c);
+// each lane sets bit `bo` (MSB first) then advances bo by len; the group's
+// lanes are padded to M_g = ceil(max L / 8) bytes (the byte holding bit L gets
+// 0xFF >> (L % 8)), lanes without elements stay zero (mappingcpuV1.cpp:146);
+// lane l's bytes go to 32 O_g + l M_g with O = exclusive prefix of M.
+//
+// Structure: one wavefront maps two groups (one per half-wave).  The lane
+// bit buffers live in LDS (ds_or per element).  O_g is resolved by decoupled
+// look-back over tickets (any number of groups; the reference's fused path is
+// limited to 1024 groups, mappingV15_fused_uint16.cu:851).  Unmap parses
+// each lane's bytes with clz over 32-bit words.
+#include "mans_internal.hpp"
+
+namespace mans {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagPrefix = 2ull << 62;
+constexpr uint64_t kValueMask = (1ull << 62) - 1;
+
+__device__ __forceinline__ uint32_t half_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t half_max(uint32_t v) {
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) {
+    uint32_t u = __shfl_xor(v, o, 64);
+    v = v > u ? v : u;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
+  __shared__ uint32_t s_bits[kWaves][64][kAdmLaneBytes / 4];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint32_t half = (uint32_t)lane >> 5, lin = (uint32_t)lane & 31u;
+  uint32_t* mybits = s_bits[wave][lane];
+  uint8_t* wavebytes = (uint8_t*)s_bits[wave];
+
+  for (;;) {
+    uint32_t tk = 0;
+    if (lane == 0) tk = atomicAdd(a.ticket, 2u);
+    tk = __builtin_amdgcn_readfirstlane(tk);
+    if ((uint64_t)tk >= a.gsize) break;
+    const uint64_t g = (uint64_t)tk + half;
+    const bool gactive = g < a.gsize;
+    const uint64_t base = g * kAdmGroup + (uint64_t)lin * kAdmChunk;
+    const uint32_t cnt = gactive && base < a.n ? (uint32_t)((a.n - base) < kAdmChunk ? (a.n - base) : kAdmChunk) : 0u;
+
+    uint32_t v[kAdmChunk];
+    if (cnt == kAdmChunk) {
+      const uint4* p = (const uint4*)(a.in + base);
+      uint4 q0 = p[0], q1 = p[1];
+      uint32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) { v[2 * i] = w[i] & 0xffffu; v[2 * i + 1] = w[i] >> 16; }
+    } else {
+#pragma unroll
+      for (int i = 0; i < (int)kAdmChunk; ++i) v[i] = (uint32_t)i < cnt ? (uint32_t)a.in[base + i] : 0u;
+    }
+    uint32_t lsum = 0;
+#pragma unroll
+    for (int i = 0; i < (int)kAdmChunk; ++i) lsum += (uint32_t)i < cnt ? v[i] : 0u;
+    const uint32_t gsum = half_sum(lsum), gcnt = half_sum(cnt);
+    const uint32_t c = gcnt ? gsum / gcnt : 0u;  // mappingcpuV1.cpp:60-66
+
+#pragma unroll
+    for (int i = 0; i < (int)(kAdmLaneBytes / 4); ++i) mybits[i] = 0;
+    __threadfence_block();
+    uint32_t codes[4] = {0, 0, 0, 0};
+    uint32_t bo = 0;
+    bool over = false;
+#pragma unroll
+    for (int i = 0; i < (int)kAdmChunk; ++i) {
+      if ((uint32_t)i < cnt) {
+        const uint32_t x = v[i];
+        const uint32_t d = x > c ? x - c : c - x;
+        const uint32_t len = x == c ? 1u : (d + kAdmMod - 1) / kAdmMod;
+        const uint32_t code = x == c ? 1u : ((d - kAdmMod * (len - 1)) * 2u + (x > c ? 0u : 1u));
+        codes[i >> 2] |= (code & 0xffu) << (8 * (i & 3));
+        if (bo < 8 * kAdmLaneBytes) atomicOr(mybits + (bo >> 5), 1u << (((bo >> 3) & 3u) * 8u + 7u - (bo & 7u)));
+        else over = true;
+        bo += len;
+      }
+    }
+    if (bo > 8 * kAdmLaneBytes) over = true;
+    const uint32_t lbytes = (bo + 7u) / 8u;
+    uint32_t m = half_max(cnt ? lbytes : 0u);
+    if (m > kAdmLaneBytes) m = kAdmLaneBytes;
+    if (cnt && bo < 8u * m) {
+      const uint32_t pad = (bo & 7u) == 0 ? 0xffu : (0xffu >> (bo & 7u));
+      atomicOr(mybits + (bo >> 5), pad << (((bo >> 3) & 3u) * 8u));
+    }
+    if (__ballot(over) && lin == 0) atomicOr(a.err, kErrAdmDomain);
+    // codes: 16 bytes per lane
+    if (cnt == kAdmChunk) {
+      *(uint4*)(a.codes + base) = make_uint4(codes[0], codes[1], codes[2], codes[3]);
+    } else {
+      for (uint32_t i = 0; i < cnt; ++i) a.codes[base + i] = (uint8_t)(codes[i >> 2] >> (8 * (i & 3)));
+    }
+
+    // ---- look-back for O_g (units of bytes per lane) ----------------------
+    const uint32_t m_lo = __builtin_amdgcn_readlane(m, 0), m_hi = __builtin_amdgcn_readlane(m, 32);
+    const uint64_t agg = (uint64_t)m_lo + m_hi;
+    const uint64_t chain = tk / 2u;
+    uint64_t excl = 0;
+    if (lane == 0) {
+      uint64_t* st = a.status + chain;
+      if (chain == 0) {
+        __hip_atomic_store(st, kFlagPrefix | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(st, kFlagAgg | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t j = chain - 1;
+        for (;;) {
+          const uint64_t sv = __hip_atomic_load(a.status + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          const uint64_t f = sv >> 62;
+          if (f == 0) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+          excl += sv & kValueMask;
+          if (f == 2) break;
+          --j;
+        }
+        __hip_atomic_store(st, kFlagPrefix | (excl + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    excl = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(excl >> 32)) << 32) |
+           (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)excl);
+    const uint64_t og = excl + (half ? m_lo : 0u);
+    if (gactive && lin == 0) {
+      a.centers[g] = (uint16_t)c;
+      a.offsets[g] = (int32_t)og;
+    }
+    if ((uint64_t)tk + 2u >= a.gsize && lane == 0) a.offsets[a.gsize] = (int32_t)(excl + agg);
+
+    // ---- bit plane: lane l's m bytes at 32 og + l m (contiguous 32 m) -----
+    __threadfence_block();
+    if (gactive && m > 0) {
+      const uint64_t dst0 = 32ull * og;
+      const uint32_t total = 32u * m;
+      const uint32_t inv = (65536u + m - 1u) / m;
+      const uint64_t valid_lanes = a.n > g * kAdmGroup ? div_up(a.n - g * kAdmGroup, kAdmChunk) : 0;
+      if (dst0 + total <= a.bits_capacity) {
+        for (uint32_t q = lin; q < total; q += 32u) {
+          const uint32_t l = (q * inv) >> 16, j = q - l * m;
+          const uint8_t byte = l < valid_lanes ? wavebytes[(half * 32u + l) * kAdmLaneBytes + j] : (uint8_t)0;
+          a.bits[dst0 + q] = byte;
+        }
+      } else if (lin == 0) {
+        atomicOr(a.err, kErrCapacity);
+      }
+    }
+    __threadfence_block();
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void adm_unmap_kernel(AdmUnmapArgs a) {
+  __shared__ uint8_t s_sig[kThreads][kAdmChunk];
+  const int t = threadIdx.x, lane = t & 63;
+  const uint32_t lin = (uint32_t)lane & 31u;
+  uint8_t* sig = s_sig[t];
+  const uint64_t nlanes = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t li = (uint64_t)blockIdx.x * kThreads + t; li < a.gsize * kAdmLanes; li += nlanes) {
+    const uint64_t g = li / kAdmLanes;
+    const uint64_t base = li * kAdmChunk;
+    if (base >= a.n) continue;
+    const uint32_t cnt = (uint32_t)((a.n - base) < kAdmChunk ? (a.n - base) : kAdmChunk);
+    const int32_t o0 = a.offsets[g], o1 = a.offsets[g + 1];
+    if (o0 < 0 || o1 < o0 || o1 - o0 > (int32_t)kAdmLaneBytes || 32ull * (uint64_t)o1 > a.bits_len) {
+      if (lin == 0) atomicOr(a.err, kErrBlockWords);
+      continue;
+    }
+    const uint32_t m = (uint32_t)(o1 - o0);
+    const uint8_t* src = a.bits + 32ull * (uint64_t)o0 + (uint64_t)lin * m;
+    uint32_t w[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const uint32_t i = 4u * r + b;
+        v = (v << 8) | (i < m ? (uint32_t)src[i] : 0u);
+      }
+      w[r] = v;  // bit position 32r + j is bit (31 - j) of w[r]
+    }
+#pragma unroll
+    for (int i = 0; i < (int)kAdmChunk; ++i) sig[i] = 0;
+    // mappingcpuV1.cpp:233-243: a one advances the element index (stop at
+    // 16), zeros after the i-th one count into signal[i] (uint8)
+    int idx = -1;
+    uint32_t prev = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      uint32_t v = w[r];
+      while (v != 0 && idx < (int)kAdmChunk) {
+        const uint32_t z = (uint32_t)__clz((int)v);
+        const uint32_t pos = 32u * r + z;
+        if (idx >= 0) sig[idx] = (uint8_t)(pos - prev - 1u);
+        ++idx;
+        prev = pos;
+        v &= ~(0x80000000u >> z);
+      }
+    }
+    if (idx >= 0 && idx < (int)kAdmChunk) sig[idx] = (uint8_t)(8u * m - prev - 1u);
+    const uint32_t c = a.centers[g];
+    uint32_t out[8];
+    const uint8_t* cp = a.codes + base;
+    uint32_t cw[4];
+    if (cnt == kAdmChunk) {
+      const uint4 q = *(const uint4*)cp;
+      cw[0] = q.x; cw[1] = q.y; cw[2] = q.z; cw[3] = q.w;
+    } else {
+      for (int i = 0; i < 4; ++i) cw[i] = 0;
+      for (uint32_t i = 0; i < cnt; ++i) cw[i >> 2] |= (uint32_t)cp[i] << (8 * (i & 3));
+    }
+#pragma unroll
+    for (int i = 0; i < (int)kAdmChunk; ++i) {
+      const uint32_t code = (cw[i >> 2] >> (8 * (i & 3))) & 0xffu;
+      const uint32_t d = (code >> 1) + kAdmMod * (uint32_t)sig[i];
+      const uint32_t x = (code & 1u) ? (c - d) & 0xffffu : (c + d) & 0xffffu;
+      if (i & 1) out[i >> 1] |= x << 16; else out[i >> 1] = x;
+    }
+    if (cnt == kAdmChunk) {
+      uint4* op = (uint4*)(a.out + base);
+      op[0] = make_uint4(out[0], out[1], out[2], out[3]);
+      op[1] = make_uint4(out[4], out[5], out[6], out[7]);
+    } else {
+      for (uint32_t i = 0; i < cnt; ++i) a.out[base + i] = (uint16_t)(out[i >> 1] >> (16 * (i & 1)));
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_adm_map(const AdmMapArgs& a, int cu, hipStream_t s) {
+  if (a.gsize == 0) return hipSuccess;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adm_map_kernel, kThreads, 0);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  uint64_t grid = div_up(div_up(a.gsize, 2), kWaves);
+  const uint64_t cap = (uint64_t)cu * per_cu;
+  if (grid > cap) grid = cap;
+  hipLaunchKernelGGL(adm_map_kernel, dim3((uint32_t)grid), dim3(kThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_adm_unmap(const AdmUnmapArgs& a, hipStream_t s) {
+  if (a.gsize == 0) return hipSuccess;
+  uint64_t grid = div_up(a.gsize * kAdmLanes, kThreads);
+  if (grid > 8192) grid = 8192;
+  hipLaunchKernelGGL(adm_unmap_kernel, dim3((uint32_t)grid), dim3(kThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace mans

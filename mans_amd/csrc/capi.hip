@@ -1,0 +1,735 @@
+// capi.hip -- the C ABI of libmans.so (include/mans.h): argument checks,
+// container layout, workspace management and kernel launches.  Host code
+// only; the kernels live in hist.hip, encode.hip, decode.hip and adm.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/mans.h"
+#include "mans_internal.hpp"
+
+using namespace mans;
+
+struct mans_workspace {
+  int device = 0;
+  int cu = 0;
+  uint32_t* hist = nullptr;      // [4][256] self-cleaning accumulators
+  uint32_t* counters = nullptr;  // 0 done, 1 encode ticket, 2 adm ticket, 3 err, 4.. scratch scalars
+  uint4* table = nullptr;        // [4][256]
+  uint32_t* hist_out = nullptr;  // [4][256]
+  uint64_t* status = nullptr;
+  uint64_t status_cap = 0;
+  uint8_t* scratch = nullptr;    // ADM planes of frames
+  uint64_t scratch_cap = 0;
+  uint64_t* d_u64 = nullptr;     // [4]
+  uint64_t* h_pinned = nullptr;  // [32] host pinned
+  bool profiling = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<const char*> ev_names;
+  int ev_used = 0;
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+#define HIP_TRY(x)                        \
+  do {                                    \
+    if ((x) != hipSuccess) return MANS_ERR_HIP; \
+  } while (0)
+
+bool aligned(const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; }
+
+mans_status check_params(const mans_params* p) {
+  if (!p) return MANS_ERR_INVALID_ARG;
+  if (p->prob_bits < 9 || p->prob_bits > 11) return MANS_ERR_PRECISION;
+  if (p->lanes != 32 && p->lanes != 64) return MANS_ERR_INVALID_ARG;
+  return MANS_OK;
+}
+
+mans_status ensure_status(mans_workspace* ws, uint64_t n) {
+  if (n <= ws->status_cap) return MANS_OK;
+  uint64_t cap = std::max<uint64_t>(n, ws->status_cap * 2);
+  if (ws->status) (void)hipFree(ws->status);
+  ws->status = nullptr;
+  ws->status_cap = 0;
+  HIP_TRY(hipMalloc(&ws->status, cap * sizeof(uint64_t)));
+  HIP_TRY(hipMemset(ws->status, 0, cap * sizeof(uint64_t)));
+  ws->status_cap = cap;
+  return MANS_OK;
+}
+
+mans_status ensure_scratch(mans_workspace* ws, uint64_t n) {
+  if (n <= ws->scratch_cap) return MANS_OK;
+  if (ws->scratch) (void)hipFree(ws->scratch);
+  ws->scratch = nullptr;
+  ws->scratch_cap = 0;
+  HIP_TRY(hipMalloc(&ws->scratch, n));
+  ws->scratch_cap = n;
+  return MANS_OK;
+}
+
+// Profiling: events recorded around launches when enabled.
+void prof_begin(mans_workspace* ws) { ws->ev_used = 0; ws->ev_names.clear(); }
+void prof_mark(mans_workspace* ws, hipStream_t s, const char* name) {
+  if (!ws->profiling) return;
+  if ((size_t)ws->ev_used >= ws->ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    ws->ev.push_back(e);
+  }
+  (void)hipEventRecord(ws->ev[ws->ev_used++], s);
+  ws->ev_names.push_back(name);
+}
+
+uint64_t chain_len(uint64_t units, int lanes) { return div_up(units, lanes == 32 ? 2 : 1); }
+
+uint64_t adm_bits_cap(uint64_t n) { return div_up(n, kAdmGroup) * kAdmLanes * kAdmLaneBytes; }
+uint64_t adm_meta_bytes(uint64_t gsize) { return round_up(4ull * (gsize + 1) + 2ull * gsize, 16); }
+
+uint64_t planes_frame_max(uint64_t n, uint32_t eb, int pb, Format f) {
+  const uint64_t nb = div_up(n, f.block_bytes);
+  const uint64_t cap = round_up(unit_max_words(f.lanes, f.block_bytes, pb), 8);
+  return kFrameHeaderBytes + eb * (section_bytes(nb, f) + 2ull * nb * cap);
+}
+uint64_t adm_frame_max(uint64_t n, int pb, Format f) {
+  const uint64_t g = div_up(n, kAdmGroup);
+  const uint64_t nb0 = div_up(n, f.block_bytes), nb1 = div_up(adm_bits_cap(n), f.block_bytes);
+  const uint64_t cap = round_up(unit_max_words(f.lanes, f.block_bytes, pb), 8);
+  return kFrameHeaderBytes + adm_meta_bytes(g) + section_bytes(nb0, f) + section_bytes(nb1, f) +
+         2ull * (nb0 + nb1) * cap;
+}
+
+void put64(uint32_t* w, int byte_off, uint64_t v) {
+  w[byte_off / 4] = (uint32_t)v;
+  w[byte_off / 4 + 1] = (uint32_t)(v >> 32);
+}
+
+// Shared by mans_encode and mans_frame_encode: one hist+normalise launch and
+// one encode launch over the planes of `d`.
+mans_status run_encode(mans_workspace* ws, const CodecDesc& d, const HistSource* src, int nsrc,
+                       uint8_t* out, uint32_t* d_out_size32, uint64_t* d_out_size64,
+                       const uint32_t* frame_hdr, hipStream_t s) {
+  const Format f = format_for_lanes(d.lanes);
+  const uint64_t chain = chain_len(d.total_units, d.lanes);
+  mans_status st = ensure_status(ws, std::max<uint64_t>(chain, 1));
+  if (st != MANS_OK) return st;
+  HistArgs h;
+  memset(&h, 0, sizeof(h));
+  for (int i = 0; i < nsrc; ++i) h.src[i] = src[i];
+  h.nsrc = nsrc;
+  h.num_planes = d.num_planes;
+  h.prob_bits = d.prob_bits;
+  h.write_headers = 1;
+  for (int k = 0; k < d.num_planes; ++k) {
+    h.plane_nsym[k] = d.plane[k].nsym;
+    h.plane_nb[k] = d.plane[k].nb;
+    h.plane_section[k] = d.plane[k].section;
+  }
+  h.hist = ws->hist;
+  h.done = ws->counters + 0;
+  h.table = ws->table;
+  h.out = out;
+  h.version = f.version;
+  h.status = ws->status;
+  h.status_count = chain;
+  h.ticket = ws->counters + 1;
+  h.d_out_size32 = d_out_size32;
+  h.d_out_size64 = d_out_size64;
+  h.empty_size = d.data_off;
+  h.total_units = d.total_units;
+  if (frame_hdr) {
+    h.has_frame = 1;
+    memcpy(h.frame_hdr, frame_hdr, sizeof(h.frame_hdr));
+    if (d.total_units == 0) put64(h.frame_hdr, 56, d.data_off);
+  }
+  prof_mark(ws, s, "hist_normalize");
+  HIP_TRY(launch_hist(h, ws->cu, s));
+  EncodeArgs e;
+  memset(&e, 0, sizeof(e));
+  e.d = d;
+  e.table = ws->table;
+  e.out = out;
+  e.status = ws->status;
+  e.ticket = ws->counters + 1;
+  e.err = ws->counters + 3;
+  e.d_out_size32 = d_out_size32;
+  e.d_out_size64 = d_out_size64;
+  e.frame = frame_hdr ? 1 : 0;
+  prof_mark(ws, s, "encode");
+  HIP_TRY(launch_encode(e, ws->cu, s));
+  prof_mark(ws, s, nullptr);
+  return MANS_OK;
+}
+
+mans_status read_u64(mans_workspace* ws, const uint64_t* d, uint64_t* h, hipStream_t s) {
+  HIP_TRY(hipMemcpyAsync(ws->h_pinned, d, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *h = ws->h_pinned[0];
+  return MANS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mans_status_string(mans_status s) {
+  switch (s) {
+    case MANS_OK: return "ok";
+    case MANS_ERR_INVALID_ARG: return "invalid argument";
+    case MANS_ERR_PRECISION: return "unsupported probability precision (use 9, 10 or 11)";
+    case MANS_ERR_ALIGNMENT: return "pointer not aligned as required";
+    case MANS_ERR_TOO_LARGE: return "size exceeds the format's 32-bit fields";
+    case MANS_ERR_CAPACITY: return "output buffer too small";
+    case MANS_ERR_CORRUPT: return "malformed or inconsistent compressed stream";
+    case MANS_ERR_HIP: return "HIP runtime error";
+    case MANS_ERR_NO_DEVICE: return "no usable GPU";
+    case MANS_ERR_DOMAIN: return "data outside the ADM mapping's lossless domain";
+    case MANS_ERR_BUDGET: return "request exceeds the device-memory budget";
+    case MANS_ERR_BUSY: return "queue full";
+  }
+  return "unknown status";
+}
+
+int32_t mans_abi_version(void) { return MANS_ABI_VERSION; }
+
+mans_status mans_workspace_create(mans_workspace** out, int32_t device) {
+  if (!out) return MANS_ERR_INVALID_ARG;
+  *out = nullptr;
+  int dev = device;
+  if (dev < 0 && hipGetDevice(&dev) != hipSuccess) return MANS_ERR_NO_DEVICE;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || dev >= count) return MANS_ERR_NO_DEVICE;
+  DeviceGuard g(dev);
+  mans_workspace* ws = new (std::nothrow) mans_workspace();
+  if (!ws) return MANS_ERR_INVALID_ARG;
+  ws->device = dev;
+  if (hipDeviceGetAttribute(&ws->cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    delete ws;
+    return MANS_ERR_NO_DEVICE;
+  }
+  bool ok = hipMalloc(&ws->hist, 4 * 256 * 4) == hipSuccess && hipMalloc(&ws->counters, 64 * 4) == hipSuccess &&
+            hipMalloc(&ws->table, 4 * 256 * 16) == hipSuccess && hipMalloc(&ws->hist_out, 4 * 256 * 4) == hipSuccess &&
+            hipMalloc(&ws->d_u64, 4 * 8) == hipSuccess &&
+            hipHostMalloc((void**)&ws->h_pinned, 32 * 8, hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipMemset(ws->hist, 0, 4 * 256 * 4) == hipSuccess && hipMemset(ws->counters, 0, 64 * 4) == hipSuccess &&
+       hipMemset(ws->table, 0, 4 * 256 * 16) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+  if (!ok) {
+    mans_workspace_destroy(ws);
+    return MANS_ERR_HIP;
+  }
+  *out = ws;
+  return MANS_OK;
+}
+
+mans_status mans_workspace_destroy(mans_workspace* ws) {
+  if (!ws) return MANS_OK;
+  DeviceGuard g(ws->device);
+  (void)hipDeviceSynchronize();
+  if (ws->hist) (void)hipFree(ws->hist);
+  if (ws->counters) (void)hipFree(ws->counters);
+  if (ws->table) (void)hipFree(ws->table);
+  if (ws->hist_out) (void)hipFree(ws->hist_out);
+  if (ws->status) (void)hipFree(ws->status);
+  if (ws->scratch) (void)hipFree(ws->scratch);
+  if (ws->d_u64) (void)hipFree(ws->d_u64);
+  if (ws->h_pinned) (void)hipHostFree(ws->h_pinned);
+  for (hipEvent_t e : ws->ev) (void)hipEventDestroy(e);
+  delete ws;
+  return MANS_OK;
+}
+
+mans_status mans_workspace_reserve(mans_workspace* ws, uint64_t max_bytes, const mans_params* p) {
+  if (!ws) return MANS_ERR_INVALID_ARG;
+  DeviceGuard g(ws->device);
+  const int lanes = (p && p->lanes == 64) ? 64 : 32;
+  const Format f = format_for_lanes(lanes);
+  const uint64_t units = 4 * div_up(max_bytes, f.block_bytes) + 8;
+  mans_status st = ensure_status(ws, std::max(chain_len(units, 32), div_up(max_bytes, kAdmGroup) + 8));
+  if (st != MANS_OK) return st;
+  if (p && p->mode != MANS_MODE_PLANES && p->elem_bytes == 2) {
+    const uint64_t n = max_bytes / 2;
+    st = ensure_scratch(ws, round_up(n, 16) + adm_bits_cap(n) + 16);
+  }
+  return st;
+}
+
+mans_status mans_workspace_check(mans_workspace* ws, void* stream, uint32_t* h_flags) {
+  if (!ws) return MANS_ERR_INVALID_ARG;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t* hf = (uint32_t*)ws->h_pinned;
+  HIP_TRY(hipMemcpyAsync(hf, ws->counters + 3, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  uint32_t flags = hf[0];
+  if (h_flags) *h_flags = flags;
+  if (flags) {
+    HIP_TRY(hipMemsetAsync(ws->counters + 3, 0, 4, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return MANS_ERR_CORRUPT;
+  }
+  return MANS_OK;
+}
+
+uint64_t mans_max_compressed_size(uint64_t n, const mans_params* p) {
+  const int lanes = (p && p->lanes == 64) ? 64 : 32;
+  const int pb = (p && p->prob_bits >= 9 && p->prob_bits <= 11) ? p->prob_bits : 11;
+  const Format f = format_for_lanes(lanes);
+  const uint64_t nb = div_up(n, f.block_bytes);
+  return section_bytes(nb, f) + 2ull * nb * round_up(unit_max_words(f.lanes, f.block_bytes, pb), 8);
+}
+
+mans_status mans_encode(const mans_params* p, const void* d_in, uint64_t n, void* d_out, uint64_t out_capacity,
+                        uint32_t* d_out_size, uint64_t* h_out_size, mans_workspace* ws, void* stream) {
+  mans_status st = check_params(p);
+  if (st != MANS_OK) return st;
+  if (!ws || !d_out || (n > 0 && !d_in)) return MANS_ERR_INVALID_ARG;
+  if (!aligned(d_out, 16)) return MANS_ERR_ALIGNMENT;
+  if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
+  if (out_capacity < mans_max_compressed_size(n, p)) return MANS_ERR_CAPACITY;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  prof_begin(ws);
+  const Format f = format_for_lanes(p->lanes);
+  CodecDesc d;
+  memset(&d, 0, sizeof(d));
+  d.num_planes = 1;
+  d.order = 0;
+  d.prob_bits = p->prob_bits;
+  d.lanes = p->lanes;
+  d.plane[0].src = (const uint8_t*)d_in;
+  d.plane[0].nsym = n;
+  d.plane[0].nb = div_up(n, f.block_bytes);
+  d.plane[0].section = 0;
+  d.plane[0].unit_base = 0;
+  d.plane[0].stride = 1;
+  d.total_units = d.plane[0].nb;
+  d.data_off = section_bytes(d.plane[0].nb, f);
+  HistSource src = {(const uint8_t*)d_in, n, 1u, 0u};
+  st = run_encode(ws, d, &src, n ? 1 : 0, (uint8_t*)d_out, d_out_size, h_out_size ? ws->d_u64 : nullptr, nullptr, s);
+  if (st != MANS_OK) return st;
+  if (h_out_size) return read_u64(ws, ws->d_u64, h_out_size, s);
+  return MANS_OK;
+}
+
+mans_status mans_decode(const mans_params* p, const void* d_in, uint64_t c, void* d_out, uint64_t out_capacity,
+                        mans_workspace* ws, void* stream) {
+  mans_status st = check_params(p);
+  if (st != MANS_OK) return st;
+  if (!ws || !d_in || !d_out) return MANS_ERR_INVALID_ARG;
+  if (!aligned(d_in, 16)) return MANS_ERR_ALIGNMENT;
+  if (c && c < kSectionFixed) return MANS_ERR_CORRUPT;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  prof_begin(ws);
+  DecodeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.d.num_planes = 1;
+  a.d.order = 0;
+  a.d.prob_bits = p->prob_bits;
+  a.d.lanes = p->lanes;
+  a.d.plane[0].dst = (uint8_t*)d_out;
+  a.d.plane[0].stride = 1;
+  a.in = (const uint8_t*)d_in;
+  a.in_bytes = c;
+  a.out_capacity = out_capacity;
+  a.err = ws->counters + 3;
+  a.from_header = 1;
+  prof_mark(ws, s, "decode");
+  HIP_TRY(launch_decode(a, ws->cu, s));
+  prof_mark(ws, s, nullptr);
+  return MANS_OK;
+}
+
+mans_status mans_histogram(const void* d_in, uint64_t n, uint32_t stride, uint32_t* d_hist, mans_workspace* ws,
+                           void* stream) {
+  if (!ws || !d_hist || (n > 0 && !d_in)) return MANS_ERR_INVALID_ARG;
+  if (stride != 1 && stride != 2 && stride != 4) return MANS_ERR_INVALID_ARG;
+  if (!aligned(d_in, stride)) return MANS_ERR_ALIGNMENT;
+  if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  prof_begin(ws);
+  HistArgs h;
+  memset(&h, 0, sizeof(h));
+  h.src[0] = HistSource{(const uint8_t*)d_in, n * stride, stride, 0u};
+  h.nsrc = n ? 1 : 0;
+  h.num_planes = (int)stride;
+  h.prob_bits = 10;
+  for (uint32_t k = 0; k < stride; ++k) h.plane_nsym[k] = n;
+  h.hist = ws->hist;
+  h.done = ws->counters + 0;
+  h.table = ws->table;
+  h.hist_out = ws->hist_out;
+  h.ticket = ws->counters + 1;
+  prof_mark(ws, s, "hist_normalize");
+  HIP_TRY(launch_hist(h, ws->cu, s));
+  prof_mark(ws, s, nullptr);
+  HIP_TRY(hipMemcpyAsync(d_hist, ws->hist_out, 256 * 4, hipMemcpyDeviceToDevice, s));
+  return MANS_OK;
+}
+
+mans_status mans_normalize(const uint32_t* d_hist, uint32_t total, int32_t prob_bits, uint32_t* d_table,
+                           mans_workspace* ws, void* stream) {
+  if (!ws || !d_hist || !d_table) return MANS_ERR_INVALID_ARG;
+  if (prob_bits < 9 || prob_bits > 11) return MANS_ERR_PRECISION;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  prof_begin(ws);
+  HIP_TRY(hipMemcpyAsync(ws->hist, d_hist, 256 * 4, hipMemcpyDeviceToDevice, s));
+  HistArgs h;
+  memset(&h, 0, sizeof(h));
+  h.nsrc = 0;
+  h.num_planes = 1;
+  h.prob_bits = prob_bits;
+  h.plane_nsym[0] = total;
+  h.hist = ws->hist;
+  h.done = ws->counters + 0;
+  h.table = ws->table;
+  h.ticket = ws->counters + 1;
+  prof_mark(ws, s, "normalize");
+  HIP_TRY(launch_hist(h, ws->cu, s));
+  prof_mark(ws, s, nullptr);
+  HIP_TRY(hipMemcpyAsync(d_table, ws->table, 256 * 16, hipMemcpyDeviceToDevice, s));
+  return MANS_OK;
+}
+
+static mans_status adm_map_impl(const uint16_t* d_in, uint64_t n, int32_t* d_offsets, uint16_t* d_centers,
+                                uint8_t* d_codes, uint8_t* d_bits, uint64_t bits_capacity, uint64_t* h_bits_len,
+                                int32_t* h_in_domain, mans_workspace* ws, hipStream_t s) {
+  const uint64_t gsize = div_up(n, kAdmGroup);
+  const uint64_t chain = div_up(gsize, 2);
+  mans_status st = ensure_status(ws, std::max<uint64_t>(chain, 1));
+  if (st != MANS_OK) return st;
+  HIP_TRY(launch_fill_u64(ws->status, chain, 0, s));
+  // counters[2] = ticket, counters[4] = this call's domain / capacity flags
+  // (kept apart from the sticky stream-error word counters[3])
+  HIP_TRY(launch_fill_u32(ws->counters + 2, 1, 0, s));
+  HIP_TRY(launch_fill_u32(ws->counters + 4, 1, 0, s));
+  if (gsize == 0) HIP_TRY(launch_fill_u32((uint32_t*)d_offsets, 1, 0, s));
+  AdmMapArgs a;
+  memset(&a, 0, sizeof(a));
+  a.in = d_in;
+  a.n = n;
+  a.gsize = gsize;
+  a.offsets = d_offsets;
+  a.centers = d_centers;
+  a.codes = d_codes;
+  a.bits = d_bits;
+  a.bits_capacity = bits_capacity;
+  a.status = ws->status;
+  a.ticket = ws->counters + 2;
+  a.err = ws->counters + 4;
+  prof_mark(ws, s, "adm_map");
+  HIP_TRY(launch_adm_map(a, ws->cu, s));
+  if (h_bits_len || h_in_domain) {
+    uint32_t* hp = (uint32_t*)ws->h_pinned;
+    HIP_TRY(hipMemcpyAsync(hp, d_offsets + gsize, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hp + 1, ws->counters + 4, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (h_bits_len) *h_bits_len = 32ull * hp[0];
+    if (h_in_domain) *h_in_domain = (hp[1] & kErrAdmDomain) ? 0 : 1;
+    if (hp[1] & kErrCapacity) return MANS_ERR_CAPACITY;
+  }
+  return MANS_OK;
+}
+
+mans_status mans_adm_map(const uint16_t* d_in, uint64_t n, int32_t* d_offsets, uint16_t* d_centers, uint8_t* d_codes,
+                         uint8_t* d_bits, uint64_t bits_capacity, uint64_t* h_bits_len, int32_t* h_in_domain,
+                         mans_workspace* ws, void* stream) {
+  if (!ws || !d_offsets || (n > 0 && (!d_in || !d_centers || !d_codes || !d_bits))) return MANS_ERR_INVALID_ARG;
+  if (!aligned(d_in, 16) || !aligned(d_codes, 16)) return MANS_ERR_ALIGNMENT;
+  if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
+  DeviceGuard g(ws->device);
+  prof_begin(ws);
+  mans_status st = adm_map_impl(d_in, n, d_offsets, d_centers, d_codes, d_bits, bits_capacity, h_bits_len,
+                                h_in_domain, ws, (hipStream_t)stream);
+  prof_mark(ws, (hipStream_t)stream, nullptr);
+  return st;
+}
+
+mans_status mans_adm_unmap(const int32_t* d_offsets, const uint16_t* d_centers, const uint8_t* d_codes,
+                           const uint8_t* d_bits, uint64_t n, uint16_t* d_out, mans_workspace* ws, void* stream) {
+  if (!ws || (n > 0 && (!d_offsets || !d_centers || !d_codes || !d_bits || !d_out))) return MANS_ERR_INVALID_ARG;
+  if (!aligned(d_out, 16) || !aligned(d_codes, 16)) return MANS_ERR_ALIGNMENT;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  prof_begin(ws);
+  AdmUnmapArgs a;
+  memset(&a, 0, sizeof(a));
+  a.offsets = d_offsets;
+  a.centers = d_centers;
+  a.codes = d_codes;
+  a.bits = d_bits;
+  a.n = n;
+  a.gsize = div_up(n, kAdmGroup);
+  a.bits_len = ~0ull;
+  a.out = d_out;
+  a.err = ws->counters + 3;
+  prof_mark(ws, s, "adm_unmap");
+  HIP_TRY(launch_adm_unmap(a, s));
+  prof_mark(ws, s, nullptr);
+  return MANS_OK;
+}
+
+uint64_t mans_frame_max_size(uint64_t n, const mans_params* p) {
+  if (!p) return 0;
+  const int pb = (p->prob_bits >= 9 && p->prob_bits <= 11) ? p->prob_bits : 11;
+  const Format f = format_for_lanes(p->lanes == 64 ? 64 : 32);
+  const uint32_t eb = (uint32_t)(p->elem_bytes == 1 || p->elem_bytes == 2 || p->elem_bytes == 4 ? p->elem_bytes : 4);
+  uint64_t m = planes_frame_max(n, eb, pb, f);
+  if (eb == 2 && p->mode != MANS_MODE_PLANES) m = std::max(m, adm_frame_max(n, pb, f));
+  return m;
+}
+
+mans_status mans_frame_encode(const mans_params* p, const void* d_in, uint64_t n, void* d_out, uint64_t out_capacity,
+                              uint64_t* d_size, uint64_t* h_size, mans_workspace* ws, void* stream) {
+  mans_status st = check_params(p);
+  if (st != MANS_OK) return st;
+  if (!ws || !d_out || (n > 0 && !d_in)) return MANS_ERR_INVALID_ARG;
+  const uint32_t eb = (uint32_t)p->elem_bytes;
+  if (eb != 1 && eb != 2 && eb != 4) return MANS_ERR_INVALID_ARG;
+  if (p->mode < MANS_MODE_PLANES || p->mode > MANS_MODE_AUTO) return MANS_ERR_INVALID_ARG;
+  if (p->mode == MANS_MODE_ADM && eb != 2) return MANS_ERR_INVALID_ARG;
+  if (!aligned(d_out, 16) || !aligned(d_in, eb)) return MANS_ERR_ALIGNMENT;
+  if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
+  if (out_capacity < mans_frame_max_size(n, p)) return MANS_ERR_CAPACITY;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  prof_begin(ws);
+  const Format f = format_for_lanes(p->lanes);
+  uint8_t* out = (uint8_t*)d_out;
+  int mode = p->mode == MANS_MODE_AUTO ? (eb == 2 && aligned(d_in, 16) ? MANS_MODE_ADM : MANS_MODE_PLANES) : p->mode;
+  if (mode == MANS_MODE_ADM && !aligned(d_in, 16)) return MANS_ERR_ALIGNMENT;
+
+  CodecDesc d;
+  memset(&d, 0, sizeof(d));
+  d.prob_bits = p->prob_bits;
+  d.lanes = p->lanes;
+  HistSource src[2];
+  int nsrc = 0;
+  uint64_t gsize = 0, meta_off = 0, off = kFrameHeaderBytes;
+  if (mode == MANS_MODE_ADM) {
+    gsize = div_up(n, kAdmGroup);
+    meta_off = off;
+    off += adm_meta_bytes(gsize);
+    const uint64_t codes_bytes = round_up(n, 16);
+    st = ensure_scratch(ws, codes_bytes + adm_bits_cap(n) + 16);
+    if (st != MANS_OK) return st;
+    uint8_t* codes = ws->scratch;
+    uint8_t* bits = ws->scratch + codes_bytes;
+    int32_t* offs = (int32_t*)(out + meta_off);
+    uint16_t* cen = (uint16_t*)(out + meta_off + 4ull * (gsize + 1));
+    uint64_t bits_len = 0;
+    int32_t in_domain = 0;
+    st = adm_map_impl((const uint16_t*)d_in, n, offs, cen, codes, bits, adm_bits_cap(n), &bits_len, &in_domain, ws, s);
+    if (st != MANS_OK) return st;
+    if (!in_domain) {
+      if (p->mode == MANS_MODE_ADM) return MANS_ERR_DOMAIN;
+      mode = MANS_MODE_PLANES;
+      off = kFrameHeaderBytes;
+      gsize = 0;
+      meta_off = 0;
+    } else {
+      // zero the meta padding (the map kernel writes offsets and centers)
+      const uint64_t used = 4ull * (gsize + 1) + 2ull * gsize;
+      const uint64_t pad = adm_meta_bytes(gsize) - used;
+      if (pad) HIP_TRY(hipMemsetAsync(out + meta_off + used, 0, pad, s));
+      d.num_planes = 2;
+      d.order = 1;
+      d.plane[0] = PlaneDesc{codes, nullptr, n, div_up(n, f.block_bytes), 0, 0, 1, 0};
+      d.plane[1] = PlaneDesc{bits, nullptr, bits_len, div_up(bits_len, f.block_bytes), 0, d.plane[0].nb, 1, 0};
+      src[0] = HistSource{codes, n, 1u, 0u};
+      src[1] = HistSource{bits, bits_len, 1u, 1u};
+      nsrc = 2;
+    }
+  }
+  if (mode == MANS_MODE_PLANES) {
+    d.num_planes = (int)eb;
+    d.order = 0;
+    for (uint32_t k = 0; k < eb; ++k)
+      d.plane[k] = PlaneDesc{(const uint8_t*)d_in + k, nullptr, n, div_up(n, f.block_bytes), 0, 0, eb, 0};
+    src[0] = HistSource{(const uint8_t*)d_in, n * eb, eb, 0u};
+    nsrc = n ? 1 : 0;
+  }
+  d.total_units = 0;
+  for (int k = 0; k < d.num_planes; ++k) {
+    d.plane[k].section = off;
+    off += section_bytes(d.plane[k].nb, f);
+    d.total_units += d.plane[k].nb;
+  }
+  d.data_off = off;
+  uint32_t hdr[32];
+  memset(hdr, 0, sizeof(hdr));
+  uint8_t* hb = (uint8_t*)hdr;
+  hdr[0] = kFrameMagic;
+  hb[4] = (uint8_t)kFrameVersion;
+  hb[5] = 0;
+  hb[6] = (uint8_t)eb;
+  hb[7] = (uint8_t)mode;
+  hb[8] = (uint8_t)f.lanes;
+  hb[9] = (uint8_t)p->prob_bits;
+  hb[10] = (uint8_t)d.num_planes;
+  hb[11] = (uint8_t)d.order;
+  hdr[3] = f.block_bytes;
+  put64(hdr, 16, n);
+  put64(hdr, 24, gsize);
+  put64(hdr, 32, meta_off);
+  put64(hdr, 40, d.data_off);
+  for (int k = 0; k < d.num_planes; ++k) {
+    put64(hdr, 64 + 8 * k, d.plane[k].section);
+    put64(hdr, 96 + 8 * k, d.plane[k].nsym);
+  }
+  uint64_t* dsz = d_size ? d_size : (h_size ? ws->d_u64 : nullptr);
+  st = run_encode(ws, d, src, nsrc, out, nullptr, dsz, hdr, s);
+  if (st != MANS_OK) return st;
+  if (h_size) return read_u64(ws, dsz, h_size, s);
+  return MANS_OK;
+}
+
+mans_status mans_frame_info(const void* h_header, uint64_t header_bytes, uint64_t* n_elems, int32_t* elem_bytes,
+                            int32_t* mode, uint64_t* total_size) {
+  if (!h_header || header_bytes < kFrameHeaderBytes) return MANS_ERR_INVALID_ARG;
+  const uint8_t* h = (const uint8_t*)h_header;
+  uint32_t magic;
+  uint64_t n, total;
+  memcpy(&magic, h, 4);
+  memcpy(&n, h + 16, 8);
+  memcpy(&total, h + 56, 8);
+  if (magic != kFrameMagic || h[4] != kFrameVersion || h[5] != 0) return MANS_ERR_CORRUPT;
+  if (n_elems) *n_elems = n;
+  if (elem_bytes) *elem_bytes = h[6];
+  if (mode) *mode = h[7];
+  if (total_size) *total_size = total;
+  return MANS_OK;
+}
+
+mans_status mans_frame_decode(const void* d_in, uint64_t c, void* d_out, uint64_t out_capacity, uint64_t* h_n_elems,
+                              mans_workspace* ws, void* stream) {
+  if (!ws || !d_in || c < kFrameHeaderBytes) return MANS_ERR_INVALID_ARG;
+  if (!aligned(d_in, 16)) return MANS_ERR_ALIGNMENT;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  prof_begin(ws);
+  uint8_t* hb = (uint8_t*)ws->h_pinned;
+  HIP_TRY(hipMemcpyAsync(hb, d_in, kFrameHeaderBytes, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  uint32_t magic, block_bytes;
+  uint64_t n, gsize, meta, data_off, data_words, total, sect[4], nsym[4];
+  memcpy(&magic, hb, 4);
+  memcpy(&block_bytes, hb + 12, 4);
+  memcpy(&n, hb + 16, 8);
+  memcpy(&gsize, hb + 24, 8);
+  memcpy(&meta, hb + 32, 8);
+  memcpy(&data_off, hb + 40, 8);
+  memcpy(&data_words, hb + 48, 8);
+  memcpy(&total, hb + 56, 8);
+  memcpy(sect, hb + 64, 32);
+  memcpy(nsym, hb + 96, 32);
+  const uint32_t eb = hb[6], mode = hb[7], lanes = hb[8], pb = hb[9], P = hb[10], order = hb[11];
+  if (magic != kFrameMagic || hb[4] != kFrameVersion || hb[5] != 0) return MANS_ERR_CORRUPT;
+  if ((lanes != 32 && lanes != 64) || pb < 9 || pb > 11 || P < 1 || P > 4 || order > 1) return MANS_ERR_CORRUPT;
+  const Format f = format_for_lanes((int)lanes);
+  if (block_bytes != f.block_bytes || (eb != 1 && eb != 2 && eb != 4)) return MANS_ERR_CORRUPT;
+  if (total > c || data_off + 2ull * data_words > c || (data_off & 15)) return MANS_ERR_CORRUPT;
+  if (mode == MANS_MODE_PLANES && (P != eb || order != 0)) return MANS_ERR_CORRUPT;
+  if (mode == MANS_MODE_ADM && (eb != 2 || P != 2 || order != 1 || gsize != div_up(n, kAdmGroup))) return MANS_ERR_CORRUPT;
+  if (mode > MANS_MODE_ADM) return MANS_ERR_CORRUPT;
+  if (h_n_elems) *h_n_elems = n;
+  if (n * eb > out_capacity) return MANS_ERR_CAPACITY;
+  if (n > 0 && !d_out) return MANS_ERR_INVALID_ARG;
+  if (mode == MANS_MODE_ADM && !aligned(d_out, 16)) return MANS_ERR_ALIGNMENT;
+
+  DecodeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.d.num_planes = (int)P;
+  a.d.order = (int)order;
+  a.d.prob_bits = (int)pb;
+  a.d.lanes = (int)lanes;
+  uint64_t units = 0;
+  uint8_t* codes = nullptr;
+  uint8_t* bits = nullptr;
+  if (mode == MANS_MODE_ADM) {
+    if (nsym[0] != n || nsym[1] > adm_bits_cap(n)) return MANS_ERR_CORRUPT;
+    const uint64_t codes_bytes = round_up(n, 16);
+    mans_status st = ensure_scratch(ws, codes_bytes + round_up(nsym[1], 16) + 16);
+    if (st != MANS_OK) return st;
+    codes = ws->scratch;
+    bits = ws->scratch + codes_bytes;
+  }
+  for (uint32_t k = 0; k < P; ++k) {
+    if (sect[k] + kSectionFixed > c || nsym[k] > 0xffffffffull) return MANS_ERR_CORRUPT;
+    uint8_t* dst = mode == MANS_MODE_ADM ? (k == 0 ? codes : bits) : (uint8_t*)d_out + k;
+    const uint32_t stride = mode == MANS_MODE_ADM ? 1u : eb;
+    if (mode == MANS_MODE_PLANES && nsym[k] != n) return MANS_ERR_CORRUPT;
+    const uint64_t nb = div_up(nsym[k], f.block_bytes);
+    if (sect[k] + section_bytes(nb, f) > data_off) return MANS_ERR_CORRUPT;
+    a.d.plane[k] = PlaneDesc{nullptr, dst, nsym[k], nb, sect[k], units, stride, 0};
+    units += nb;
+  }
+  a.d.total_units = units;
+  a.d.data_off = data_off;
+  a.in = (const uint8_t*)d_in;
+  a.in_bytes = c;
+  a.data_words = data_words;
+  a.err = ws->counters + 3;
+  a.from_header = 0;
+  prof_mark(ws, s, "decode");
+  HIP_TRY(launch_decode(a, ws->cu, s));
+  if (mode == MANS_MODE_ADM) {
+    if (meta + adm_meta_bytes(gsize) > c) return MANS_ERR_CORRUPT;
+    AdmUnmapArgs u;
+    memset(&u, 0, sizeof(u));
+    u.offsets = (const int32_t*)((const uint8_t*)d_in + meta);
+    u.centers = (const uint16_t*)((const uint8_t*)d_in + meta + 4ull * (gsize + 1));
+    u.codes = codes;
+    u.bits = bits;
+    u.n = n;
+    u.gsize = gsize;
+    u.bits_len = nsym[1];
+    u.out = (uint16_t*)d_out;
+    u.err = ws->counters + 3;
+    prof_mark(ws, s, "adm_unmap");
+    HIP_TRY(launch_adm_unmap(u, s));
+  }
+  prof_mark(ws, s, nullptr);
+  return MANS_OK;
+}
+
+mans_status mans_set_profiling(mans_workspace* ws, int32_t enable) {
+  if (!ws) return MANS_ERR_INVALID_ARG;
+  ws->profiling = enable != 0;
+  return MANS_OK;
+}
+
+int32_t mans_last_kernel_times(mans_workspace* ws, double* h_ms, const char** h_names, int32_t max) {
+  if (!ws || !ws->profiling || ws->ev_used < 2) return 0;
+  DeviceGuard g(ws->device);
+  int32_t k = 0;
+  for (int i = 0; i + 1 < ws->ev_used && k < max; ++i) {
+    if (!ws->ev_names[i]) continue;
+    float ms = 0;
+    if (hipEventSynchronize(ws->ev[i + 1]) != hipSuccess) return k;
+    if (hipEventElapsedTime(&ms, ws->ev[i], ws->ev[i + 1]) != hipSuccess) return k;
+    if (h_ms) h_ms[k] = ms;
+    if (h_names) h_names[k] = ws->ev_names[i];
+    ++k;
+  }
+  return k;
+}
+
+}  // extern "C"

@@ -1,0 +1,208 @@
+// mans_internal.hpp -- constants, container layout and kernel launch
+// descriptors shared by the HIP translation units of libmans.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mans {
+
+// ---- rANS constants (reference include/ans/GpuANSCodec.h:18-61) ----------
+constexpr uint32_t kNumSymbols = 256;
+constexpr uint32_t kStateBits = 31;                 // state < 2^31
+constexpr uint32_t kMinState = 1u << 15;            // start state == lower bound
+constexpr uint32_t kAnsMagic = 0xd00du;
+constexpr uint32_t kAnsHeaderBytes = 32;            // ANSCoalescedHeader
+constexpr uint32_t kProbsBytes = 2 * kNumSymbols;   // u16 pdf[256]
+constexpr uint32_t kSectionFixed = kAnsHeaderBytes + kProbsBytes;  // 544
+
+// ---- frame container (this project's multi-plane wrapper) -----------------
+constexpr uint32_t kFrameMagic = 0x534E414Du;  // "MANS"
+constexpr uint32_t kFrameVersion = 1;
+constexpr uint32_t kFrameHeaderBytes = 128;
+constexpr int kMaxPlanes = 4;
+
+// ---- ADM constants (mapping/cpu/mappingcpuV1.cpp:12-16) --------------------
+constexpr uint32_t kAdmLanes = 32;
+constexpr uint32_t kAdmChunk = 16;
+constexpr uint32_t kAdmGroup = kAdmLanes * kAdmChunk;  // 512 elements
+constexpr uint32_t kAdmMod = 126;
+constexpr uint32_t kAdmLaneBytes = 32;                 // 256-bit lane buffer
+
+__host__ __device__ constexpr uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+__host__ __device__ constexpr uint64_t round_up(uint64_t a, uint64_t b) { return div_up(a, b) * b; }
+
+struct Format {
+  uint32_t lanes;
+  uint32_t block_bytes;
+  uint32_t version;
+};
+__host__ __device__ inline Format format_for_lanes(int lanes) {
+  return lanes == 64 ? Format{64u, 8192u, 2u} : Format{32u, 4096u, 1u};
+}
+
+// Bound on the 16-bit words one interleaved lane emits for nsym symbols:
+// each step grows the virtual state x*2^(16w) by at most
+// (2^p/pdf)(1 + 2^(p-15)) <= 2^(p+0.1), it starts at 2^15 and ends >= 2^15,
+// so 16w <= nsym (p + 0.1).  One word of slack.
+__host__ __device__ constexpr uint32_t lane_max_words(uint32_t nsym, int p) {
+  return (uint32_t)(((uint64_t)nsym * (uint64_t)(10 * p + 1) + 159u) / 160u) + 1u;
+}
+__host__ __device__ constexpr uint32_t unit_max_words(uint32_t lanes, uint32_t block_bytes, int p) {
+  return lanes * lane_max_words(block_bytes / lanes, p);
+}
+// LDS staging words for one wavefront: 64 lanes in either format.
+__host__ __device__ constexpr uint32_t wave_stage_words(int p) {
+  return (uint32_t)round_up(64u * lane_max_words(128u, p), 8u);
+}
+
+__host__ __device__ inline uint64_t section_bytes(uint64_t nb, Format f) {
+  return kSectionFixed + 4ull * f.lanes * nb + 8ull * round_up(nb, 2);
+}
+
+// One byte plane coded as one ANS "plane section" (header, probs, states,
+// block words) whose data lives in the shared data region.
+struct PlaneDesc {
+  const uint8_t* src;   // encode: symbol i = src[i * stride]
+  uint8_t* dst;         // decode: symbol i -> dst[i * stride]
+  uint64_t nsym;        // symbols in the plane
+  uint64_t nb;          // blocks = ceil(nsym / block_bytes)
+  uint64_t section;     // byte offset of the plane's ANS header in the container
+  uint64_t unit_base;   // first unit index of the plane (concatenated order)
+  uint32_t stride;
+  uint32_t pad_;
+};
+
+// Unit u -> (plane, block): order 0 interleaves planes (u % P, u / P) so one
+// wavefront's units read the same input bytes; order 1 concatenates them.
+struct CodecDesc {
+  PlaneDesc plane[kMaxPlanes];
+  int32_t num_planes;
+  int32_t order;
+  int32_t prob_bits;
+  int32_t lanes;
+  uint64_t total_units;
+  uint64_t data_off;    // byte offset of the shared data region
+};
+
+__device__ __forceinline__ void unit_to_plane(const CodecDesc& d, uint64_t u, int& k, uint64_t& b) {
+  if (d.order == 0) {
+    uint32_t P = (uint32_t)d.num_planes;
+    k = (int)(u % P);
+    b = u / P;
+  } else {
+    k = 0;
+    while (k + 1 < d.num_planes && u >= d.plane[k + 1].unit_base) ++k;
+    b = u - d.plane[k].unit_base;
+  }
+}
+
+// Device-side error flags (workspace).
+enum : uint32_t {
+  kErrMagic = 1u << 0,
+  kErrPrecision = 1u << 1,
+  kErrTable = 1u << 2,
+  kErrBlockWords = 1u << 3,
+  kErrUnderflow = 1u << 4,
+  kErrFinalState = 1u << 5,
+  kErrCapacity = 1u << 6,
+  kErrStageOverflow = 1u << 7,
+  kErrAdmDomain = 1u << 8,
+};
+
+// Histogram sources: bytes of src are grouped in elements of `stride` bytes;
+// byte j of each element feeds plane plane0 + j (stride 1: plane plane0).
+struct HistSource {
+  const uint8_t* ptr;
+  uint64_t nbytes;
+  uint32_t stride;
+  uint32_t plane0;
+};
+
+struct HistArgs {
+  HistSource src[2];
+  int32_t nsrc;
+  int32_t num_planes;
+  int32_t prob_bits;
+  int32_t write_headers;         // 0: only tables (stage API)
+  uint64_t plane_nsym[kMaxPlanes];
+  uint64_t plane_nb[kMaxPlanes];
+  uint64_t plane_section[kMaxPlanes];
+  uint32_t* hist;                // [P][256] self-cleaning accumulators
+  uint32_t* done;                // self-cleaning WG counter
+  uint4* table;                  // [P][256] {pdf, cdf, magic, shift}
+  uint32_t* hist_out;            // optional copy of the final histogram
+  uint8_t* out;                  // container base (headers), may be null
+  uint32_t version;
+  uint32_t pad_;
+  // finisher also prepares the encode launch:
+  uint64_t* status;              // look-back status to clear
+  uint64_t status_count;
+  uint32_t* ticket;              // encode ticket counter to clear
+  uint32_t* d_out_size32;        // written when there is nothing to encode
+  uint64_t* d_out_size64;
+  uint64_t empty_size;           // container size when total units == 0
+  uint64_t total_units;
+  int32_t has_frame;             // copy frame_hdr (128 bytes) to out[0..128)
+  uint32_t frame_hdr[32];
+};
+
+struct EncodeArgs {
+  CodecDesc d;
+  const uint4* table;            // [P][256]
+  uint8_t* out;
+  uint64_t* status;              // [chain] look-back words
+  uint32_t* ticket;
+  uint32_t* err;
+  uint32_t* d_out_size32;
+  uint64_t* d_out_size64;
+  int32_t frame;                 // 1: patch data_words / total_size in the frame header
+};
+
+struct DecodeArgs {
+  CodecDesc d;                   // when from_header: plane 0 section/dst/stride only
+  const uint8_t* in;
+  uint64_t in_bytes;             // 0 = unknown
+  uint64_t out_capacity;         // 0 = unchecked
+  uint64_t data_words;           // frames: from the header; single stream: 0 -> header
+  uint32_t* err;
+  int32_t from_header;           // single stream: nb / nsym / version read on device
+};
+
+struct AdmMapArgs {
+  const uint16_t* in;
+  uint64_t n;
+  uint64_t gsize;
+  int32_t* offsets;      // [gsize + 1]
+  uint16_t* centers;     // [gsize]
+  uint8_t* codes;        // [n]
+  uint8_t* bits;         // bit plane
+  uint64_t bits_capacity;
+  uint64_t* status;      // look-back words [gsize]
+  uint32_t* ticket;
+  uint32_t* err;         // kErrAdmDomain / kErrCapacity
+};
+
+struct AdmUnmapArgs {
+  const int32_t* offsets;
+  const uint16_t* centers;
+  const uint8_t* codes;
+  const uint8_t* bits;
+  uint64_t n;
+  uint64_t gsize;
+  uint64_t bits_len;     // bytes in the bit plane
+  uint16_t* out;
+  uint32_t* err;         // kErrBlockWords on inconsistent offsets
+};
+
+// Launchers (one per translation unit).  Each returns hipSuccess or the
+// launch error.  `cu` is the device's compute-unit count.
+hipError_t launch_hist(const HistArgs& a, int cu, hipStream_t s);
+hipError_t launch_encode(const EncodeArgs& a, int cu, hipStream_t s);
+hipError_t launch_decode(const DecodeArgs& a, int cu, hipStream_t s);
+hipError_t launch_adm_map(const AdmMapArgs& a, int cu, hipStream_t s);
+hipError_t launch_adm_unmap(const AdmUnmapArgs& a, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+}  // namespace mans

@@ -497,6 +497,14 @@ int mo_adm_map(const uint16_t* x, uint64_t n, uint16_t* centers, int32_t* offset
     M[g] = maxbytes;
     (void)lane_len;
   }
+  if (!in_domain) {
+    /* outside the 256-bit lane buffer the reference overflows it; there is
+     * no defined bit plane to produce, only the verdict */
+    free(lane_bits);
+    free(M);
+    *bits_len = 0;
+    return 0;
+  }
   /* pass 2: exclusive prefix (:127-132) */
   int64_t total = 0;
   for (uint64_t g = 0; g < gsize; ++g) { offsets[g] = (int32_t)total; total += M[g]; }
