@@ -1,0 +1,206 @@
+"""Benchmark: plain multi-byte ANS codec on one MI355X.
+
+A step is one round trip of the hot path over 1 GiB of uint16 quantisation
+codes resident in HBM: frame encode (per-plane histogram + normalisation
+launch, fused rANS encode launch) then frame decode.  The codes are the
+Lorenzo-residual codes of a 1024 x 512 x 1024 sinusoid field, coded as two
+byte planes (the ADM mapping is outside its lossless domain on this data) in
+the native 64-lane format at probability precision 10.
+
+Prints one JSON line.  `value` is round-trip throughput on uncompressed bytes
+(GB/s, 10^9).  Encode and decode throughput and their HBM fractions (encode
+bytes 2N + C, decode bytes C + N) are reported beside it; `roofline` is the
+dominant kernel's (the encode launch: reads N, writes C) from HIP events
+recorded around every launch in the timed region.  `cpu_baseline` is the CPU
+port of the same frame codec (oracle/, OpenMP) on a sample of the same codes.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "encode/decode GB/s (% of HBM peak) at 1 GPU; batched/served GB/s + p99 vs plain"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--shape", default="1024x512x1024", help="field shape z x y x x (codes = product)")
+    ap.add_argument("--lanes", type=int, default=64)
+    ap.add_argument("--prob-bits", type=int, default=10)
+    ap.add_argument("--cpu-sample-mib", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device copy (HBM counter calibration)")
+    return ap.parse_args()
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only (the codec path is single-GPU; "
+                                   "sharding is described in DESIGN.md)", "n_gpus": args.gpus}))
+        return 2
+    import torch
+
+    import mans_amd
+    from mans_amd import workloads
+
+    shape = tuple(int(v) for v in args.shape.split("x"))
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    x = workloads.lorenzo_codes(shape)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    n = x.numel()
+    nbytes = n * 2
+    p = mans_amd.params(args.prob_bits, args.lanes, 2, mans_amd.MODE_PLANES)
+    ws = mans_amd.Workspace(0)
+    ws.reserve(nbytes, p)
+    out = torch.empty(mans_amd.frame_max_size(n, p), dtype=torch.uint8, device="cuda")
+    dec = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+
+    # size + header once (the frame is deterministic)
+    _, csize = mans_amd.frame_encode(x, p, ws, out=out)
+    header = bytes(out[:128].cpu().numpy().tobytes())
+    ws.check()
+
+    def step(ev):
+        ev[0].record(stream)
+        mans_amd.frame_encode(x, p, ws, out=out, sync_size=False)
+        ev[1].record(stream)
+        mans_amd.frame_decode(out, csize, torch.int16, n, ws, out=dec, header=header)
+        ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step([torch.cuda.Event(enable_timing=True) for _ in range(3)])
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dec, x))
+    ws.check()
+
+    ws.set_profiling(True)
+    ws.kernel_times()  # start a fresh record
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(evs[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    kt = ws.kernel_times()
+    ws.set_profiling(False)
+    ok = ok and bool(torch.equal(dec, x))
+    ws.check()
+
+    enc_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    dec_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    ms_step = wall * 1e3 / args.steps
+    per_kernel: dict[str, list[float]] = {}
+    for name, ms in kt:
+        per_kernel.setdefault(name, []).append(ms)
+    kavg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
+    C = int(csize)
+    enc_kernel_ms = kavg.get("encode", enc_ms)
+    dom_bytes = nbytes + C  # encode launch: reads the N input bytes once, writes the C-byte frame
+    achieved = dom_bytes / (enc_kernel_ms * 1e-3) / 1e9
+
+    if args.copy_calibration:
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del y
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(x, args)
+
+    line = {
+        "metric": METRIC,
+        "value": round(nbytes / (ms_step * 1e-3) / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic",
+        "config": {
+            "workload": f"uint16 Lorenzo quantisation codes (error bound 1e-3, radius 512) of a "
+                        f"{shape[0]}x{shape[1]}x{shape[2]} sinusoid field, {nbytes / 2**30:.3f} GiB; "
+                        f"frame encode + decode round trip, 2 byte planes, "
+                        f"{'native 64-lane' if args.lanes == 64 else 'reference 32-lane'} format, p={args.prob_bits}",
+            "uncompressed_bytes": nbytes,
+            "compressed_bytes": C,
+        },
+        "encode_GBps": round(nbytes / (enc_ms * 1e-3) / 1e9, 2),
+        "decode_GBps": round(nbytes / (dec_ms * 1e-3) / 1e9, 2),
+        "encode_hbm_frac": round((2 * nbytes + C) / (enc_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        "decode_hbm_frac": round((C + nbytes) / (dec_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        "compression_ratio": round(nbytes / C, 4),
+        "round_trip_bit_exact": ok,
+        "kernel_ms": {k: round(v, 4) for k, v in kavg.items()},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "encode",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "gen_s": round(gen_s, 2),
+    }
+    print(json.dumps(line), flush=True)
+    return 0 if ok else 1
+
+
+def cpu_baseline(x, args):
+    """The CPU port (oracle/) of the same frame codec on the first
+    cpu_sample_mib MiB of the same codes, all host threads available."""
+    try:
+        import oracle
+    except Exception as e:  # pragma: no cover
+        return {"error": str(e)}
+    n = min(x.numel(), args.cpu_sample_mib * (1 << 20) // 2)
+    sample = x[:n].cpu().numpy().view(np.uint16)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:  # pragma: no cover
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+    fmt = oracle.FORMAT_NATIVE if args.lanes == 64 else oracle.FORMAT_REF
+    t = time.perf_counter()
+    c = oracle.frame_encode(sample, oracle.MODE_PLANES, args.prob_bits, fmt, threads=cores)
+    te = time.perf_counter() - t
+    t = time.perf_counter()
+    d = oracle.frame_decode(c, threads=cores)
+    td = time.perf_counter() - t
+    ok = bool((d == sample).all())
+    b = sample.nbytes
+    return {
+        "value": round(b / (te + td) / 1e9, 4),
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"first {b / 2**20:.0f} MiB of the same codes, frame encode + decode, same format",
+        "encode_GBps": round(b / te / 1e9, 4),
+        "decode_GBps": round(b / td / 1e9, 4),
+        "bit_exact": ok,
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

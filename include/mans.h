@@ -128,17 +128,20 @@ uint64_t mans_frame_max_size(uint64_t n_elems, const mans_params* p);
 mans_status mans_frame_encode(const mans_params* p, const void* d_in, uint64_t n_elems, void* d_out,
                               uint64_t out_capacity, uint64_t* d_size, uint64_t* h_size,
                               mans_workspace* ws, void* stream);
-/* Reads the 128-byte frame header (synchronising) and decodes into d_out. */
-mans_status mans_frame_decode(const void* d_in, uint64_t c, void* d_out, uint64_t out_capacity,
-                              uint64_t* h_n_elems, mans_workspace* ws, void* stream);
+/* Decodes the frame at d_in (c bytes) into d_out.  The layout comes from the
+ * 128-byte frame header: pass a host copy in h_header for a fully
+ * asynchronous call, or NULL to have it read from d_in (synchronising). */
+mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header, void* d_out,
+                              uint64_t out_capacity, uint64_t* h_n_elems, mans_workspace* ws, void* stream);
 /* Parses a frame header already on the host. */
 mans_status mans_frame_info(const void* h_header, uint64_t header_bytes, uint64_t* n_elems,
                             int32_t* elem_bytes, int32_t* mode, uint64_t* total_size);
 
 /* ---- profiling --------------------------------------------------------- */
-/* When enabled, each call records HIP events around its kernels; the
- * durations (ms) of the last call are returned by mans_last_kernel_times in
- * launch order with their kernel names. */
+/* When enabled, every call records HIP events on its stream around each of
+ * its kernels.  mans_last_kernel_times waits for them and returns the
+ * durations (ms) of all kernels launched since the previous query, in launch
+ * order with their names (static strings), then starts a new record. */
 mans_status mans_set_profiling(mans_workspace* ws, int32_t enable);
 int32_t mans_last_kernel_times(mans_workspace* ws, double* h_ms, const char** h_names, int32_t max);
 

@@ -50,7 +50,7 @@ SIGNATURES = {
     "mans_adm_unmap": (C.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp]),
     "mans_frame_max_size": (_u64, [_u64, _P]),
     "mans_frame_encode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, C.POINTER(_u64), _vp, _vp]),
-    "mans_frame_decode": (C.c_int, [_vp, _u64, _vp, _u64, C.POINTER(_u64), _vp, _vp]),
+    "mans_frame_decode": (C.c_int, [_vp, _u64, _vp, _vp, _u64, C.POINTER(_u64), _vp, _vp]),
     "mans_frame_info": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64)]),
     "mans_set_profiling": (C.c_int, [_vp, _i32]),
     "mans_last_kernel_times": (_i32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_char_p), _i32]),
@@ -137,10 +137,11 @@ class Workspace:
     def set_profiling(self, on: bool = True) -> None:
         _check(lib().mans_set_profiling(self.handle, 1 if on else 0), "set_profiling")
 
-    def kernel_times(self) -> list[tuple[str, float]]:
-        ms = (C.c_double * 16)()
-        names = (C.c_char_p * 16)()
-        n = lib().mans_last_kernel_times(self.handle, ms, names, 16)
+    def kernel_times(self, max_records: int = 4096) -> list[tuple[str, float]]:
+        """(kernel, ms) for every launch since the previous query."""
+        ms = (C.c_double * max_records)()
+        names = (C.c_char_p * max_records)()
+        n = lib().mans_last_kernel_times(self.handle, ms, names, max_records)
         return [(names[i].decode(), ms[i]) for i in range(n)]
 
 
@@ -244,12 +245,16 @@ def frame_encode(x, p: Params, ws: Workspace, out=None, stream=None, sync_size: 
     return (out, h_size.value) if sync_size else (out, d_size)
 
 
-def frame_decode(comp, size: int, dtype, n: int, ws: Workspace, out=None, stream=None):
+def frame_decode(comp, size: int, dtype, n: int, ws: Workspace, out=None, stream=None,
+                 header: bytes | None = None):
+    """Decode a MANS frame.  With `header` (the frame's first 128 bytes on the
+    host) the call does not synchronise."""
     torch = _torch()
     if out is None:
         out = torch.empty(max(n, 8), dtype=dtype, device=comp.device)
     nout = C.c_uint64(0)
-    rc = lib().mans_frame_decode(comp.data_ptr(), size, out.data_ptr(), out.numel() * out.element_size(),
+    hbuf = (C.c_uint8 * 128).from_buffer_copy(header[:128]) if header is not None else None
+    rc = lib().mans_frame_decode(comp.data_ptr(), size, hbuf, out.data_ptr(), out.numel() * out.element_size(),
                                  C.byref(nout), ws.handle, _stream(stream))
     _check(rc, "mans_frame_decode")
     return out[:n]

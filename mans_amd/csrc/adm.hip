@@ -117,12 +117,12 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
     if (lane == 0) {
       uint64_t* st = a.status + chain;
       if (chain == 0) {
-        __hip_atomic_store(st, kFlagPrefix | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(st, kFlagPrefix | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        __hip_atomic_store(st, kFlagAgg | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(st, kFlagAgg | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint64_t j = chain - 1;
         for (;;) {
-          const uint64_t sv = __hip_atomic_load(a.status + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          const uint64_t sv = __hip_atomic_load(a.status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           const uint64_t f = sv >> 62;
           if (f == 0) {
             __builtin_amdgcn_s_sleep(1);
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
           if (f == 2) break;
           --j;
         }
-        __hip_atomic_store(st, kFlagPrefix | (excl + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(st, kFlagPrefix | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     excl = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(excl >> 32)) << 32) |

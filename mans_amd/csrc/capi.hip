@@ -84,7 +84,6 @@ mans_status ensure_scratch(mans_workspace* ws, uint64_t n) {
 }
 
 // Profiling: events recorded around launches when enabled.
-void prof_begin(mans_workspace* ws) { ws->ev_used = 0; ws->ev_names.clear(); }
 void prof_mark(mans_workspace* ws, hipStream_t s, const char* name) {
   if (!ws->profiling) return;
   if ((size_t)ws->ev_used >= ws->ev.size()) {
@@ -303,7 +302,6 @@ mans_status mans_encode(const mans_params* p, const void* d_in, uint64_t n, void
   if (out_capacity < mans_max_compressed_size(n, p)) return MANS_ERR_CAPACITY;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
-  prof_begin(ws);
   const Format f = format_for_lanes(p->lanes);
   CodecDesc d;
   memset(&d, 0, sizeof(d));
@@ -335,7 +333,6 @@ mans_status mans_decode(const mans_params* p, const void* d_in, uint64_t c, void
   if (c && c < kSectionFixed) return MANS_ERR_CORRUPT;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
-  prof_begin(ws);
   DecodeArgs a;
   memset(&a, 0, sizeof(a));
   a.d.num_planes = 1;
@@ -363,7 +360,6 @@ mans_status mans_histogram(const void* d_in, uint64_t n, uint32_t stride, uint32
   if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
-  prof_begin(ws);
   HistArgs h;
   memset(&h, 0, sizeof(h));
   h.src[0] = HistSource{(const uint8_t*)d_in, n * stride, stride, 0u};
@@ -389,7 +385,6 @@ mans_status mans_normalize(const uint32_t* d_hist, uint32_t total, int32_t prob_
   if (prob_bits < 9 || prob_bits > 11) return MANS_ERR_PRECISION;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
-  prof_begin(ws);
   HIP_TRY(hipMemcpyAsync(ws->hist, d_hist, 256 * 4, hipMemcpyDeviceToDevice, s));
   HistArgs h;
   memset(&h, 0, sizeof(h));
@@ -455,7 +450,6 @@ mans_status mans_adm_map(const uint16_t* d_in, uint64_t n, int32_t* d_offsets, u
   if (!aligned(d_in, 16) || !aligned(d_codes, 16)) return MANS_ERR_ALIGNMENT;
   if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
   DeviceGuard g(ws->device);
-  prof_begin(ws);
   mans_status st = adm_map_impl(d_in, n, d_offsets, d_centers, d_codes, d_bits, bits_capacity, h_bits_len,
                                 h_in_domain, ws, (hipStream_t)stream);
   prof_mark(ws, (hipStream_t)stream, nullptr);
@@ -468,7 +462,6 @@ mans_status mans_adm_unmap(const int32_t* d_offsets, const uint16_t* d_centers, 
   if (!aligned(d_out, 16) || !aligned(d_codes, 16)) return MANS_ERR_ALIGNMENT;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
-  prof_begin(ws);
   AdmUnmapArgs a;
   memset(&a, 0, sizeof(a));
   a.offsets = d_offsets;
@@ -510,7 +503,6 @@ mans_status mans_frame_encode(const mans_params* p, const void* d_in, uint64_t n
   if (out_capacity < mans_frame_max_size(n, p)) return MANS_ERR_CAPACITY;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
-  prof_begin(ws);
   const Format f = format_for_lanes(p->lanes);
   uint8_t* out = (uint8_t*)d_out;
   int mode = p->mode == MANS_MODE_AUTO ? (eb == 2 && aligned(d_in, 16) ? MANS_MODE_ADM : MANS_MODE_PLANES) : p->mode;
@@ -618,16 +610,21 @@ mans_status mans_frame_info(const void* h_header, uint64_t header_bytes, uint64_
   return MANS_OK;
 }
 
-mans_status mans_frame_decode(const void* d_in, uint64_t c, void* d_out, uint64_t out_capacity, uint64_t* h_n_elems,
-                              mans_workspace* ws, void* stream) {
+mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header, void* d_out, uint64_t out_capacity,
+                              uint64_t* h_n_elems, mans_workspace* ws, void* stream) {
   if (!ws || !d_in || c < kFrameHeaderBytes) return MANS_ERR_INVALID_ARG;
   if (!aligned(d_in, 16)) return MANS_ERR_ALIGNMENT;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
-  prof_begin(ws);
-  uint8_t* hb = (uint8_t*)ws->h_pinned;
-  HIP_TRY(hipMemcpyAsync(hb, d_in, kFrameHeaderBytes, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  uint8_t hcopy[kFrameHeaderBytes];
+  const uint8_t* hb = hcopy;
+  if (h_header) {
+    memcpy(hcopy, h_header, kFrameHeaderBytes);
+  } else {
+    HIP_TRY(hipMemcpyAsync(ws->h_pinned, d_in, kFrameHeaderBytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    memcpy(hcopy, ws->h_pinned, kFrameHeaderBytes);
+  }
   uint32_t magic, block_bytes;
   uint64_t n, gsize, meta, data_off, data_words, total, sect[4], nsym[4];
   memcpy(&magic, hb, 4);
@@ -717,18 +714,20 @@ mans_status mans_set_profiling(mans_workspace* ws, int32_t enable) {
 }
 
 int32_t mans_last_kernel_times(mans_workspace* ws, double* h_ms, const char** h_names, int32_t max) {
-  if (!ws || !ws->profiling || ws->ev_used < 2) return 0;
+  if (!ws || !ws->profiling) return 0;
   DeviceGuard g(ws->device);
   int32_t k = 0;
   for (int i = 0; i + 1 < ws->ev_used && k < max; ++i) {
     if (!ws->ev_names[i]) continue;
     float ms = 0;
-    if (hipEventSynchronize(ws->ev[i + 1]) != hipSuccess) return k;
-    if (hipEventElapsedTime(&ms, ws->ev[i], ws->ev[i + 1]) != hipSuccess) return k;
+    if (hipEventSynchronize(ws->ev[i + 1]) != hipSuccess) break;
+    if (hipEventElapsedTime(&ms, ws->ev[i], ws->ev[i + 1]) != hipSuccess) break;
     if (h_ms) h_ms[k] = ms;
     if (h_names) h_names[k] = ws->ev_names[i];
     ++k;
   }
+  ws->ev_used = 0;
+  ws->ev_names.clear();
   return k;
 }
 

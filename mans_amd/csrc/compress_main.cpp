@@ -159,10 +159,10 @@ int run_frame(const std::string& in_path, const std::string& tmp_path, const std
   if (!write_file(tmp_path, comp.data(), csize)) return 1;
   CHECK_HIP(hipMalloc(&d_dec, data.size() ? data.size() : 16));
   uint64_t nout = 0;
-  st = mans_frame_decode(d_enc, csize, d_dec, data.size(), &nout, ws, stream);
+  st = mans_frame_decode(d_enc, csize, nullptr, d_dec, data.size(), &nout, ws, stream);
   CHECK_HIP(hipStreamSynchronize(stream));
   t0 = now_ms();
-  st = mans_frame_decode(d_enc, csize, d_dec, data.size(), &nout, ws, stream);
+  st = mans_frame_decode(d_enc, csize, nullptr, d_dec, data.size(), &nout, ws, stream);
   CHECK_HIP(hipStreamSynchronize(stream));
   double decomp_ms = now_ms() - t0;
   if (st != MANS_OK || mans_workspace_check(ws, stream, nullptr) != MANS_OK) {

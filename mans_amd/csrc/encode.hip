@@ -151,6 +151,10 @@ __global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a) {
     if (active) ((uint32_t*)(sec + kSectionFixed))[b * L + lin] = x;
 
     // ---- decoupled look-back over tickets --------------------------------
+    // Each status word carries its flag and its value together, and no other
+    // memory is handed over through it, so relaxed device-scope atomics are
+    // enough.  (Release / acquire at agent scope would write back / invalidate
+    // the whole L2 of the XCD on every publish and poll.)
     const uint32_t a_lo = __builtin_amdgcn_readlane(padded, 0);
     const uint32_t a_hi = L == 32 ? __builtin_amdgcn_readlane(padded, 32) : 0u;
     const uint64_t agg = (uint64_t)a_lo + a_hi;
@@ -159,12 +163,12 @@ __global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a) {
     if (lane == 0) {
       uint64_t* st = a.status + chain;
       if (chain == 0) {
-        __hip_atomic_store(st, kFlagPrefix | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(st, kFlagPrefix | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        __hip_atomic_store(st, kFlagAgg | agg, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(st, kFlagAgg | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint64_t j = chain - 1;
         for (;;) {
-          const uint64_t v = __hip_atomic_load(a.status + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          const uint64_t v = __hip_atomic_load(a.status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           const uint64_t f = v >> 62;
           if (f == 0) {
             __builtin_amdgcn_s_sleep(1);
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a) {
           if (f == 2) break;
           --j;
         }
-        __hip_atomic_store(st, kFlagPrefix | (excl + agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(st, kFlagPrefix | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     excl = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(excl >> 32)) << 32) |

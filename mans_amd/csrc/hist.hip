@@ -103,17 +103,42 @@ __device__ uint4 normalize_symbol(uint32_t count, uint32_t total, int p, uint32_
   return make_uint4(pdf, cdf, magic, shift);
 }
 
+// Per-thread run-length merging: a thread keeps (current byte, run length)
+// per plane and adds the run to LDS only when the byte changes.  Planes that
+// are nearly one value (high bytes of quantisation codes, ADM bit planes)
+// would otherwise send all 64 lanes of a wavefront to one LDS bin per step;
+// uniform planes pay two extra VALU ops per byte.
 template <int S>
-__device__ __forceinline__ void count_word(uint32_t* h, uint32_t v) {
+struct Runs {
+  uint32_t cur[S];
+  uint32_t cnt[S];
+};
+
+template <int S>
+__device__ __forceinline__ void run_add(uint32_t* h, Runs<S>& r, int p, uint32_t b) {
+  const bool flush = b != r.cur[p];
+  if (flush) atomicAdd(h + (p << 8) + r.cur[p], r.cnt[p]);  // adds 0 for the initial empty run
+  r.cnt[p] = flush ? 1u : r.cnt[p] + 1u;
+  r.cur[p] = b;
+}
+
+template <int S>
+__device__ __forceinline__ void count_word(uint32_t* h, Runs<S>& r, uint32_t v) {
   // byte b of a 4-byte word belongs to plane b % S (elements are S-aligned)
-  atomicAdd(h + ((0 % S) << 8) + (v & 0xffu), 1u);
-  atomicAdd(h + ((1 % S) << 8) + ((v >> 8) & 0xffu), 1u);
-  atomicAdd(h + ((2 % S) << 8) + ((v >> 16) & 0xffu), 1u);
-  atomicAdd(h + ((3 % S) << 8) + (v >> 24), 1u);
+  run_add<S>(h, r, 0 % S, v & 0xffu);
+  run_add<S>(h, r, 1 % S, (v >> 8) & 0xffu);
+  run_add<S>(h, r, 2 % S, (v >> 16) & 0xffu);
+  run_add<S>(h, r, 3 % S, v >> 24);
 }
 
 template <int S>
 __device__ void count_source(const HistSource& src, uint32_t* h /* wave's [P][256] + plane0 */) {
+  Runs<S> r;
+#pragma unroll
+  for (int p = 0; p < S; ++p) {
+    r.cur[p] = 0;
+    r.cnt[p] = 0;
+  }
   const uint8_t* p = src.ptr;
   const uint64_t n = src.nbytes;
   const uint32_t mis = (uint32_t)((uintptr_t)p & 15u);
@@ -127,13 +152,16 @@ __device__ void count_source(const HistSource& src, uint32_t* h /* wave's [P][25
   for (; i + nthreads < n4; i += 2 * nthreads) {
     uint4 a = body[i];
     uint4 b = body[i + nthreads];
-    count_word<S>(h, a.x); count_word<S>(h, a.y); count_word<S>(h, a.z); count_word<S>(h, a.w);
-    count_word<S>(h, b.x); count_word<S>(h, b.y); count_word<S>(h, b.z); count_word<S>(h, b.w);
+    count_word<S>(h, r, a.x); count_word<S>(h, r, a.y); count_word<S>(h, r, a.z); count_word<S>(h, r, a.w);
+    count_word<S>(h, r, b.x); count_word<S>(h, r, b.y); count_word<S>(h, r, b.z); count_word<S>(h, r, b.w);
   }
   if (i < n4) {
     uint4 a = body[i];
-    count_word<S>(h, a.x); count_word<S>(h, a.y); count_word<S>(h, a.z); count_word<S>(h, a.w);
+    count_word<S>(h, r, a.x); count_word<S>(h, r, a.y); count_word<S>(h, r, a.z); count_word<S>(h, r, a.w);
   }
+#pragma unroll
+  for (int p = 0; p < S; ++p)
+    if (r.cnt[p]) atomicAdd(h + (p << 8) + r.cur[p], r.cnt[p]);
   if (blockIdx.x == 0) {
     // unaligned head (< 16 bytes) and tail (< 16 bytes); offsets from the
     // element-aligned source start give the plane

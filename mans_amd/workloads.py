@@ -1,0 +1,83 @@
+"""Seeded synthetic inputs of the benchmark shapes.
+
+* laplace_u16      -- clip(32768 + round(Laplace(0, b)), 0, 65535), the small
+                      CPU-runnable case (2,097,152 values, b = 4, seed 0).
+* lorenzo_codes    -- uint16 error-bounded quantisation codes: a float32 3-D
+                      field (sum of 64 random-phase plane sinusoids, wavelengths
+                      16-256 cells, plus N(0, 0.01) noise) quantised with
+                      absolute error bound 1e-3 as q = round(f / 2e-3), 3-D
+                      Lorenzo residual r of q, code = 32768 + r if |r| < 512
+                      else 0.  Generated on the GPU in z-slabs.  The 1 GiB
+                      case is the 1024 x 512 x 1024 field (2^29 codes).
+* zipf_u32         -- Zipf(s) ranks over 2^20 values mapped through a seeded
+                      permutation to 32-bit words with the high byte in [0, 15].
+* detector_u16     -- frames of round(N(1000, 8)) + 40 * Poisson(0.05).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def laplace_u16(n: int = 2_097_152, b: float = 4.0, seed: int = 0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return np.clip(32768 + np.round(rng.laplace(0, b, n)), 0, 65535).astype(np.uint16)
+
+
+def lorenzo_codes(shape=(1024, 512, 1024), seed: int = 1, error_bound: float = 1e-3, radius: int = 512,
+                  device: str = "cuda", slab: int = 32):
+    """int16-typed CUDA tensor holding the uint16 codes (z-major, then y, x)."""
+    import torch
+    import torch.nn.functional as F
+
+    nz, ny, nx = shape
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(64, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    lam = np.exp(rng.uniform(np.log(16.0), np.log(256.0), 64))
+    phi = rng.uniform(0, 2 * np.pi, 64)
+    kvec = 2 * np.pi * d / lam[:, None]  # (64, 3) over (x, y, z)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    xs = torch.arange(nx, device=device, dtype=torch.float32)
+    ys = torch.arange(ny, device=device, dtype=torch.float32)
+    zs = torch.arange(nz, device=device, dtype=torch.float32)
+    out = torch.empty(nz * ny * nx, dtype=torch.int16, device=device)
+    q_prev = torch.zeros((1, ny, nx), dtype=torch.int32, device=device)
+    for z0 in range(0, nz, slab):
+        zc = zs[z0:z0 + slab]
+        f = torch.zeros((zc.numel(), ny, nx), dtype=torch.float32, device=device)
+        for j in range(64):
+            kx, ky, kz = (float(v) for v in kvec[j])
+            px = (kx * xs)[None, None, :]
+            py = (ky * ys)[None, :, None]
+            pz = (kz * zc + float(phi[j]))[:, None, None]
+            f += torch.sin(px + py + pz)
+        f += 0.01 * torch.randn(f.shape, generator=gen, device=device, dtype=torch.float32)
+        q = torch.round(f / (2 * error_bound)).to(torch.int32)
+        del f
+        qp = F.pad(torch.cat([q_prev, q], 0), (1, 0, 1, 0))  # zero row/column in front of y and x
+        a = qp[1:, 1:, 1:]
+        res = (a - qp[:-1, 1:, 1:] - qp[1:, :-1, 1:] - qp[1:, 1:, :-1]
+               + qp[:-1, :-1, 1:] + qp[:-1, 1:, :-1] + qp[1:, :-1, :-1] - qp[:-1, :-1, :-1])
+        code = torch.where(res.abs() < radius, res + 32768, torch.zeros_like(res))
+        code = torch.where(code >= 32768, code - 65536, code).to(torch.int16)
+        out[z0 * ny * nx:(z0 + zc.numel()) * ny * nx] = code.reshape(-1)
+        q_prev = q[-1:].clone()
+        del q, qp, a, res, code
+    return out
+
+
+def zipf_u32(n: int, s: float = 1.1, seed: int = 2) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    values = rng.integers(0, 1 << 28, 1 << 20, dtype=np.uint32)  # high byte in [0, 15]
+    ranks = rng.zipf(s, n)
+    ranks = np.minimum(ranks, 1 << 20) - 1
+    return values[ranks].astype(np.uint32)
+
+
+def detector_u16(frames: int = 64, side: int = 2048, seed: int = 3) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    n = frames * side * side
+    ped = np.round(rng.normal(1000.0, 8.0, n))
+    photons = rng.poisson(0.05, n) * 40
+    return np.clip(ped + photons, 0, 65535).astype(np.uint16)
