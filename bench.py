@@ -38,8 +38,8 @@ def parse():
     ap.add_argument("--prob-bits", type=int, default=10)
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run one 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--dump", default=None,
+                    help="write the generated codes to this file and exit (input of mans_profile)")
     return ap.parse_args()
 
 
@@ -60,6 +60,10 @@ def main() -> int:
     x = workloads.lorenzo_codes(shape)
     torch.cuda.synchronize()
     gen_s = time.time() - t0
+    if args.dump:
+        x.cpu().numpy().tofile(args.dump)
+        print(json.dumps({"dumped": args.dump, "bytes": x.numel() * 2}), flush=True)
+        return 0
     n = x.numel()
     nbytes = n * 2
     p = mans_amd.params(args.prob_bits, args.lanes, 2, mans_amd.MODE_PLANES)
@@ -112,12 +116,6 @@ def main() -> int:
     enc_kernel_ms = kavg.get("encode", enc_ms)
     dom_bytes = nbytes + C  # encode launch: reads the N input bytes once, writes the C-byte frame
     achieved = dom_bytes / (enc_kernel_ms * 1e-3) / 1e9
-
-    if args.copy_calibration:
-        y = torch.empty_like(x)
-        y.copy_(x)
-        torch.cuda.synchronize()
-        del y
 
     cpu = None
     if not args.no_cpu:

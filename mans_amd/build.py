@@ -17,6 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmans.so")
 CLI = os.path.join(HERE, "bin", "compress")
+PROFILER = os.path.join(HERE, "bin", "mans_profile")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MANS_OFFLOAD_ARCH", "gfx950")
 
@@ -62,6 +63,10 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, *HOST_FLAGS, os.path.join(CSRC, "compress_main.cpp"), "-o", cli_tmp,
           f"-L{HERE}", "-lmans", "-Wl,-rpath,$ORIGIN/.."])
     os.replace(cli_tmp, CLI)
+    prof_tmp = PROFILER + ".tmp"
+    _run([HIPCC, *DEVICE_FLAGS, "-x", "hip", os.path.join(CSRC, "profile_main.cpp"), "-o", prof_tmp,
+          f"-L{HERE}", "-lmans", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(prof_tmp, PROFILER)
     if verbose:
         print(f"built {LIB} and {CLI}")
     return LIB

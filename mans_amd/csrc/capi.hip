@@ -297,7 +297,9 @@ mans_status mans_encode(const mans_params* p, const void* d_in, uint64_t n, void
   mans_status st = check_params(p);
   if (st != MANS_OK) return st;
   if (!ws || !d_out || (n > 0 && !d_in)) return MANS_ERR_INVALID_ARG;
-  if (!aligned(d_out, 16)) return MANS_ERR_ALIGNMENT;
+  // input 4-byte aligned as the reference requires (kANSRequiredAlignment,
+  // GpuANSCodec.h:216); the encoder reads whole dwords
+  if (!aligned(d_out, 16) || !aligned(d_in, 4)) return MANS_ERR_ALIGNMENT;
   if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
   if (out_capacity < mans_max_compressed_size(n, p)) return MANS_ERR_CAPACITY;
   DeviceGuard g(ws->device);
@@ -498,7 +500,7 @@ mans_status mans_frame_encode(const mans_params* p, const void* d_in, uint64_t n
   if (eb != 1 && eb != 2 && eb != 4) return MANS_ERR_INVALID_ARG;
   if (p->mode < MANS_MODE_PLANES || p->mode > MANS_MODE_AUTO) return MANS_ERR_INVALID_ARG;
   if (p->mode == MANS_MODE_ADM && eb != 2) return MANS_ERR_INVALID_ARG;
-  if (!aligned(d_out, 16) || !aligned(d_in, eb)) return MANS_ERR_ALIGNMENT;
+  if (!aligned(d_out, 16) || !aligned(d_in, 4)) return MANS_ERR_ALIGNMENT;
   if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
   if (out_capacity < mans_frame_max_size(n, p)) return MANS_ERR_CAPACITY;
   DeviceGuard g(ws->device);

@@ -1,4 +1,4 @@
-// encode.hip -- interleaved rANS encoder, one gfx950 wavefront per 64 lanes.
+// encode.hip -- interleaved rANS encoder on gfx950 wavefronts.
 //
 // Reference: include/ans/GpuANSEncode.h:24-164 (encodeOne / ansEncodeBatch),
 // :166-178 + BatchPrefixSum.h:29-187 (prefix of padded word counts) and
@@ -6,15 +6,20 @@
 // into an uncoalesced staging buffer in HBM, scans the word counts with up to
 // three launches (limit 512^2 blocks) and copies every block again into the
 // final container.  Here one persistent launch does all of it:
-//  * a wavefront takes a ticket (atomic counter) for its units: two 32-lane
-//    reference blocks (one per half-wave) or one 64-lane native block;
-//  * emitted 16-bit words go to an LDS staging area (bounded by
-//    lane_max_words), in lane order within each step (64-bit ballot + mbcnt);
+//  * a wavefront takes a ticket (atomic counter) for a group of units: the
+//    CH planes of one element block (two blocks, one per half-wave, in the
+//    32-lane reference format).  Each lane runs CH independent rANS chains,
+//    one per byte plane, fed by a single element load per step, so the
+//    serial per-chain dependency is interleaved CH ways;
+//  * element loads are software-pipelined: the next 16 steps are in flight
+//    while the current 16 are coded;
+//  * emitted 16-bit words go to LDS staging (bounded by lane_max_words), in
+//    lane order within each step (64-bit ballot + mbcnt);
 //  * the wave publishes its padded word count and resolves its output offset
 //    by decoupled look-back over earlier tickets (tickets are taken by running
 //    waves, so every wave waited on is resident and finishes);
-//  * it then writes states, block words and the LDS words straight to their
-//    final place, 16 bytes per lane per store.
+//  * it then writes states, block words and the staged words straight to
+//    their final place, 16 bytes per lane per store.
 // The arithmetic per symbol is the reference's: emit x & 0xffff when
 // x >= pdf * 2^(31-p); q = (umulhi(x, magic) + x) >> shift = x / pdf;
 // x = q * 2^p + (x - q * pdf) + cdf, computed as x + q * (2^p - pdf) + cdf.
@@ -23,33 +28,92 @@
 namespace mans {
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
 constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagPrefix = 2ull << 62;
 constexpr uint64_t kValueMask = (1ull << 62) - 1;
+constexpr int kBatch = 16;  // steps per prefetch batch
 
-struct LanePlane {   // per-plane fields copied to LDS (dynamic index by lane)
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+template <int S> struct ElemOf;
+template <> struct ElemOf<1> { using T = uint8_t; };
+template <> struct ElemOf<2> { using T = uint16_t; };
+template <> struct ElemOf<4> { using T = uint32_t; };
+
+struct LanePlane {  // per-plane fields copied to LDS (dynamic index by lane)
   const uint8_t* src;
   uint64_t nsym;
   uint64_t section;
   uint64_t nb;
 };
 
-template <int L, int PB, int S>
-__global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a) {
+// Words staged per chain per wavefront: 64 lanes x 128 symbols (one native
+// unit, or two 32-lane reference units, one per half), rounded to 16 words so
+// every staging area is 32-byte aligned.
+template <int PB>
+constexpr uint32_t chain_stage_words() {
+  return (uint32_t)round_up(64ull * lane_max_words(128u, PB), 16u);
+}
+
+template <int L>
+struct Emit {
+  // Position of this lane's word among the emitting lanes of its unit and
+  // the unit's count of words in this step.  In the 32-lane format each half
+  // of the wavefront is a separate unit.
+  static __device__ __forceinline__ void pos(uint64_t m, uint32_t half, uint32_t cnt, uint32_t& p, uint32_t& add) {
+    const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)(m >> 32);
+    uint32_t base;
+    if (L == 64) {
+      base = cnt;
+      add = (uint32_t)__popcll(m);
+    } else {
+      base = half ? cnt - (uint32_t)__popc(mlo) : cnt;
+      add = half ? (uint32_t)__popc(mhi) : (uint32_t)__popc(mlo);
+    }
+    p = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, base));
+  }
+};
+
+template <int L>
+__device__ __forceinline__ void enc_step(uint32_t& x, uint32_t& cnt, uint16_t* stage, const uint4* tab,
+                                         uint32_t sym, bool v, uint32_t half) {
+  const uint4 e = tab[sym];
+  const bool w = v && (x >= e.x);
+  const uint64_t m = __ballot(w);
+  uint32_t p, add;
+  Emit<L>::pos(m, half, cnt, p, add);
+  if (w) {
+    stage[p] = (uint16_t)x;
+    x >>= 16;
+  }
+  cnt += add;
+  if (v) {
+    const uint32_t q = (__umulhi(x, e.y) + x) >> (e.z & 31u);
+    x = __umul24(q, e.w) + x + (e.z >> 8);
+  }
+}
+
+// L lanes per unit, PB probability bits, S element bytes, CH chains per lane.
+// CH == S: the CH chains of a lane code the CH byte planes of the same
+// elements (unit order 0 with P == S).  CH == 1, S == 1: one chain per lane on
+// any unit (single streams, concatenated ADM planes).
+template <int L, int PB, int S, int CH, int NW>
+__global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
+  using T = typename ElemOf<S>::T;
+  constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
+  constexpr uint32_t HALVES = L == 32 ? 2u : 1u;
+  constexpr uint32_t UPW = CH * HALVES;  // units per ticket
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  static_assert(CH == S, "chain c codes byte c of each element");
   extern __shared__ uint4 smem[];
   __shared__ LanePlane s_pl[kMaxPlanes];
   __shared__ uint64_t s_ubase[kMaxPlanes + 1];
-  constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
-  constexpr uint32_t UPW = L == 32 ? 2u : 1u;
-  constexpr uint32_t STAGE = (uint32_t)round_up(64u * lane_max_words(128u, PB), 16u);
-  constexpr uint32_t STAGE_UNIT = STAGE / UPW;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int P = a.d.num_planes;
 
   uint4* s_tab = smem;  // [P][256] {thresh, magic, shift | cdf << 8, 2^PB - pdf}
-  for (int i = t; i < P * (int)kNumSymbols; i += kThreads) {
+  for (int i = t; i < P * (int)kNumSymbols; i += 64 * NW) {
     uint4 e = a.table[i];
     s_tab[i] = make_uint4(e.x << (31 - PB), e.z, e.w | (e.y << 8), (1u << PB) - e.x);
   }
@@ -65,99 +129,144 @@ __global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a) {
 
   const uint32_t half = L == 32 ? (uint32_t)(lane >> 5) : 0u;
   const uint32_t lin = (uint32_t)lane & (L - 1);
-  uint16_t* stage = (uint16_t*)(smem + P * kNumSymbols) + wave * STAGE + half * STAGE_UNIT;
+  // staging: [wave][chain][half] regions of CSTAGE / HALVES words
+  uint16_t* wstage = (uint16_t*)(smem + P * kNumSymbols) + wave * CH * CSTAGE;
 
   for (;;) {
     uint32_t tk = 0;
     if (lane == 0) tk = atomicAdd(a.ticket, UPW);
     tk = __builtin_amdgcn_readfirstlane(tk);
     if ((uint64_t)tk >= a.d.total_units) break;
-    const uint64_t unit = (uint64_t)tk + half;
-    const bool active = unit < a.d.total_units;
-    int k = 0;
-    uint64_t b = 0;
-    if (active) {
-      if (a.d.order == 0) {  // planes of one element array: P is 1, 2 or 4
-        k = (int)((uint32_t)unit & (uint32_t)(P - 1));
-        b = unit >> (P == 4 ? 2 : (P == 2 ? 1 : 0));
-      } else {
-        while (k + 1 < P && unit >= s_ubase[k + 1]) ++k;
-        b = unit - s_ubase[k];
+
+    uint32_t x[CH], cnt[CH], n[CH];
+    uint16_t* stage[CH];
+    const uint4* tab[CH];
+    uint64_t blk[CH];
+    int kp[CH];
+    bool act[CH];
+    const uint8_t* src[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const uint64_t unit = (uint64_t)tk + half * CH + c;
+      act[c] = unit < a.d.total_units;
+      int k = 0;
+      uint64_t b = 0;
+      if (act[c]) {
+        if (a.d.order == 0) {  // planes of one element array: P is 1, 2 or 4
+          k = (int)((uint32_t)unit & (uint32_t)(P - 1));
+          b = unit >> (P == 4 ? 2 : (P == 2 ? 1 : 0));
+        } else {
+          while (k + 1 < P && unit >= s_ubase[k + 1]) ++k;
+          b = unit - s_ubase[k];
+        }
       }
+      kp[c] = k;
+      blk[c] = b;
+      const LanePlane pl = s_pl[k];
+      const uint64_t start = b * BLOCK;
+      n[c] = act[c] ? (uint32_t)((pl.nsym - start) < BLOCK ? (pl.nsym - start) : BLOCK) : 0u;
+      src[c] = pl.src;
+      x[c] = kMinState;
+      cnt[c] = 0;
+      stage[c] = wstage + c * CSTAGE + half * (CSTAGE / HALVES);
+      tab[c] = s_tab + k * kNumSymbols;
     }
-    const LanePlane pl = s_pl[k];
-    const uint64_t start = b * BLOCK;
-    const uint32_t n = active ? (uint32_t)((pl.nsym - start) < BLOCK ? (pl.nsym - start) : BLOCK) : 0u;
-    uint32_t n_min, n_max;
+    // wave-uniform step counts over all chains and halves
+    uint32_t n_min = n[0], n_max = n[0];
+#pragma unroll
+    for (int c = 1; c < CH; ++c) {
+      n_min = n[c] < n_min ? n[c] : n_min;
+      n_max = n[c] > n_max ? n[c] : n_max;
+    }
     if (L == 32) {
-      uint32_t n0 = __builtin_amdgcn_readlane(n, 0), n1 = __builtin_amdgcn_readlane(n, 32);
-      n_min = n0 < n1 ? n0 : n1;
-      n_max = n0 < n1 ? n1 : n0;
+      const uint32_t a0 = __builtin_amdgcn_readlane(n_min, 0), a1 = __builtin_amdgcn_readlane(n_min, 32);
+      const uint32_t b0 = __builtin_amdgcn_readlane(n_max, 0), b1 = __builtin_amdgcn_readlane(n_max, 32);
+      n_min = a0 < a1 ? a0 : a1;
+      n_max = b0 > b1 ? b0 : b1;
     } else {
-      n_min = n_max = __builtin_amdgcn_readfirstlane(n);
+      n_min = __builtin_amdgcn_readfirstlane(n_min);
+      n_max = __builtin_amdgcn_readfirstlane(n_max);
     }
-    const uint8_t* sp = pl.src + (start + lin) * S;
-    const uint4* tab = s_tab + k * kNumSymbols;
-    uint32_t x = kMinState;
-    uint32_t cnt = 0;  // words emitted by this lane's unit (uniform per half)
-
-    auto step = [&](uint32_t sym, bool v) {
-      const uint4 e = tab[sym];
-      const bool w = v && (x >= e.x);
-      const uint64_t m = __ballot(w);
-      const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)(m >> 32);
-      uint32_t base, add;
-      if (L == 64) {
-        base = cnt;
-        add = (uint32_t)__popcll(m);
-      } else {
-        base = half ? cnt - (uint32_t)__popc(mlo) : cnt;
-        add = half ? (uint32_t)__popc(mhi) : (uint32_t)__popc(mlo);
-      }
-      const uint32_t pos = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, base));
-      if (w) {
-        stage[pos] = (uint16_t)x;
-        x >>= 16;
-      }
-      cnt += add;
-      if (v) {
-        const uint32_t q = (__umulhi(x, e.y) + x) >> (e.z & 31u);
-        x = __umul24(q, e.w) + x + (e.z >> 8);
-      }
-    };
-
     const uint32_t full = n_min / L;
     const uint32_t all = (uint32_t)div_up(n_max, L);
-    uint32_t s = 0;
-    for (; s + 8 <= full; s += 8) {
-      uint32_t sy[8];
+
+    {
+      // One element per lane per step feeds all CH chains (byte c -> chain
+      // c).  Chain 0 of a ticket is plane 0 whenever CH > 1, so its plane
+      // pointer is the element array; with CH == 1 it is the chain's own
+      // byte stream (single streams, concatenated planes).
+      const uint8_t* eb = src[0] + blk[0] * BLOCK * S;  // element block start (4-byte aligned)
+      uint32_t s = 0;
+      // Full blocks: each lane loads the aligned dword holding its element
+      // and takes its bytes at a per-lane bit offset.  Two register buffers:
+      // while one batch is coded the next is loading.  (Whole dwords keep the
+      // compiler from narrowing the buffers and waiting on the next batch.)
+      const gptr<uint32_t> dp = (gptr<uint32_t>)eb + ((lin * S) >> 2);
+      const uint32_t sh = 8u * ((lin * S) & 3u);
+      auto load = [&](uint32_t (&buf)[kBatch], uint32_t s0) {
+        const gptr<uint32_t> bp = dp + (uint64_t)s0 * (L * S / 4);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) sy[j] = sp[(uint64_t)(s + j) * L * S];
+        for (int j = 0; j < kBatch; ++j) buf[j] = bp[j * (L * S / 4)];
+      };
+      auto code = [&](const uint32_t (&buf)[kBatch]) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) step(sy[j], true);
-    }
-    for (; s < full; ++s) step(sp[(uint64_t)s * L * S], true);
-    for (; s < all; ++s) {
-      const bool v = s * L + lin < n;
-      const uint32_t sym = v ? (uint32_t)sp[(uint64_t)s * L * S] : 0u;
-      step(sym, v);
+        for (int j = 0; j < kBatch; ++j) {
+#pragma unroll
+          for (int c = 0; c < CH; ++c)
+            enc_step<L>(x[c], cnt[c], stage[c], tab[c], __builtin_amdgcn_ubfe(buf[j], sh + 8u * c, 8u), true, half);
+        }
+      };
+      // The prefetch is unconditional (the last batch reloads itself) so the
+      // wait before each batch only covers that batch's loads.  Partial blocks
+      // (one per plane) take the byte path below, which never reads past the
+      // end of the input.
+      const uint32_t nbatch = n_min == BLOCK ? BLOCK / L / kBatch : 0u;
+      if (nbatch > 0) {
+        const uint32_t last = (nbatch - 1) * kBatch;
+        uint32_t bufA[kBatch], bufB[kBatch];
+        load(bufA, 0);
+        for (;;) {
+          load(bufB, s + kBatch <= last ? s + kBatch : last);
+          code(bufA);
+          s += kBatch;
+          if (s > last) break;
+          load(bufA, s + kBatch <= last ? s + kBatch : last);
+          code(bufB);
+          s += kBatch;
+          if (s > last) break;
+        }
+      }
+      for (; s < all; ++s) {
+        const uint32_t i = s * L + lin;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const bool v = i < n[c];
+          const uint32_t sym = v ? (uint32_t)eb[(uint64_t)i * S + c] : 0u;
+          enc_step<L>(x[c], cnt[c], stage[c], tab[c], sym, v, half);
+        }
+      }
     }
 
-    const uint32_t words = cnt;
-    const uint32_t padded = (uint32_t)round_up(words, 8);
-    if (words > STAGE_UNIT && lin == 0) atomicOr(a.err, kErrStageOverflow);
-    if (lin < padded - words) stage[words + lin] = 0;
-    uint8_t* sec = a.out + pl.section;
-    if (active) ((uint32_t*)(sec + kSectionFixed))[b * L + lin] = x;
+    // ---- per chain: pad, states; wave aggregate over the ticket's units ---
+    uint32_t padded[CH];
+    uint64_t agg = 0;
+    uint32_t a_lo[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      padded[c] = (uint32_t)round_up(cnt[c], 8);
+      if (cnt[c] > CSTAGE / HALVES && lin == 0) atomicOr(a.err, kErrStageOverflow);
+      if (lin < padded[c] - cnt[c]) stage[c][cnt[c] + lin] = 0;
+      if (act[c]) ((uint32_t*)(a.out + s_pl[kp[c]].section + kSectionFixed))[blk[c] * L + lin] = x[c];
+      a_lo[c] = __builtin_amdgcn_readlane(padded[c], 0);
+      agg += a_lo[c];
+      if (L == 32) agg += __builtin_amdgcn_readlane(padded[c], 32);
+    }
 
     // ---- decoupled look-back over tickets --------------------------------
     // Each status word carries its flag and its value together, and no other
     // memory is handed over through it, so relaxed device-scope atomics are
     // enough.  (Release / acquire at agent scope would write back / invalidate
     // the whole L2 of the XCD on every publish and poll.)
-    const uint32_t a_lo = __builtin_amdgcn_readlane(padded, 0);
-    const uint32_t a_hi = L == 32 ? __builtin_amdgcn_readlane(padded, 32) : 0u;
-    const uint64_t agg = (uint64_t)a_lo + a_hi;
     const uint64_t chain = tk / UPW;
     uint64_t excl = 0;
     if (lane == 0) {
@@ -183,18 +292,28 @@ __global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a) {
     }
     excl = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(excl >> 32)) << 32) |
            (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)excl);
-    const uint64_t y = excl + (half ? a_lo : 0u);
 
-    if (active && lin == 0) {
-      uint32_t* bw = (uint32_t*)(sec + kSectionFixed + 4ull * L * pl.nb + 8ull * b);
-      bw[0] = (n << 16) | words;
-      bw[1] = (uint32_t)y;
-      atomicAdd((uint32_t*)(sec + 12), padded);
+    // unit order inside the ticket: (half 0: chains 0..CH-1), (half 1: ...)
+    uint64_t y = excl;
+    if (half) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) y += a_lo[c];
     }
     __threadfence_block();
-    uint4* dst = (uint4*)(a.out + a.d.data_off + 2ull * y);
-    const uint4* src4 = (const uint4*)stage;
-    for (uint32_t j = lin; j < padded / 8; j += L) dst[j] = src4[j];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (act[c] && lin == 0) {
+        uint8_t* sec = a.out + s_pl[kp[c]].section;
+        uint32_t* bw = (uint32_t*)(sec + kSectionFixed + 4ull * L * s_pl[kp[c]].nb + 8ull * blk[c]);
+        bw[0] = (n[c] << 16) | cnt[c];
+        bw[1] = (uint32_t)y;
+        atomicAdd((uint32_t*)(sec + 12), padded[c]);
+      }
+      uint4* dst = (uint4*)(a.out + a.d.data_off + 2ull * y);
+      const uint4* src4 = (const uint4*)stage[c];
+      for (uint32_t j = lin; j < padded[c] / 8; j += L) dst[j] = src4[j];
+      y += padded[c];
+    }
 
     if ((uint64_t)tk + UPW >= a.d.total_units && lane == 0) {
       const uint64_t total_words = excl + agg;
@@ -210,31 +329,31 @@ __global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a) {
   }
 }
 
-template <int L, int PB, int S>
+template <int L, int PB, int S, int CH, int NW>
 hipError_t launch_t(const EncodeArgs& a, int cu, hipStream_t s) {
-  constexpr uint32_t STAGE = (uint32_t)round_up(64u * lane_max_words(128u, PB), 16u);
-  const size_t lds = (size_t)a.d.num_planes * kNumSymbols * sizeof(uint4) + (size_t)kWaves * STAGE * 2u;
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  const size_t lds = (size_t)a.d.num_planes * kNumSymbols * sizeof(uint4) + (size_t)NW * CH * CSTAGE * 2u;
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S>, kThreads, lds);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S, CH, NW>, 64 * NW, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
-  constexpr uint32_t UPW = L == 32 ? 2u : 1u;
+  constexpr uint32_t UPW = CH * (L == 32 ? 2u : 1u);
   const uint64_t waves = div_up(a.d.total_units, UPW);
-  uint64_t grid = div_up(waves, kWaves);
+  uint64_t grid = div_up(waves, NW);
   const uint64_t cap = (uint64_t)cu * (uint64_t)per_cu;
   if (grid > cap) grid = cap;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((encode_kernel<L, PB, S>), dim3((uint32_t)grid), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL((encode_kernel<L, PB, S, CH, NW>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
 template <int L, int PB>
 hipError_t launch_s(const EncodeArgs& a, int cu, hipStream_t s) {
-  switch (a.d.plane[0].stride) {
-    case 1: return launch_t<L, PB, 1>(a, cu, s);
-    case 2: return launch_t<L, PB, 2>(a, cu, s);
-    case 4: return launch_t<L, PB, 4>(a, cu, s);
-  }
+  const int P = a.d.num_planes;
+  const uint32_t S = a.d.plane[0].stride;
+  if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 2>(a, cu, s);
+  if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 1>(a, cu, s);
+  if (S == 1) return launch_t<L, PB, 1, 1, 4>(a, cu, s);
   return hipErrorInvalidValue;
 }
 

@@ -85,7 +85,7 @@ __device__ uint4 normalize_symbol(uint32_t count, uint32_t total, int p, uint32_
   } else if (diff < 0) {
     int d = -diff;
     while (d > 0) {
-      int g = __syncthreads_count(q > 1);
+      int g = (int)block_sum((uint32_t)__popcll(__ballot(q > 1)) * ((t & 63) == 0 ? 1u : 0u), s_tmp);
       int it = d < g ? d : g;
       if (it <= 0) { bad = true; break; }
       if (rank >= g - it && rank < g) q -= 1;
@@ -103,42 +103,29 @@ __device__ uint4 normalize_symbol(uint32_t count, uint32_t total, int p, uint32_
   return make_uint4(pdf, cdf, magic, shift);
 }
 
-// Per-thread run-length merging: a thread keeps (current byte, run length)
-// per plane and adds the run to LDS only when the byte changes.  Planes that
-// are nearly one value (high bytes of quantisation codes, ADM bit planes)
-// would otherwise send all 64 lanes of a wavefront to one LDS bin per step;
-// uniform planes pay two extra VALU ops per byte.
-template <int S>
-struct Runs {
-  uint32_t cur[S];
-  uint32_t cnt[S];
-};
+// Conflict-free counting.  Each wavefront owns one plane (wave w counts
+// plane w % P of its group's stripe of the input) and kRep lane-private
+// replicas laid out [bin][replica]: lane l adds into replica l % kRep, so the
+// 32 lanes of an LDS pass hit 32 distinct (bin * kRep + l % kRep) words and a
+// skewed plane (one or two byte values) never serialises on a bin.
+constexpr int kRep = 16;
+constexpr int kCountWords = kNumSymbols * kRep;   // per wave: 16 KiB
+constexpr int kLoads = 8;                         // 16-byte loads in flight per lane per buffer
 
 template <int S>
-__device__ __forceinline__ void run_add(uint32_t* h, Runs<S>& r, int p, uint32_t b) {
-  const bool flush = b != r.cur[p];
-  if (flush) atomicAdd(h + (p << 8) + r.cur[p], r.cnt[p]);  // adds 0 for the initial empty run
-  r.cnt[p] = flush ? 1u : r.cnt[p] + 1u;
-  r.cur[p] = b;
-}
-
-template <int S>
-__device__ __forceinline__ void count_word(uint32_t* h, Runs<S>& r, uint32_t v) {
-  // byte b of a 4-byte word belongs to plane b % S (elements are S-aligned)
-  run_add<S>(h, r, 0 % S, v & 0xffu);
-  run_add<S>(h, r, 1 % S, (v >> 8) & 0xffu);
-  run_add<S>(h, r, 2 % S, (v >> 16) & 0xffu);
-  run_add<S>(h, r, 3 % S, v >> 24);
-}
-
-template <int S>
-__device__ void count_source(const HistSource& src, uint32_t* h /* wave's [P][256] + plane0 */) {
-  Runs<S> r;
+__device__ __forceinline__ void count_u4(uint32_t* c, const uint4& v, uint32_t j0) {
+  // bytes j0, j0 + S, ... of each dword belong to this wave's plane
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int p = 0; p < S; ++p) {
-    r.cur[p] = 0;
-    r.cnt[p] = 0;
-  }
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int q = 0; q < 4 / S; ++q)
+      atomicAdd(c + __builtin_amdgcn_ubfe(w[k], 8u * (j0 + (uint32_t)(q * S)), 8u) * kRep, 1u);
+}
+
+template <int S>
+__device__ void count_source(const HistSource& src, uint32_t* c /* lane's replica base */, uint32_t j0,
+                             uint64_t slot, uint64_t nslots, bool edges) {
   const uint8_t* p = src.ptr;
   const uint64_t n = src.nbytes;
   const uint32_t mis = (uint32_t)((uintptr_t)p & 15u);
@@ -146,38 +133,60 @@ __device__ void count_source(const HistSource& src, uint32_t* h /* wave's [P][25
   const uint64_t n4 = (n - head) / 16u;
   const uint64_t tail0 = head + 16u * n4;
   const uint4* body = (const uint4*)(p + head);
-  const uint64_t tid = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  const uint64_t nthreads = (uint64_t)gridDim.x * kThreads;
-  uint64_t i = tid;
-  for (; i + nthreads < n4; i += 2 * nthreads) {
-    uint4 a = body[i];
-    uint4 b = body[i + nthreads];
-    count_word<S>(h, r, a.x); count_word<S>(h, r, a.y); count_word<S>(h, r, a.z); count_word<S>(h, r, a.w);
-    count_word<S>(h, r, b.x); count_word<S>(h, r, b.y); count_word<S>(h, r, b.z); count_word<S>(h, r, b.w);
-  }
-  if (i < n4) {
-    uint4 a = body[i];
-    count_word<S>(h, r, a.x); count_word<S>(h, r, a.y); count_word<S>(h, r, a.z); count_word<S>(h, r, a.w);
-  }
+  const uint32_t lane = threadIdx.x & 63;
+  // lane-strided rows of kLoads uint4 per lane; two buffers so one row loads
+  // while the other is counted
+  const uint64_t row = 64ull * kLoads;
+  uint64_t r0 = slot * row + lane;
+  const uint64_t rstep = nslots * row;
+  uint4 bufA[kLoads], bufB[kLoads];
+  // loads are unconditional (indices past the end re-read the last uint4 and
+  // are not counted) so the wait before counting a row covers only that row
+  auto load = [&](uint4 (&buf)[kLoads], uint64_t base) {
 #pragma unroll
-  for (int p = 0; p < S; ++p)
-    if (r.cnt[p]) atomicAdd(h + (p << 8) + r.cur[p], r.cnt[p]);
-  if (blockIdx.x == 0) {
-    // unaligned head (< 16 bytes) and tail (< 16 bytes); offsets from the
+    for (int k = 0; k < kLoads; ++k) {
+      const uint64_t i = base + 64ull * k;
+      buf[k] = body[i < n4 ? i : n4 - 1];
+    }
+  };
+  auto count = [&](const uint4 (&buf)[kLoads], uint64_t base) {
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k)
+      if (base + 64ull * k < n4) count_u4<S>(c, buf[k], j0);
+  };
+  if (slot * row < n4) {
+    load(bufA, r0);
+    for (;;) {
+      const uint64_t r1 = r0 + rstep;
+      load(bufB, r1);
+      count(bufA, r0);
+      if (r1 - lane >= n4) break;
+      const uint64_t r2 = r1 + rstep;
+      load(bufA, r2);
+      count(bufB, r1);
+      if (r2 - lane >= n4) break;
+      r0 = r2;
+    }
+  }
+  if (edges) {
+    // unaligned head and tail (< 16 bytes each); offsets from the
     // element-aligned source start give the plane
-    if (threadIdx.x < head) atomicAdd(h + ((threadIdx.x % S) << 8) + p[threadIdx.x], 1u);
-    uint64_t j = tail0 + threadIdx.x;
-    if (threadIdx.x < 16 && j < n) atomicAdd(h + ((j % S) << 8) + p[j], 1u);
+    if (lane < head && lane % S == j0) atomicAdd(c + p[lane] * kRep, 1u);
+    const uint64_t j = tail0 + lane;
+    if (lane < 16 && j < n && j % S == j0) atomicAdd(c + p[j] * kRep, 1u);
   }
 }
 
 __global__ __launch_bounds__(kThreads) void hist_kernel(HistArgs a) {
-  __shared__ uint32_t s_hist[kWaves][kMaxPlanes][kNumSymbols];
-  __shared__ uint32_t s_keys[kNumSymbols];
-  __shared__ uint32_t s_tmp[kWaves];
-  __shared__ int s_last;
-  const int t = threadIdx.x;
+  // 64 KiB: kWaves x [256][kRep] counters; the finisher reuses the front
+  extern __shared__ uint32_t smem[];
+  uint32_t* s_keys = smem;                 // [256]  (finisher)
+  uint32_t* s_tmp = smem + kNumSymbols;    // [kWaves]
+  int* s_last = (int*)(smem + kNumSymbols + kWaves);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int P = a.num_planes;
+  const int groups = kWaves / P;           // P divides kWaves (1, 2 or 4)
+  const int plane = w % P, group = w / P;
 
   // Clear the encode launch's look-back words (the previous user of this
   // workspace has finished: kernels on one stream run in order).
@@ -185,30 +194,38 @@ __global__ __launch_bounds__(kThreads) void hist_kernel(HistArgs a) {
        i += (uint64_t)gridDim.x * kThreads)
     a.status[i] = 0;
 
-  for (int w = 0; w < kWaves; ++w)
-    for (int k = 0; k < P; ++k) s_hist[w][k][t] = 0;
-  __syncthreads();
-
-  uint32_t* wh = &s_hist[t >> 6][0][0];
+  uint32_t* wc = smem + w * kCountWords;
+  for (int i = lane; i < kCountWords / 4; i += 64) ((uint4*)wc)[i] = make_uint4(0, 0, 0, 0);
+  __threadfence_block();
+  uint32_t* c = wc + (lane % kRep);
+  const uint64_t slot = (uint64_t)blockIdx.x * groups + group;
+  const uint64_t nslots = (uint64_t)gridDim.x * groups;
   for (int s = 0; s < a.nsrc; ++s) {
     const HistSource& src = a.src[s];
-    uint32_t* h = wh + src.plane0 * kNumSymbols;
-    if (src.stride == 1) count_source<1>(src, h);
-    else if (src.stride == 2) count_source<2>(src, h);
-    else count_source<4>(src, h);
+    if (plane < (int)src.plane0 || plane >= (int)(src.plane0 + src.stride)) continue;
+    const uint32_t j0 = (uint32_t)plane - src.plane0;
+    const bool edges = blockIdx.x == 0 && group == 0;
+    if (src.stride == 1) count_source<1>(src, c, j0, slot, nslots, edges);
+    else if (src.stride == 2) count_source<2>(src, c, j0, slot, nslots, edges);
+    else count_source<4>(src, c, j0, slot, nslots, edges);
   }
-  __syncthreads();
-  for (int k = 0; k < P; ++k) {
+  __threadfence_block();
+  // merge the wave's replicas and add to the device histogram of its plane
+  for (int b = lane; b < (int)kNumSymbols; b += 64) {
+    const uint4* r4 = (const uint4*)(wc + b * kRep);
     uint32_t v = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) v += s_hist[w][k][t];
-    if (v) atomicAdd(a.hist + k * kNumSymbols + t, v);
+    for (int k = 0; k < kRep / 4; ++k) {
+      const uint4 q = r4[k];
+      v += q.x + q.y + q.z + q.w;
+    }
+    if (v) atomicAdd(a.hist + plane * kNumSymbols + b, v);
   }
   __threadfence();
   __syncthreads();
-  if (t == 0) s_last = atomicAdd(a.done, 1u) == gridDim.x - 1;
+  if (t == 0) *s_last = atomicAdd(a.done, 1u) == gridDim.x - 1;
   __syncthreads();
-  if (!s_last) return;
+  if (!*s_last) return;
   __threadfence();
 
   // ---- finisher: normalise each plane, write tables and headers ----------
@@ -263,11 +280,13 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
 
 hipError_t launch_hist(const HistArgs& a, int cu, hipStream_t s) {
   uint64_t n4 = 0;
-  for (int i = 0; i < a.nsrc; ++i) n4 += a.src[i].nbytes / 16u;
-  uint64_t want = div_up(n4, 2ull * kThreads);
-  uint64_t cap = (uint64_t)cu * 8u;
-  uint32_t grid = (uint32_t)(want < 1 ? 1 : (want > cap ? cap : want));
-  hipLaunchKernelGGL(hist_kernel, dim3(grid), dim3(kThreads), 0, s, a);
+  for (int i = 0; i < a.nsrc; ++i) n4 = n4 > a.src[i].nbytes / 16u ? n4 : a.src[i].nbytes / 16u;
+  const uint64_t groups = (uint64_t)kWaves / (uint64_t)a.num_planes;
+  const uint64_t want = div_up(n4, 64ull * kLoads * groups);
+  const uint64_t cap = (uint64_t)cu * 2u;  // two 64 KiB workgroups per CU
+  const uint32_t grid = (uint32_t)(want < 1 ? 1 : (want > cap ? cap : want));
+  const size_t lds = (size_t)kWaves * kCountWords * sizeof(uint32_t);
+  hipLaunchKernelGGL(hist_kernel, dim3(grid), dim3(kThreads), lds, s, a);
   return hipGetLastError();
 }
 
