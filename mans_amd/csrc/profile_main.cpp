@@ -81,9 +81,36 @@ int main(int argc, char** argv) {
   }
   std::vector<uint8_t> header(128);
   CHECK_HIP(hipMemcpy(header.data(), d_enc, 128, hipMemcpyDeviceToHost));
-  for (int k = 1; k < steps; ++k) mans_frame_encode(&p, d_in, n, d_enc, cap, d_size, nullptr, ws, s);
+  // one untimed decode, then K encodes and K decodes with per-launch events
+  mans_frame_decode(d_enc, csize, header.data(), d_dec, bytes, nullptr, ws, s);
+  CHECK_HIP(hipStreamSynchronize(s));
+  mans_set_profiling(ws, 1);
+  mans_last_kernel_times(ws, nullptr, nullptr, 0);
+  for (int k = 0; k < steps; ++k) mans_frame_encode(&p, d_in, n, d_enc, cap, d_size, nullptr, ws, s);
   for (int k = 0; k < steps; ++k) mans_frame_decode(d_enc, csize, header.data(), d_dec, bytes, nullptr, ws, s);
   CHECK_HIP(hipStreamSynchronize(s));
+  {
+    std::vector<double> ms(4096);
+    std::vector<const char*> names(4096);
+    const int got = mans_last_kernel_times(ws, ms.data(), names.data(), (int)ms.size());
+    mans_set_profiling(ws, 0);
+    std::vector<std::string> order;
+    std::vector<double> sum;
+    std::vector<int> cnt;
+    for (int i = 0; i < got; ++i) {
+      size_t j = 0;
+      while (j < order.size() && order[j] != names[i]) ++j;
+      if (j == order.size()) {
+        order.push_back(names[i]);
+        sum.push_back(0.0);
+        cnt.push_back(0);
+      }
+      sum[j] += ms[i];
+      cnt[j] += 1;
+    }
+    for (size_t j = 0; j < order.size(); ++j)
+      printf("kernel %-16s launches %3d avg_ms %.4f\n", order[j].c_str(), cnt[j], sum[j] / cnt[j]);
+  }
   if (mans_workspace_check(ws, s, nullptr) != MANS_OK) {
     fprintf(stderr, "decode flagged a corrupt stream\n");
     return 3;
