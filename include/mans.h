@@ -137,6 +137,39 @@ mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header
 mans_status mans_frame_info(const void* h_header, uint64_t header_bytes, uint64_t* n_elems,
                             int32_t* elem_bytes, int32_t* mode, uint64_t* total_size);
 
+/* ---- batches of independent buffers ------------------------------------ */
+/* Many buffers of uneven size and element width, each coded as its own
+ * byte-plane frame in one persistent launch.  Frame i is byte-identical to
+ * mans_frame_encode of buffer i alone with the same prob_bits, 64 lanes,
+ * elem_bytes = bufs[i].elem_bytes and mode MANS_MODE_PLANES.  The reference
+ * has no batch entry point (its DietGPU-style multi-input vestiges,
+ * GpuANSStatistics.h:138-151 and GpuANSDecode.h:202-214, are fixed at one
+ * input); this replaces calling ansEncodeBatch (src/GpuANS.cu:16) once per
+ * buffer.  A batch is planned once (create: validates, lays out every frame,
+ * uploads descriptors) and run any number of times; run only launches, so it
+ * can be captured in a HIP graph.  Runs sharing a workspace must be
+ * stream-ordered. */
+typedef struct mans_buffer {
+  const void* d_in;      /* elements, 16-byte aligned */
+  uint64_t n_elems;
+  void* d_out;           /* frame, 16-byte aligned */
+  uint64_t out_capacity; /* >= mans_frame_max_size(n_elems, {p, 64, elem_bytes}) */
+  int32_t elem_bytes;    /* 1, 2 or 4 */
+  int32_t reserved;      /* 0 */
+} mans_buffer;
+
+typedef struct mans_batch mans_batch;
+
+/* p->lanes must be 64; p->elem_bytes and p->mode are ignored. */
+mans_status mans_batch_encode_create(mans_batch** b, const mans_params* p, const mans_buffer* h_bufs,
+                                     int32_t count, mans_workspace* ws);
+/* d_sizes: device array of count uint64 frame sizes, written by the launch. */
+mans_status mans_batch_encode_run(mans_batch* b, uint64_t* d_sizes, void* stream);
+mans_status mans_batch_destroy(mans_batch* b);
+/* create + run + destroy (synchronises the stream before freeing). */
+mans_status mans_encode_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, uint64_t* d_sizes,
+                              mans_workspace* ws, void* stream);
+
 /* ---- profiling --------------------------------------------------------- */
 /* When enabled, every call records HIP events on its stream around each of
  * its kernels.  mans_last_kernel_times waits for them and returns the

@@ -33,7 +33,13 @@ class Params(C.Structure):
     _fields_ = [("prob_bits", C.c_int32), ("lanes", C.c_int32), ("elem_bytes", C.c_int32), ("mode", C.c_int32)]
 
 
+class Buffer(C.Structure):
+    _fields_ = [("d_in", C.c_void_p), ("n_elems", C.c_uint64), ("d_out", C.c_void_p),
+                ("out_capacity", C.c_uint64), ("elem_bytes", C.c_int32), ("reserved", C.c_int32)]
+
+
 _P = C.POINTER(Params)
+_B = C.POINTER(Buffer)
 SIGNATURES = {
     "mans_status_string": (C.c_char_p, [C.c_int]),
     "mans_abi_version": (_i32, []),
@@ -52,6 +58,10 @@ SIGNATURES = {
     "mans_frame_encode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, C.POINTER(_u64), _vp, _vp]),
     "mans_frame_decode": (C.c_int, [_vp, _u64, _vp, _vp, _u64, C.POINTER(_u64), _vp, _vp]),
     "mans_frame_info": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64)]),
+    "mans_batch_encode_create": (C.c_int, [C.POINTER(_vp), _P, _B, _i32, _vp]),
+    "mans_batch_encode_run": (C.c_int, [_vp, _vp, _vp]),
+    "mans_batch_destroy": (C.c_int, [_vp]),
+    "mans_encode_batch": (C.c_int, [_P, _B, _i32, _vp, _vp, _vp]),
     "mans_set_profiling": (C.c_int, [_vp, _i32]),
     "mans_last_kernel_times": (_i32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_char_p), _i32]),
 }
@@ -258,3 +268,47 @@ def frame_decode(comp, size: int, dtype, n: int, ws: Workspace, out=None, stream
                                  C.byref(nout), ws.handle, _stream(stream))
     _check(rc, "mans_frame_decode")
     return out[:n]
+
+
+class EncodeBatch:
+    """A planned batch of independent buffers, each coded as a byte-plane
+    frame (mans_batch_encode_create).  `inputs`: CUDA tensors of 1/2/4-byte
+    elements (16-byte aligned storage); frames go to `outs` (allocated at
+    frame_max_size when not given).  run() only launches, so it can be
+    captured in a CUDA/HIP graph; sizes are left on the device."""
+
+    def __init__(self, inputs, p: Params, ws: Workspace, outs=None):
+        torch = _torch()
+        self.inputs = list(inputs)
+        self.ws = ws
+        if outs is None:
+            outs = []
+            for x in self.inputs:
+                q = Params(p.prob_bits, 64, x.element_size(), MODE_PLANES)
+                outs.append(torch.empty(frame_max_size(x.numel(), q), dtype=torch.uint8, device=x.device))
+        self.outs = list(outs)
+        arr = (Buffer * max(1, len(self.inputs)))()
+        for i, (x, o) in enumerate(zip(self.inputs, self.outs)):
+            arr[i] = Buffer(_ptr(x), x.numel(), o.data_ptr(), o.numel(), x.element_size(), 0)
+        self.sizes = torch.zeros(max(1, len(self.inputs)), dtype=torch.int64,
+                                 device=self.inputs[0].device if self.inputs else "cuda")
+        h = C.c_void_p()
+        _check(lib().mans_batch_encode_create(C.byref(h), C.byref(p), arr, len(self.inputs), ws.handle),
+               "mans_batch_encode_create")
+        self.handle = h
+
+    def run(self, stream=None):
+        _check(lib().mans_batch_encode_run(self.handle, self.sizes.data_ptr(), _stream(stream)),
+               "mans_batch_encode_run")
+        return self.sizes
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().mans_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

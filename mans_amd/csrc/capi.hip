@@ -25,6 +25,8 @@ struct mans_workspace {
   uint64_t status_cap = 0;
   uint8_t* scratch = nullptr;    // ADM planes of frames
   uint64_t scratch_cap = 0;
+  uint16_t* spill = nullptr;     // encoder spill areas (one per resident wave)
+  uint64_t spill_cap = 0;
   uint64_t* d_u64 = nullptr;     // [4]
   uint64_t* h_pinned = nullptr;  // [32] host pinned
   bool profiling = false;
@@ -83,6 +85,31 @@ mans_status ensure_scratch(mans_workspace* ws, uint64_t n) {
   return MANS_OK;
 }
 
+mans_status ensure_spill(mans_workspace* ws, uint64_t bytes) {
+  if (bytes <= ws->spill_cap) return MANS_OK;
+  if (ws->spill) (void)hipFree(ws->spill);
+  ws->spill = nullptr;
+  ws->spill_cap = 0;
+  HIP_TRY(hipMalloc(&ws->spill, bytes));
+  ws->spill_cap = bytes;
+  return MANS_OK;
+}
+
+// Spill area the encoder needs on a large input of P planes of stride S.
+uint64_t spill_for(mans_workspace* ws, int lanes, int pb, int P, uint32_t S, int order) {
+  EncodeArgs e;
+  memset(&e, 0, sizeof(e));
+  e.d.lanes = lanes;
+  e.d.prob_bits = pb;
+  e.d.num_planes = P;
+  e.d.order = order;
+  for (int k = 0; k < P; ++k) e.d.plane[k].stride = S;
+  e.d.total_units = 1ull << 30;
+  uint64_t bytes = 0;
+  if (encode_spill_bytes(e, ws->cu, &bytes) != hipSuccess) return 0;
+  return bytes;
+}
+
 // Profiling: events recorded around launches when enabled.
 void prof_mark(mans_workspace* ws, hipStream_t s, const char* name) {
   if (!ws->profiling) return;
@@ -116,6 +143,31 @@ uint64_t adm_frame_max(uint64_t n, int pb, Format f) {
 void put64(uint32_t* w, int byte_off, uint64_t v) {
   w[byte_off / 4] = (uint32_t)v;
   w[byte_off / 4 + 1] = (uint32_t)(v >> 32);
+}
+
+// The 128-byte frame header of n elements of eb bytes coded as `d`.
+void frame_header(uint32_t (&hdr)[32], uint64_t n, uint32_t eb, int mode, Format f, int prob_bits,
+                  const CodecDesc& d, uint64_t gsize, uint64_t meta_off) {
+  memset(hdr, 0, sizeof(hdr));
+  uint8_t* hb = (uint8_t*)hdr;
+  hdr[0] = kFrameMagic;
+  hb[4] = (uint8_t)kFrameVersion;
+  hb[5] = 0;
+  hb[6] = (uint8_t)eb;
+  hb[7] = (uint8_t)mode;
+  hb[8] = (uint8_t)f.lanes;
+  hb[9] = (uint8_t)prob_bits;
+  hb[10] = (uint8_t)d.num_planes;
+  hb[11] = (uint8_t)d.order;
+  hdr[3] = f.block_bytes;
+  put64(hdr, 16, n);
+  put64(hdr, 24, gsize);
+  put64(hdr, 32, meta_off);
+  put64(hdr, 40, d.data_off);
+  for (int k = 0; k < d.num_planes; ++k) {
+    put64(hdr, 64 + 8 * k, d.plane[k].section);
+    put64(hdr, 96 + 8 * k, d.plane[k].nsym);
+  }
 }
 
 // Shared by mans_encode and mans_frame_encode: one hist+normalise launch and
@@ -169,6 +221,12 @@ mans_status run_encode(mans_workspace* ws, const CodecDesc& d, const HistSource*
   e.d_out_size32 = d_out_size32;
   e.d_out_size64 = d_out_size64;
   e.frame = frame_hdr ? 1 : 0;
+  uint64_t spill = 0;
+  HIP_TRY(encode_spill_bytes(e, ws->cu, &spill));
+  st = ensure_spill(ws, spill);
+  if (st != MANS_OK) return st;
+  e.spill = ws->spill;
+  e.spill_bytes = ws->spill_cap;
   prof_mark(ws, s, "encode");
   HIP_TRY(launch_encode(e, ws->cu, s));
   prof_mark(ws, s, nullptr);
@@ -245,6 +303,7 @@ mans_status mans_workspace_destroy(mans_workspace* ws) {
   if (ws->hist_out) (void)hipFree(ws->hist_out);
   if (ws->status) (void)hipFree(ws->status);
   if (ws->scratch) (void)hipFree(ws->scratch);
+  if (ws->spill) (void)hipFree(ws->spill);
   if (ws->d_u64) (void)hipFree(ws->d_u64);
   if (ws->h_pinned) (void)hipHostFree(ws->h_pinned);
   for (hipEvent_t e : ws->ev) (void)hipEventDestroy(e);
@@ -263,6 +322,15 @@ mans_status mans_workspace_reserve(mans_workspace* ws, uint64_t max_bytes, const
   if (p && p->mode != MANS_MODE_PLANES && p->elem_bytes == 2) {
     const uint64_t n = max_bytes / 2;
     st = ensure_scratch(ws, round_up(n, 16) + adm_bits_cap(n) + 16);
+    if (st != MANS_OK) return st;
+  }
+  if (p && lanes == 64 && check_params(p) == MANS_OK) {
+    // the encoder's spill areas for this element width (byte planes) and for
+    // single byte streams (single streams, ADM planes)
+    const int eb = p->elem_bytes == 4 ? 4 : (p->elem_bytes == 2 ? 2 : 1);
+    const uint64_t need = std::max(spill_for(ws, 64, p->prob_bits, eb, (uint32_t)eb, 0),
+                                   spill_for(ws, 64, p->prob_bits, 1, 1u, 1));
+    st = ensure_spill(ws, need);
   }
   return st;
 }
@@ -568,26 +636,7 @@ mans_status mans_frame_encode(const mans_params* p, const void* d_in, uint64_t n
   }
   d.data_off = off;
   uint32_t hdr[32];
-  memset(hdr, 0, sizeof(hdr));
-  uint8_t* hb = (uint8_t*)hdr;
-  hdr[0] = kFrameMagic;
-  hb[4] = (uint8_t)kFrameVersion;
-  hb[5] = 0;
-  hb[6] = (uint8_t)eb;
-  hb[7] = (uint8_t)mode;
-  hb[8] = (uint8_t)f.lanes;
-  hb[9] = (uint8_t)p->prob_bits;
-  hb[10] = (uint8_t)d.num_planes;
-  hb[11] = (uint8_t)d.order;
-  hdr[3] = f.block_bytes;
-  put64(hdr, 16, n);
-  put64(hdr, 24, gsize);
-  put64(hdr, 32, meta_off);
-  put64(hdr, 40, d.data_off);
-  for (int k = 0; k < d.num_planes; ++k) {
-    put64(hdr, 64 + 8 * k, d.plane[k].section);
-    put64(hdr, 96 + 8 * k, d.plane[k].nsym);
-  }
+  frame_header(hdr, n, eb, mode, f, p->prob_bits, d, gsize, meta_off);
   uint64_t* dsz = d_size ? d_size : (h_size ? ws->d_u64 : nullptr);
   st = run_encode(ws, d, src, nsrc, out, nullptr, dsz, hdr, s);
   if (st != MANS_OK) return st;
@@ -707,6 +756,205 @@ mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header
   }
   prof_mark(ws, s, nullptr);
   return MANS_OK;
+}
+
+// ---- batches -----------------------------------------------------------------
+
+}  // extern "C"
+
+struct mans_batch {
+  mans_workspace* ws = nullptr;
+  int pb = 10;
+  uint32_t nbufs = 0, nhist = 0, nitems = 0, nticks = 0;
+  BatchBuf* bufs = nullptr;   // device
+  uint32_t* item_buf = nullptr;
+  uint32_t* hist = nullptr;   // [nbufs][4][256]
+  uint32_t* done = nullptr;   // [nbufs]
+  uint32_t* ready = nullptr;  // [nbufs]
+  uint4* tables = nullptr;    // [nbufs][4][256]
+  uint64_t* status = nullptr; // [max(nticks, 1)]
+  uint32_t* ptot = nullptr;   // [nbufs][4]
+  uint32_t* fin = nullptr;    // [nbufs]
+  uint32_t* queue = nullptr;  // [3] next item, exited waves, generation
+  uint64_t spill = 0;
+};
+
+namespace {
+
+void batch_free(mans_batch* b) {
+  if (b->bufs) (void)hipFree(b->bufs);
+  if (b->item_buf) (void)hipFree(b->item_buf);
+  if (b->hist) (void)hipFree(b->hist);
+  if (b->done) (void)hipFree(b->done);
+  if (b->ready) (void)hipFree(b->ready);
+  if (b->tables) (void)hipFree(b->tables);
+  if (b->status) (void)hipFree(b->status);
+  if (b->ptot) (void)hipFree(b->ptot);
+  if (b->fin) (void)hipFree(b->fin);
+  if (b->queue) (void)hipFree(b->queue);
+  delete b;
+}
+
+BatchEncodeArgs batch_args(const mans_batch* b, uint64_t* d_sizes) {
+  BatchEncodeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.bufs = b->bufs;
+  a.item_buf = b->item_buf;
+  a.nbufs = b->nbufs;
+  a.nhist_items = b->nhist;
+  a.nitems = b->nitems;
+  a.hist = b->hist;
+  a.done = b->done;
+  a.ready = b->ready;
+  a.tables = b->tables;
+  a.status = b->status;
+  a.ptot = b->ptot;
+  a.fin = b->fin;
+  a.queue = b->queue;
+  a.err = b->ws->counters + 3;
+  a.sizes = d_sizes;
+  a.spill = b->ws->spill;
+  a.spill_bytes = b->ws->spill_cap;
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+mans_status mans_batch_encode_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs,
+                                     int32_t count, mans_workspace* ws) {
+  if (!out) return MANS_ERR_INVALID_ARG;
+  *out = nullptr;
+  mans_status st = check_params(p);
+  if (st != MANS_OK) return st;
+  if (!ws || count < 0 || (count > 0 && !h_bufs) || p->lanes != 64) return MANS_ERR_INVALID_ARG;
+  const Format f = format_for_lanes(64);
+  std::vector<BatchBuf> bufs((size_t)count);
+  std::vector<uint32_t> item_buf;
+  uint64_t nhist = 0, nticks = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    const mans_buffer& m = h_bufs[i];
+    const uint32_t eb = (uint32_t)m.elem_bytes;
+    if (eb != 1 && eb != 2 && eb != 4) return MANS_ERR_INVALID_ARG;
+    if (!m.d_out || (m.n_elems > 0 && !m.d_in)) return MANS_ERR_INVALID_ARG;
+    if (!aligned(m.d_in, 16) || !aligned(m.d_out, 16)) return MANS_ERR_ALIGNMENT;
+    if (m.n_elems > 0xffffffffull) return MANS_ERR_TOO_LARGE;
+    mans_params q = *p;
+    q.elem_bytes = (int32_t)eb;
+    q.mode = MANS_MODE_PLANES;
+    if (m.out_capacity < mans_frame_max_size(m.n_elems, &q)) return MANS_ERR_CAPACITY;
+    CodecDesc d;
+    memset(&d, 0, sizeof(d));
+    d.prob_bits = p->prob_bits;
+    d.lanes = 64;
+    d.num_planes = (int)eb;
+    d.order = 0;
+    const uint64_t nb = div_up(m.n_elems, f.block_bytes);
+    uint64_t off = kFrameHeaderBytes;
+    for (uint32_t k = 0; k < eb; ++k) {
+      d.plane[k] = PlaneDesc{(const uint8_t*)m.d_in + k, nullptr, m.n_elems, nb, off, 0, eb, 0};
+      off += section_bytes(nb, f);
+    }
+    d.data_off = off;
+    BatchBuf& B = bufs[(size_t)i];
+    memset(&B, 0, sizeof(B));
+    B.in = (const uint8_t*)m.d_in;
+    B.out = (uint8_t*)m.d_out;
+    B.n = m.n_elems;
+    B.nb = nb;
+    for (uint32_t k = 0; k < eb; ++k) B.section[k] = d.plane[k].section;
+    B.data_off = off;
+    B.eb = eb;
+    B.hist_first = (uint32_t)nhist;
+    B.nhist = (uint32_t)std::max<uint64_t>(1, div_up(m.n_elems * eb, kBatchHistChunk));
+    B.tick_first = (uint32_t)nticks;
+    B.nticks = (uint32_t)div_up(eb * nb, 2);
+    frame_header(B.frame_hdr, m.n_elems, eb, MANS_MODE_PLANES, f, p->prob_bits, d, 0, 0);
+    put64(B.frame_hdr, 56, off);  // total size of an empty frame
+    nhist += B.nhist;
+    nticks += B.nticks;
+    if (nhist + nticks > 0x7fffffffull) return MANS_ERR_TOO_LARGE;
+  }
+  item_buf.reserve(nhist + nticks);
+  for (int32_t i = 0; i < count; ++i)
+    for (uint32_t k = 0; k < bufs[(size_t)i].nhist; ++k) item_buf.push_back((uint32_t)i);
+  for (int32_t i = 0; i < count; ++i)
+    for (uint32_t k = 0; k < bufs[(size_t)i].nticks; ++k) item_buf.push_back((uint32_t)i);
+
+  DeviceGuard g(ws->device);
+  mans_batch* b = new mans_batch();
+  b->ws = ws;
+  b->pb = p->prob_bits;
+  b->nbufs = (uint32_t)count;
+  b->nhist = (uint32_t)nhist;
+  b->nticks = (uint32_t)nticks;
+  b->nitems = (uint32_t)(nhist + nticks);
+  const size_t nb1 = std::max<size_t>(1, (size_t)count);
+  auto fail = [&](mans_status s) {
+    batch_free(b);
+    return s;
+  };
+  if (hipMalloc(&b->bufs, nb1 * sizeof(BatchBuf)) != hipSuccess ||
+      hipMalloc(&b->item_buf, std::max<size_t>(1, item_buf.size()) * 4) != hipSuccess ||
+      hipMalloc(&b->hist, nb1 * 4096) != hipSuccess || hipMalloc(&b->done, nb1 * 4) != hipSuccess ||
+      hipMalloc(&b->ready, nb1 * 4) != hipSuccess || hipMalloc(&b->tables, nb1 * 4 * 256 * sizeof(uint4)) != hipSuccess ||
+      hipMalloc(&b->status, std::max<uint64_t>(1, nticks) * 8) != hipSuccess || hipMalloc(&b->queue, 12) != hipSuccess ||
+      hipMalloc(&b->ptot, nb1 * 16) != hipSuccess || hipMalloc(&b->fin, nb1 * 4) != hipSuccess)
+    return fail(MANS_ERR_HIP);
+  if (count > 0 && hipMemcpy(b->bufs, bufs.data(), bufs.size() * sizeof(BatchBuf), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(MANS_ERR_HIP);
+  if (!item_buf.empty() &&
+      hipMemcpy(b->item_buf, item_buf.data(), item_buf.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(MANS_ERR_HIP);
+  if (hipMemset(b->hist, 0, nb1 * 4096) != hipSuccess || hipMemset(b->done, 0, nb1 * 4) != hipSuccess ||
+      hipMemset(b->ready, 0, nb1 * 4) != hipSuccess ||
+      hipMemset(b->status, 0, std::max<uint64_t>(1, nticks) * 8) != hipSuccess ||
+      hipMemset(b->queue, 0, 12) != hipSuccess || hipMemset(b->ptot, 0, nb1 * 16) != hipSuccess ||
+      hipMemset(b->fin, 0, nb1 * 4) != hipSuccess)
+    return fail(MANS_ERR_HIP);
+  const uint32_t gen1 = 1;  // generation of the first launch (0 marks nothing published)
+  if (hipMemcpy(b->queue + 2, &gen1, 4, hipMemcpyHostToDevice) != hipSuccess) return fail(MANS_ERR_HIP);
+  if (b->nitems > 0) {
+    if (launch_batch_encode(batch_args(b, nullptr), b->pb, ws->cu, nullptr, &b->spill) != hipSuccess)
+      return fail(MANS_ERR_HIP);
+    st = ensure_spill(ws, b->spill);
+    if (st != MANS_OK) return fail(st);
+  }
+  *out = b;
+  return MANS_OK;
+}
+
+mans_status mans_batch_encode_run(mans_batch* b, uint64_t* d_sizes, void* stream) {
+  if (!b || (b->nbufs > 0 && !d_sizes)) return MANS_ERR_INVALID_ARG;
+  if (b->nitems == 0) return MANS_OK;
+  mans_workspace* ws = b->ws;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  if (b->spill > ws->spill_cap) return MANS_ERR_INVALID_ARG;  // the workspace was replaced under the batch
+  prof_mark(ws, s, "batch_encode");
+  HIP_TRY(launch_batch_encode(batch_args(b, d_sizes), b->pb, ws->cu, s, nullptr));
+  prof_mark(ws, s, nullptr);
+  return MANS_OK;
+}
+
+mans_status mans_batch_destroy(mans_batch* b) {
+  if (!b) return MANS_OK;
+  DeviceGuard g(b->ws->device);
+  (void)hipDeviceSynchronize();
+  batch_free(b);
+  return MANS_OK;
+}
+
+mans_status mans_encode_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, uint64_t* d_sizes,
+                              mans_workspace* ws, void* stream) {
+  mans_batch* b = nullptr;
+  mans_status st = mans_batch_encode_create(&b, p, h_bufs, count, ws);
+  if (st != MANS_OK) return st;
+  st = mans_batch_encode_run(b, d_sizes, stream);
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess && st == MANS_OK) st = MANS_ERR_HIP;
+  mans_batch_destroy(b);
+  return st;
 }
 
 mans_status mans_set_profiling(mans_workspace* ws, int32_t enable) {

@@ -1,0 +1,254 @@
+// codec_device.hpp -- device-side pieces shared by the single-buffer and
+// batched rANS kernels: the per-symbol encode and decode steps, the
+// decoupled look-back, and the staging bound.
+#pragma once
+#include "mans_internal.hpp"
+
+namespace mans {
+
+template <int S> struct ElemOf;
+template <> struct ElemOf<1> { using T = uint8_t; };
+template <> struct ElemOf<2> { using T = uint16_t; };
+template <> struct ElemOf<4> { using T = uint32_t; };
+
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagPrefix = 2ull << 62;
+constexpr uint64_t kValueMask = (1ull << 62) - 1;
+
+// Words per chain per wavefront of one unit at most: 64 lanes x 128 symbols
+// (one native unit, or two 32-lane reference units, one per half), rounded
+// to 16 words so every area is 32-byte aligned.
+template <int PB>
+constexpr uint32_t chain_stage_words() {
+  return (uint32_t)round_up(64ull * lane_max_words(128u, PB), 16u);
+}
+
+// One symbol of one chain with its table entry e already in registers.  The
+// emit is branch-free: every lane stores, lanes that emit nothing store into
+// their own dummy slot, so the steps of several chains can be interleaved by
+// the scheduler (the table reads of the next step are issued before these
+// stores; no wait on the LDS counter sits between them).  Slots are u16
+// indices from the wave's staging base wbase: an emitted word goes to soff
+// (the chain's region) + ((cnt + rank) & MASK), where cnt is the unit's words
+// so far (wave-uniform in the 64-lane format) and rank the number of
+// emitting lanes of its unit below it; a lane that emits nothing writes its
+// dummy slot doff.
+template <int L, uint32_t MASK>
+__device__ __forceinline__ void enc_step(uint32_t& x, uint32_t& cnt, uint16_t* wbase, uint32_t soff, uint32_t doff,
+                                         const uint4& e, bool v, uint32_t half) {
+  const bool w = v && (x >= e.x);
+  const uint64_t m = __ballot(w);
+  const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)(m >> 32);
+  uint32_t add, below;
+  if (L == 64) {
+    add = (uint32_t)__popcll(m);
+    below = 0;
+  } else {
+    add = half ? (uint32_t)__popc(mhi) : (uint32_t)__popc(mlo);
+    below = half ? (uint32_t)__popc(mlo) : 0u;
+  }
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u)) - below;
+  // the slot goes through an opaque register so the compiler keeps one
+  // unconditional store instead of splitting the select into two branches
+  uint32_t slot = w ? soff + ((cnt + rank) & MASK) : doff;
+  __asm__ volatile("" : "+v"(slot));
+  wbase[slot] = (uint16_t)x;
+  x = w ? x >> 16 : x;
+  cnt += add;
+  const uint32_t q = (__umulhi(x, e.y) + x) >> (e.z & 31u);
+  const uint32_t xn = __umul24(q, e.w) + x + (e.z >> 8);
+  x = v ? xn : x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Decoupled look-back for ticket group `chain` with aggregate agg (all
+// lanes): publishes the aggregate, sums predecessors 64 at a time (lane l
+// reads status[j - l]) back to the nearest inclusive prefix, publishes this
+// group's inclusive prefix and returns the exclusive one.  Each status word
+// holds its flag and value together and carries nothing else, so relaxed
+// device-scope atomics suffice (release / acquire at agent scope would write
+// back / invalidate the XCD's whole L2 on every publish and poll).
+__device__ __forceinline__ uint64_t look_back(uint64_t* status, uint64_t chain, uint64_t agg, int lane) {
+  if (chain == 0) {
+    if (lane == 0) __hip_atomic_store(status, kFlagPrefix | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  if (lane == 0) __hip_atomic_store(status + chain, kFlagAgg | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t excl = 0;
+  int64_t j = (int64_t)chain - 1;
+  for (;;) {
+    const int64_t idx = j - lane;
+    // below group 0 (never needed: group 0 is always a prefix) reads as a
+    // zero prefix
+    uint64_t v = kFlagPrefix;
+    if (idx >= 0) v = __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t f = (uint32_t)(v >> 62);
+    const uint64_t pm = __ballot(f == 2), nm = __ballot(f == 0);
+    // lanes up to and including the nearest prefix (all 64 if none)
+    const uint64_t upto = pm ? (pm & (0 - pm)) * 2u - 1u : ~0ull;
+    if (nm & upto) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += wave_sum(((upto >> lane) & 1u) ? (v & kValueMask) : 0u);
+    if (pm) break;
+    j -= 64;
+  }
+  if (lane == 0)
+    __hip_atomic_store(status + chain, kFlagPrefix | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
+// One symbol of each of the CH chains; returns the CH decoded bytes packed
+// into an element.  Every lane reads a word (no branch, so the chains'
+// steps interleave): lanes that take none read some in-range slot and keep
+// their state.  ptr (words of the unit not yet taken) never drops below a
+// step's count on a valid stream; on a corrupt one it is clamped at 0 so
+// every read stays inside the stage, and `taken` (the words taken so far)
+// then ends above the unit's word count.  Word i of a unit sits at
+// stage[i & MASK] (MASK = R - 1 for a ring, all ones for a whole-unit
+// stage); a step reads stage[(base & MASK) + rank] with rank <= 64, which the
+// ring's mirror covers.  The three phases (LUT reads; state update and word
+// reads; selects) each run over all chains, so one chain's LDS latency
+// overlaps the others' work.
+template <int L, int PB, int CH, uint32_t MASK>
+__device__ __forceinline__ uint32_t dec_steps(uint32_t (&x)[CH], uint32_t (&ptr)[CH], uint32_t (&taken)[CH],
+                                              const uint2* const (&lut)[CH], const uint16_t* const (&stage)[CH],
+                                              const bool (&v)[CH], uint32_t half) {
+  constexpr uint32_t W = 1u << PB;
+  uint2 e[CH];
+  uint32_t word[CH];
+  bool r[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) e[c] = lut[c][x[c] & (W - 1u)];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    // the 24-bit multiply ignores the symbol in bits 24..31 of e.x
+    const uint32_t xn = __umul24(e[c].x, x[c] >> PB) + e[c].y;
+    x[c] = v[c] ? xn : x[c];
+    r[c] = v[c] && (x[c] < kMinState);
+    const uint64_t m = __ballot(r[c]);
+    const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)(m >> 32);
+    uint32_t cnt, below;
+    if (L == 64) {
+      cnt = (uint32_t)__popcll(m);
+      below = 0;
+    } else {
+      cnt = half ? (uint32_t)__popc(mhi) : (uint32_t)__popc(mlo);
+      below = half ? (uint32_t)__popc(mlo) : 0u;
+    }
+    taken[c] += cnt;
+    // ptr < 2^16 and cnt <= 64: a signed max keeps this scalar (s_max_i32)
+    const int32_t bd = (int32_t)ptr[c] - (int32_t)cnt;
+    const uint32_t base = (uint32_t)(bd > 0 ? bd : 0);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u)) - below;
+    word[c] = (stage[c] + (base & MASK))[rank];
+    ptr[c] = base;
+  }
+  // keep the reads unconditional: without this the compiler moves each under
+  // its `r` and puts a branch between the chains
+#pragma unroll
+  for (int c = 0; c < CH; ++c) __asm__ volatile("" : "+v"(word[c]));
+  uint32_t el = 0;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    x[c] = r[c] ? (x[c] << 16) | word[c] : x[c];
+    el |= (e[c].x >> 24) << (8 * c);
+  }
+  return el;
+}
+
+// Sum of a 32-bit value over the 64 lanes (every lane gets it).
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Normalises one plane with a single wavefront: lane l owns symbols
+// 4l .. 4l + 3 (count[j] is symbol 4l + j).  The same restatement of
+// GpuANSStatistics.h:186-385 as the 256-thread finisher of the histogram
+// launch (fp32 quantisation, rank by (q << 16 | s) descending, the +1 rule
+// for symbol ids below diff % 256, repeated decrements at ranks
+// [G - it, G)), with wave-wide sums and scans.  s_keys: 256 words of this
+// wave's LDS.  Returns {pdf, cdf, magic, shift} per owned symbol.
+__device__ inline void normalize_wave(const uint32_t (&count)[4], uint32_t total, int p, uint32_t* s_keys,
+                                      uint4 (&out)[4], bool& bad) {
+  const uint32_t lane = threadIdx.x & 63u;
+  if (total == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[j] = make_uint4(0, 0, 0, 0);
+    return;
+  }
+  const uint32_t W = 1u << p;
+  uint32_t q[4], key[4], qs = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float ratio = __fdiv_rn((float)count[j], (float)total);
+    const float prod = __fmul_rn((float)W, ratio);
+    q[j] = (uint32_t)prod;
+    if (count[j] > 0 && q[j] == 0) q[j] = 1;
+    qs += q[j];
+    key[j] = (q[j] << 16) | (4u * lane + (uint32_t)j);
+    s_keys[4u * lane + (uint32_t)j] = key[j];
+  }
+  const uint32_t qsum = wave_sum32(qs);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  int rank[4] = {0, 0, 0, 0};
+  for (int i = 0; i < (int)kNumSymbols; ++i) {
+    const uint32_t k = s_keys[i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) rank[j] += k > key[j];
+  }
+  const int diff = (int)W - (int)qsum;
+  if (diff > 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) q[j] += (uint32_t)(diff >> 8) + ((4u * lane + (uint32_t)j < (uint32_t)(diff & 255)) ? 1u : 0u);
+  } else if (diff < 0) {
+    int d = -diff;
+    while (d > 0) {
+      uint32_t gl = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) gl += q[j] > 1 ? 1u : 0u;
+      const int g = (int)wave_sum32(gl);
+      const int it = d < g ? d : g;
+      if (it <= 0) {
+        bad = true;
+        break;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (rank[j] >= g - it && rank[j] < g) q[j] -= 1;
+      d -= it;
+    }
+  }
+  // exclusive scan in symbol order: lane sums, then a wave scan
+  const uint32_t ls = q[0] + q[1] + q[2] + q[3];
+  uint32_t inc = ls;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o, 64);
+    if (lane >= (uint32_t)o) inc += u;
+  }
+  uint32_t cdf = inc - ls;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t pdf = q[j];
+    uint32_t shift = 0, magic = 0;
+    if (pdf > 0) {
+      shift = 32u - (uint32_t)__clz((int)(pdf - 1u));
+      const uint64_t m = ((1ull << 32) * ((1ull << shift) - pdf)) / pdf + 1ull;
+      magic = (uint32_t)m;
+    }
+    out[j] = make_uint4(pdf, cdf, magic, shift);
+    cdf += pdf;
+  }
+}
+
+}  // namespace mans

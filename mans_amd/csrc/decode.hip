@@ -1,28 +1,35 @@
-// decode.hip -- interleaved rANS decoder, one gfx950 wavefront per 64 lanes.
+// decode.hip -- interleaved rANS decoder on gfx950 wavefronts.
 //
 // Reference: include/ans/GpuANSDecode.h:27-46 (packed LUT entry),
 // :48-142 (decodeOneWarp / decodeOnePartialWarp), :144-302 (block loop,
 // grid-stride over blocks), :304-377 (ansDecodeTable).  Differences in
 // structure, not in results:
 //  * every workgroup builds the 2^p-entry LUT of each plane in LDS straight
-//    from the container's pdf table (no separate table launch, no global
-//    table);
-//  * a wavefront decodes two 32-lane reference blocks (one per half-wave) or
-//    one 64-lane native block; the block's words are first copied to LDS with
-//    16-byte loads, so the serial per-lane chain reads LDS, not HBM;
+//    from the container's pdf table (no separate table launch);
+//  * each lane runs CH chains, one per byte plane of the same elements (the
+//    32-lane reference format puts two blocks on one wavefront, one per
+//    half), so the serial per-chain dependency is interleaved CH ways and the
+//    decoded bytes are assembled into whole elements: one coalesced store per
+//    lane per step instead of one byte store per plane;
+//  * the chain reads its words from LDS.  In the 64-lane format a chain keeps
+//    only a ring of R words of its unit (plus a 64-word mirror of the ring's
+//    head, so the words of one step never wrap): the next R/2 words are
+//    loaded from HBM half a ring ahead of use and written to LDS just before
+//    they are needed.  Small LDS per wave keeps many waves resident, which is
+//    what hides the LDS and HBM latency of the serial chains.  The 32-lane
+//    format stages each unit's words whole;
+//  * every lane reads a word each step (lanes that take none discard it), so
+//    there is no branch between the chains' steps;
 //  * a block is valid only if it consumes exactly its words and every lane
 //    ends in the start state 2^15; anything else raises an error flag.
-// Per symbol: slot = x & (2^p - 1); {sym, pdf, slot - cdf} = LUT[slot];
+// Per symbol: slot = x & (2^p - 1); {pdf | sym << 24, slot - cdf} = LUT[slot];
 // x = pdf * (x >> p) + slot - cdf; lanes with x < 2^15 read a word from the
 // end of the block's remaining words, the highest such lane taking the last
 // word; x = (x << 16) + word.
-#include "mans_internal.hpp"
+#include "codec_device.hpp"
 
 namespace mans {
 namespace {
-
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
 
 struct DPlane {
   uint8_t* dst;
@@ -31,24 +38,37 @@ struct DPlane {
   uint64_t nb;
 };
 
-template <int L, int PB, int S>
-__global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a) {
+// L lanes per unit, PB probability bits, S element bytes, CH chains per lane
+// (CH == S: chain c decodes byte c of each element), NW wavefronts per group,
+// RB: log2 of the ring in words (64-lane format), 0 = whole-unit staging.
+template <int L, int PB, int S, int CH, int NW, int RB>
+__global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
+  using T = typename ElemOf<S>::T;
   constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
-  constexpr uint32_t UPW = L == 32 ? 2u : 1u;
+  constexpr uint32_t HALVES = L == 32 ? 2u : 1u;
+  constexpr uint32_t UPW = CH * HALVES;
   constexpr uint32_t W = 1u << PB;
-  constexpr uint32_t STAGE = (uint32_t)round_up(64u * lane_max_words(128u, PB), 16u);
-  constexpr uint32_t STAGE_UNIT = STAGE / UPW;
+  constexpr bool RING = RB > 0;
+  constexpr uint32_t R = RING ? 1u << RB : 0u;  // ring words
+  constexpr uint32_t H = RING ? R / 2 : 8u;     // refill chunk words
+  constexpr uint32_t H4 = H / 8;                // refill chunk uint4s
+  constexpr uint32_t MASK = RING ? R - 1u : 0xffffffffu;
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  constexpr uint32_t HSTAGE = CSTAGE / HALVES;
+  constexpr uint32_t REG = RING ? R + 64u : CSTAGE;  // LDS words per chain
   constexpr uint32_t UNIT_CAP = L * lane_max_words(BLOCK / L, PB);
+  static_assert(CH == S, "chain c decodes byte c of each element");
+  static_assert(!RING || (L == 64 && H >= 128 && H4 <= 64), "ring: 64 lanes, 128 <= R/2 <= 512 words");
   extern __shared__ uint4 smem[];
   __shared__ DPlane s_pl[kMaxPlanes];
   __shared__ uint64_t s_ubase[kMaxPlanes + 1];
   __shared__ uint32_t s_cdf[kNumSymbols + 1];
-  __shared__ uint32_t s_tmp[kWaves];
+  __shared__ uint32_t s_tmp[4];
   __shared__ uint64_t s_total_units, s_data_off, s_data_words;
   __shared__ int s_bad;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int P = a.d.num_planes;
-  uint32_t* s_lut = (uint32_t*)smem;  // [P][W]
+  uint2* s_lut = (uint2*)smem;  // [P][W] {pdf | sym << 24, slot - cdf}
 
   if (t == 0) s_bad = 0;
   if (t < P) {
@@ -87,8 +107,6 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a) {
     if (blockIdx.x == 0 && t == 0) atomicOr(a.err, (uint32_t)s_bad);
     return;
   }
-
-  // ---- LUT per plane: entry = (slot - cdf) << 20 | pdf << 8 | sym --------
   if (!a.from_header && t < P) {
     // frames: each plane section must carry the stream magic / version /
     // precision and the block count of the frame header
@@ -99,27 +117,45 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a) {
     else if (h[1] != s_pl[t].nb || h[2] != s_pl[t].nsym) atomicOr(&s_bad, (int)kErrBlockWords);
   }
   __syncthreads();
+
+  // ---- LUT per plane: entry = {pdf | sym << 24, slot - cdf} -------------
+  // exclusive scan of the 256 pdfs by the first 256 / PER threads (PER pdfs
+  // each, at most 4 wavefronts), then one binary search per slot
+  constexpr int PER = 64 * NW >= (int)kNumSymbols ? 1 : (int)kNumSymbols / (64 * NW);
+  constexpr int SCAN_T = (int)kNumSymbols / PER;
   for (int k = 0; k < P; ++k) {
     const uint16_t* probs = (const uint16_t*)(a.in + s_pl[k].section + kAnsHeaderBytes);
-    const uint32_t pdf = probs[t];
-    uint32_t inc = pdf;
+    uint32_t v[PER], sum = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      v[j] = t < SCAN_T ? probs[t * PER + j] : 0u;
+      sum += v[j];
+    }
+    uint32_t inc = sum;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       uint32_t u = __shfl_up(inc, o, 64);
       if (lane >= o) inc += u;
     }
-    if (lane == 63) s_tmp[wave] = inc;
+    if (lane == 63 && t < SCAN_T) s_tmp[wave] = inc;
     __syncthreads();
-    uint32_t base = 0;
-    for (int j = 0; j < wave; ++j) base += s_tmp[j];
-    s_cdf[t] = base + inc - pdf;
-    if (t == kThreads - 1) s_cdf[kNumSymbols] = base + inc;
+    if (t < SCAN_T) {
+      uint32_t base = 0;
+      for (int j = 0; j < wave; ++j) base += s_tmp[j];
+      uint32_t run = base + inc - sum;
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        s_cdf[t * PER + j] = run;
+        run += v[j];
+      }
+      if (t == SCAN_T - 1) s_cdf[kNumSymbols] = run;
+    }
     __syncthreads();
     if (s_pl[k].nsym > 0 && s_cdf[kNumSymbols] != W) {
       if (t == 0) s_bad = kErrTable;
     } else {
-      uint32_t* lut = s_lut + k * W;
-      for (uint32_t slot = t; slot < W; slot += kThreads) {
+      uint2* lut = s_lut + k * W;
+      for (uint32_t slot = t; slot < W; slot += 64 * NW) {
         // largest symbol s with cdf[s] <= slot (that symbol has pdf > 0)
         uint32_t lo = 0, hi = kNumSymbols;
         while (hi - lo > 1) {
@@ -127,7 +163,7 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a) {
           if (s_cdf[mid] <= slot) lo = mid; else hi = mid;
         }
         const uint32_t c = s_cdf[lo], p = s_cdf[lo + 1] - c;
-        lut[slot] = ((slot - c) << 20) | (p << 8) | lo;
+        lut[slot] = make_uint2(p | (lo << 24), slot - c);
       }
     }
     __syncthreads();
@@ -139,127 +175,228 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a) {
 
   const uint32_t half = L == 32 ? (uint32_t)(lane >> 5) : 0u;
   const uint32_t lin = (uint32_t)lane & (L - 1);
-  uint16_t* stage = (uint16_t*)(smem + div_up((uint64_t)P * W * 4u, 16u)) + wave * STAGE + half * STAGE_UNIT;
+  // per wave: CH regions of REG words (ring + mirror, or a whole unit), then
+  // with whole-unit staging 64 words of slack (a lane that takes no word may
+  // read one slot past its unit's words)
+  const uint32_t wave_u = __builtin_amdgcn_readfirstlane((uint32_t)wave);
+  uint16_t* wstage =
+      (uint16_t*)(smem + div_up((uint64_t)P * W * 8u, 16u)) + wave_u * (CH * REG + (RING ? 0u : 64u));
   const uint64_t total_units = s_total_units;
   const uint64_t data_off = s_data_off, data_words = s_data_words;
-  const uint64_t nwaves = (uint64_t)gridDim.x * kWaves;
+  const uint64_t nwaves = (uint64_t)gridDim.x * NW;
 
-  for (uint64_t wu = (uint64_t)blockIdx.x * kWaves + wave; wu * UPW < total_units; wu += nwaves) {
-    const uint64_t unit = wu * UPW + half;
-    bool active = unit < total_units;
-    int k = 0;
-    uint64_t b = 0;
-    if (active) {
-      if (a.d.order == 0) {
-        k = (int)((uint32_t)unit & (uint32_t)(P - 1));
-        b = unit >> (P == 4 ? 2 : (P == 2 ? 1 : 0));
-      } else {
-        while (k + 1 < P && unit >= s_ubase[k + 1]) ++k;
-        b = unit - s_ubase[k];
+  for (uint64_t wu = (uint64_t)blockIdx.x * NW + wave_u; wu * UPW < total_units; wu += nwaves) {
+    uint32_t x[CH], ptr[CH], n[CH];
+    uint32_t wds[CH], taken[CH];
+    bool act[CH];
+    uint64_t blk[CH];
+    const uint2* lut[CH];
+    const uint16_t* stage[CH];
+    // ring state (wave-uniform): lowest word index held in LDS, whether a
+    // refill is in flight, and its registers
+    uint32_t lo[CH];
+    bool pend[CH];
+    uint4 rv[CH];
+    const uint4* src4[CH];
+    uint8_t* dst0 = nullptr;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const uint64_t unit = wu * UPW + half * CH + c;
+      act[c] = unit < total_units;
+      int k = 0;
+      uint64_t b = 0;
+      if (act[c]) {
+        if (a.d.order == 0) {
+          k = (int)((uint32_t)unit & (uint32_t)(P - 1));
+          b = unit >> (P == 4 ? 2 : (P == 2 ? 1 : 0));
+        } else {
+          while (k + 1 < P && unit >= s_ubase[k + 1]) ++k;
+          b = unit - s_ubase[k];
+        }
       }
-    }
-    const DPlane pl = s_pl[k];
-    const uint8_t* sec = a.in + pl.section;
-    const uint64_t start = b * BLOCK;
-    uint32_t n = 0, words = 0, y = 0;
-    if (active) {
-      const uint2 bw = *(const uint2*)(sec + kSectionFixed + 4ull * L * pl.nb + 8ull * b);
-      n = bw.x >> 16;
-      words = bw.x & 0xffffu;
-      y = bw.y;
-      const uint64_t expect = (pl.nsym - start) < BLOCK ? (pl.nsym - start) : BLOCK;
-      if (n != expect || words > UNIT_CAP || words > STAGE_UNIT ||
-          (uint64_t)y + round_up(words, 8) > data_words ||
-          (a.in_bytes && data_off + 2ull * ((uint64_t)y + words) > a.in_bytes)) {
-        if (lin == 0) atomicOr(a.err, kErrBlockWords);
-        active = false;
-        n = 0;
-        words = 0;
+      blk[c] = b;
+      const DPlane pl = s_pl[k];
+      if (c == 0) dst0 = pl.dst;
+      const uint8_t* sec = a.in + pl.section;
+      const uint64_t start = b * BLOCK;
+      uint32_t nn = 0, words = 0, y = 0;
+      if (act[c]) {
+        const uint2 bw = *(const uint2*)(sec + kSectionFixed + 4ull * L * pl.nb + 8ull * b);
+        nn = bw.x >> 16;
+        words = bw.x & 0xffffu;
+        y = bw.y;
+        const uint64_t expect = (pl.nsym - start) < BLOCK ? (pl.nsym - start) : BLOCK;
+        if (nn != expect || words > UNIT_CAP || (!RING && words > HSTAGE) ||
+            (uint64_t)y + round_up(words, 8) > data_words ||
+            (a.in_bytes && data_off + 2ull * ((uint64_t)y + words) > a.in_bytes)) {
+          if (lin == 0) atomicOr(a.err, kErrBlockWords);
+          act[c] = false;
+          nn = 0;
+          words = 0;
+        }
       }
-    }
-    // words of the unit -> LDS (16-byte loads; y is a multiple of 8 words)
-    {
-      const uint4* src4 = (const uint4*)(a.in + data_off + 2ull * y);
-      uint4* dst4 = (uint4*)stage;
-      const uint32_t n4 = (uint32_t)div_up(words, 8);
-      for (uint32_t j = lin; j < n4; j += L) dst4[j] = src4[j];
-    }
-    __threadfence_block();
-    uint32_t x = active ? ((const uint32_t*)(sec + kSectionFixed))[b * L + lin] : kMinState;
-    uint32_t ptr = words;
-    uint32_t n_min, n_max;
-    if (L == 32) {
-      uint32_t n0 = __builtin_amdgcn_readlane(n, 0), n1 = __builtin_amdgcn_readlane(n, 32);
-      n_min = n0 < n1 ? n0 : n1;
-      n_max = n0 < n1 ? n1 : n0;
-    } else {
-      n_min = n_max = __builtin_amdgcn_readfirstlane(n);
-    }
-    const uint32_t* lut = s_lut + k * W;
-    uint8_t* dp = pl.dst + (start + lin) * S;
-
-    auto step = [&](uint32_t s, bool v) {
-      const uint32_t e = lut[x & (W - 1u)];
-      if (v) x = __umul24((e >> 8) & 0xfffu, x >> PB) + (e >> 20);
-      const bool r = v && (x < kMinState);
-      const uint64_t m = __ballot(r);
-      const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)(m >> 32);
-      uint32_t base, cnt;
+      n[c] = nn;
+      lut[c] = s_lut + k * W;
+      // all lanes of a unit read the same block words: with L == 64 the
+      // count is wave-uniform, and kept in a scalar register
       if (L == 64) {
-        cnt = (uint32_t)__popcll(m);
-        base = ptr - cnt;
-      } else {
-        cnt = half ? (uint32_t)__popc(mhi) : (uint32_t)__popc(mlo);
-        base = half ? ptr - cnt - (uint32_t)__popc(mlo) : ptr - cnt;
+        words = __builtin_amdgcn_readfirstlane(words);
+        y = __builtin_amdgcn_readfirstlane(y);
       }
-      const uint32_t idx = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, base));
-      if (r) x = (x << 16) | (uint32_t)stage[idx];
-      ptr -= cnt;
-      if (v) dp[(uint64_t)s * L * S] = (uint8_t)e;
+      src4[c] = (const uint4*)(a.in + data_off + 2ull * y);  // y is a multiple of 8 words
+      uint16_t* reg = wstage + c * REG + (RING ? 0u : half * HSTAGE);
+      stage[c] = reg;
+      lo[c] = 0;
+      pend[c] = false;
+      rv[c] = make_uint4(0, 0, 0, 0);
+      if constexpr (RING) {
+        // the top one or two chunks of the unit: chunk j holds words
+        // [j H, (j + 1) H) at ring half j & 1; half 0's first 64 words are
+        // mirrored after the ring
+        if (words > 0) {
+          const uint32_t jt = (words - 1) / H;
+          const uint32_t j0 = jt > 0 ? jt - 1 : 0;
+          lo[c] = j0 * H;
+          for (uint32_t j = j0; j <= jt; ++j) {
+            const uint32_t left = words - j * H;
+            const uint32_t n4 = (uint32_t)div_up(left < H ? left : H, 8);
+            if ((uint32_t)lane < n4) {
+              const uint4 q = src4[c][j * H4 + lane];
+              ((uint4*)(reg + (j & 1) * H))[lane] = q;
+              if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + R))[lane] = q;
+            }
+          }
+        }
+      } else {
+        // whole unit -> LDS (16-byte loads)
+        uint4* d4 = (uint4*)reg;
+        const uint32_t n4 = (uint32_t)div_up(words, 8);
+        for (uint32_t j = lin; j < n4; j += L) d4[j] = src4[c][j];
+      }
+      x[c] = act[c] ? ((const uint32_t*)(sec + kSectionFixed))[b * L + lin] : kMinState;
+      ptr[c] = words;
+      wds[c] = words;
+      taken[c] = 0;
+    }
+    // wait for the state loads here, once: a first use inside the step loop
+    // would put a wait for all outstanding loads (the ring refills) and
+    // stores in every step
+#pragma unroll
+    for (int c = 0; c < CH; ++c) __asm__ volatile("" : "+v"(x[c]));
+    __threadfence_block();
+
+    // ring upkeep after a step: start loading the chunk below once the one
+    // above the lowest is used up, and write it to LDS once fewer than 64
+    // words remain above lo (the next step may take up to 64)
+    auto refill = [&]() {
+      if constexpr (RING) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          if (!pend[c] && lo[c] > 0 && ptr[c] <= lo[c] + H) {
+            const uint32_t j = lo[c] / H - 1;
+            if ((uint32_t)lane < H4) rv[c] = src4[c][j * H4 + lane];
+            pend[c] = true;
+          }
+          if (pend[c] && ptr[c] < lo[c] + 64u) {
+            const uint32_t j = lo[c] / H - 1;
+            uint16_t* reg = wstage + c * REG;
+            if ((uint32_t)lane < H4) ((uint4*)(reg + (j & 1) * H))[lane] = rv[c];
+            if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + R))[lane] = rv[c];
+            lo[c] -= H;
+            pend[c] = false;
+          }
+        }
+      }
     };
 
+    uint32_t n_min = n[0], n_max = n[0];
+#pragma unroll
+    for (int c = 1; c < CH; ++c) {
+      n_min = n[c] < n_min ? n[c] : n_min;
+      n_max = n[c] > n_max ? n[c] : n_max;
+    }
+    if (L == 32) {
+      const uint32_t a0 = __builtin_amdgcn_readlane(n_min, 0), a1 = __builtin_amdgcn_readlane(n_min, 32);
+      const uint32_t b0 = __builtin_amdgcn_readlane(n_max, 0), b1 = __builtin_amdgcn_readlane(n_max, 32);
+      n_min = a0 < a1 ? a0 : a1;
+      n_max = b0 > b1 ? b0 : b1;
+    } else {
+      n_min = __builtin_amdgcn_readfirstlane(n_min);
+      n_max = __builtin_amdgcn_readfirstlane(n_max);
+    }
+    // chain 0 of a group is plane 0 whenever CH > 1: its destination is the
+    // element array
+    __attribute__((address_space(1))) T* ep = (__attribute__((address_space(1))) T*)dst0 + blk[0] * BLOCK + lin;
     const uint32_t full = n_min / L;
     const uint32_t all = (uint32_t)div_up(n_max, L);
     for (uint32_t s = all; s > full;) {
       --s;
-      step(s, s * L + lin < n);
+      const uint32_t i = s * L + lin;
+      bool v[CH], any = false;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        v[c] = i < n[c];
+        any |= v[c];
+      }
+      const uint32_t el = dec_steps<L, PB, CH, MASK>(x, ptr, taken, lut, stage, v, half);
+      if (any) ep[(uint64_t)s * L] = (T)el;
+      refill();
     }
+    bool vall[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) vall[c] = true;
     for (uint32_t s = full; s > 0;) {
       --s;
-      step(s, true);
+      ep[(uint64_t)s * L] = (T)dec_steps<L, PB, CH, MASK>(x, ptr, taken, lut, stage, vall, half);
+      refill();
     }
     // a valid block uses every word and returns each lane to 2^15
-    const bool bad = active && (ptr != 0 || x != kMinState);
-    if (__ballot(bad) && lin == 0 && active) atomicOr(a.err, kErrFinalState);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const bool bad = act[c] && (taken[c] != wds[c] || x[c] != kMinState);
+      if (__ballot(bad) && lin == 0 && act[c]) atomicOr(a.err, kErrFinalState);
+    }
     __threadfence_block();
   }
 }
 
-template <int L, int PB, int S>
+template <int L, int PB, int S, int CH, int NW, int RB>
 hipError_t launch_t(const DecodeArgs& a, int cu, hipStream_t s) {
-  constexpr uint32_t STAGE = (uint32_t)round_up(64u * lane_max_words(128u, PB), 16u);
-  const size_t lds = (size_t)round_up((uint64_t)a.d.num_planes * (1u << PB) * 4u, 16u) + (size_t)kWaves * STAGE * 2u;
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  constexpr uint32_t REG = RB > 0 ? (1u << RB) + 64u : CSTAGE;
+  const size_t lds = (size_t)round_up((uint64_t)a.d.num_planes * (1u << PB) * 8u, 16u) +
+                     (size_t)NW * (CH * REG + (RB > 0 ? 0u : 64u)) * 2u;
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decode_kernel<L, PB, S>, kThreads, lds);
+  hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decode_kernel<L, PB, S, CH, NW, RB>, 64 * NW, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   uint64_t grid = (uint64_t)cu * (uint64_t)per_cu;
   if (!a.from_header) {
-    constexpr uint32_t UPW = L == 32 ? 2u : 1u;
-    const uint64_t want = div_up(div_up(a.d.total_units, UPW), kWaves);
+    constexpr uint32_t UPW = CH * (L == 32 ? 2u : 1u);
+    const uint64_t want = div_up(div_up(a.d.total_units, UPW), NW);
     if (want < grid) grid = want;
   }
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((decode_kernel<L, PB, S>), dim3((uint32_t)grid), dim3(kThreads), lds, s, a);
+  hipLaunchKernelGGL((decode_kernel<L, PB, S, CH, NW, RB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
 template <int L, int PB>
 hipError_t launch_s(const DecodeArgs& a, int cu, hipStream_t s) {
-  switch (a.d.plane[0].stride) {
-    case 1: return launch_t<L, PB, 1>(a, cu, s);
-    case 2: return launch_t<L, PB, 2>(a, cu, s);
-    case 4: return launch_t<L, PB, 4>(a, cu, s);
+  const int P = a.d.num_planes;
+  const uint32_t S = a.d.plane[0].stride;
+  if constexpr (L == 64) {
+    // rings of 1024 words (512 with four chains): 4-9 KiB of LDS per wave
+    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 8, 10>(a, cu, s);
+    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 8, 9>(a, cu, s);
+    if (S == 1) return launch_t<L, PB, 1, 1, 8, 10>(a, cu, s);
+  } else {
+    // whole-unit staging; wavefronts per group chosen so LUTs + staging fit
+    // 64 KiB at p <= 10
+    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 2, 0>(a, cu, s);
+    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 1, 0>(a, cu, s);
+    if (S == 1) return launch_t<L, PB, 1, 1, 4, 0>(a, cu, s);
   }
   return hipErrorInvalidValue;
 }

@@ -8,38 +8,37 @@
 // final container.  Here one persistent launch does all of it:
 //  * a wavefront takes a ticket (atomic counter) for a group of units: the
 //    CH planes of one element block (two blocks, one per half-wave, in the
-//    32-lane reference format).  Each lane runs CH independent rANS chains,
-//    one per byte plane, fed by a single element load per step, so the
-//    serial per-chain dependency is interleaved CH ways;
+//    32-lane reference format), and takes its next ticket while it codes this
+//    one.  Each lane runs CH independent rANS chains, one per byte plane, fed
+//    by a single element load per step, so the serial per-chain dependency is
+//    interleaved CH ways;
 //  * element loads are software-pipelined: the next 16 steps are in flight
-//    while the current 16 are coded;
-//  * emitted 16-bit words go to LDS staging (bounded by lane_max_words), in
-//    lane order within each step (64-bit ballot + mbcnt);
+//    while the current 16 are coded, and the table entries of a step are read
+//    one step ahead;
+//  * emitted 16-bit words are compacted in lane order within each step
+//    (64-bit ballot + mbcnt) into LDS.  In the 64-lane format a chain keeps a
+//    ring of R words: every R/2 words go to the wave's own spill area (cache
+//    resident: each wave reuses it for every unit), so a wave needs a few KiB
+//    of LDS and many waves stay resident.  The 32-lane format stages units
+//    whole (bounded by lane_max_words);
 //  * the wave publishes its padded word count and resolves its output offset
-//    by decoupled look-back over earlier tickets (tickets are taken by running
-//    waves, so every wave waited on is resident and finishes);
-//  * it then writes states, block words and the staged words straight to
+//    by decoupled look-back over earlier tickets, 64 of them per round trip
+//    (tickets are taken by running waves, so every wave waited on is resident
+//    and finishes);
+//  * it then writes states, block words and the unit's words straight to
 //    their final place, 16 bytes per lane per store.
 // The arithmetic per symbol is the reference's: emit x & 0xffff when
 // x >= pdf * 2^(31-p); q = (umulhi(x, magic) + x) >> shift = x / pdf;
 // x = q * 2^p + (x - q * pdf) + cdf, computed as x + q * (2^p - pdf) + cdf.
-#include "mans_internal.hpp"
+#include "codec_device.hpp"
 
 namespace mans {
 namespace {
 
-constexpr uint64_t kFlagAgg = 1ull << 62;
-constexpr uint64_t kFlagPrefix = 2ull << 62;
-constexpr uint64_t kValueMask = (1ull << 62) - 1;
 constexpr int kBatch = 16;  // steps per prefetch batch
 
 template <typename T>
 using gptr = const __attribute__((address_space(1))) T*;
-
-template <int S> struct ElemOf;
-template <> struct ElemOf<1> { using T = uint8_t; };
-template <> struct ElemOf<2> { using T = uint16_t; };
-template <> struct ElemOf<4> { using T = uint32_t; };
 
 struct LanePlane {  // per-plane fields copied to LDS (dynamic index by lane)
   const uint8_t* src;
@@ -48,64 +47,25 @@ struct LanePlane {  // per-plane fields copied to LDS (dynamic index by lane)
   uint64_t nb;
 };
 
-// Words staged per chain per wavefront: 64 lanes x 128 symbols (one native
-// unit, or two 32-lane reference units, one per half), rounded to 16 words so
-// every staging area is 32-byte aligned.
-template <int PB>
-constexpr uint32_t chain_stage_words() {
-  return (uint32_t)round_up(64ull * lane_max_words(128u, PB), 16u);
-}
-
-template <int L>
-struct Emit {
-  // Position of this lane's word among the emitting lanes of its unit and
-  // the unit's count of words in this step.  In the 32-lane format each half
-  // of the wavefront is a separate unit.
-  static __device__ __forceinline__ void pos(uint64_t m, uint32_t half, uint32_t cnt, uint32_t& p, uint32_t& add) {
-    const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)(m >> 32);
-    uint32_t base;
-    if (L == 64) {
-      base = cnt;
-      add = (uint32_t)__popcll(m);
-    } else {
-      base = half ? cnt - (uint32_t)__popc(mlo) : cnt;
-      add = half ? (uint32_t)__popc(mhi) : (uint32_t)__popc(mlo);
-    }
-    p = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, base));
-  }
-};
-
-template <int L>
-__device__ __forceinline__ void enc_step(uint32_t& x, uint32_t& cnt, uint16_t* stage, const uint4* tab,
-                                         uint32_t sym, bool v, uint32_t half) {
-  const uint4 e = tab[sym];
-  const bool w = v && (x >= e.x);
-  const uint64_t m = __ballot(w);
-  uint32_t p, add;
-  Emit<L>::pos(m, half, cnt, p, add);
-  if (w) {
-    stage[p] = (uint16_t)x;
-    x >>= 16;
-  }
-  cnt += add;
-  if (v) {
-    const uint32_t q = (__umulhi(x, e.y) + x) >> (e.z & 31u);
-    x = __umul24(q, e.w) + x + (e.z >> 8);
-  }
-}
-
-// L lanes per unit, PB probability bits, S element bytes, CH chains per lane.
-// CH == S: the CH chains of a lane code the CH byte planes of the same
-// elements (unit order 0 with P == S).  CH == 1, S == 1: one chain per lane on
-// any unit (single streams, concatenated ADM planes).
-template <int L, int PB, int S, int CH, int NW>
+// L lanes per unit, PB probability bits, S element bytes, CH chains per lane,
+// NW wavefronts per group, RB log2 of the LDS ring in words (64-lane format;
+// 0 = whole-unit staging).  CH == S: the CH chains of a lane code the CH byte
+// planes of the same elements (unit order 0 with P == S).  CH == 1, S == 1:
+// one chain per lane on any unit (single streams, concatenated ADM planes).
+template <int L, int PB, int S, int CH, int NW, int RB>
 __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
-  using T = typename ElemOf<S>::T;
   constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
   constexpr uint32_t HALVES = L == 32 ? 2u : 1u;
   constexpr uint32_t UPW = CH * HALVES;  // units per ticket
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  constexpr bool RING = RB > 0;
+  constexpr uint32_t R = RING ? 1u << RB : 0u;   // ring words
+  constexpr uint32_t H = RING ? R / 2 : 8u;      // spill chunk words
+  constexpr uint32_t H4 = H / 8;                 // spill chunk uint4s
+  constexpr uint32_t MASK = RING ? R - 1u : 0xffffffffu;
+  constexpr uint32_t REG = RING ? R : CSTAGE;    // LDS words per chain
   static_assert(CH == S, "chain c codes byte c of each element");
+  static_assert(!RING || (L == 64 && H >= 128 && H4 <= 64), "ring: 64 lanes, 128 <= R/2 <= 512 words");
   extern __shared__ uint4 smem[];
   __shared__ LanePlane s_pl[kMaxPlanes];
   __shared__ uint64_t s_ubase[kMaxPlanes + 1];
@@ -129,17 +89,29 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
 
   const uint32_t half = L == 32 ? (uint32_t)(lane >> 5) : 0u;
   const uint32_t lin = (uint32_t)lane & (L - 1);
-  // staging: [wave][chain][half] regions of CSTAGE / HALVES words
-  uint16_t* wstage = (uint16_t*)(smem + P * kNumSymbols) + wave * CH * CSTAGE;
+  // staging: per wave, CH regions of REG words (a ring, or [half] whole
+  // units), then one dword-spaced dummy slot per lane
+  const uint32_t wave_u = __builtin_amdgcn_readfirstlane((uint32_t)wave);
+  uint16_t* wstage = (uint16_t*)(smem + P * kNumSymbols) + wave_u * (CH * REG + 128);
+  const uint32_t doff = CH * REG + 2u * (uint32_t)lane;  // dummy slot, from wstage
+  // this wave's spill area: CH regions of CSTAGE words
+  uint16_t* const spill = RING ? a.spill + ((uint64_t)blockIdx.x * NW + wave_u) * (CH * CSTAGE) : nullptr;
+  // words coded per plane by this wave (added to the section headers once,
+  // at the end)
+  uint32_t ptot[kMaxPlanes];
+#pragma unroll
+  for (int k = 0; k < kMaxPlanes; ++k) ptot[k] = 0;
 
-  for (;;) {
-    uint32_t tk = 0;
-    if (lane == 0) tk = atomicAdd(a.ticket, UPW);
-    tk = __builtin_amdgcn_readfirstlane(tk);
-    if ((uint64_t)tk >= a.d.total_units) break;
+  uint32_t tk = 0;
+  if (lane == 0) tk = atomicAdd(a.ticket, UPW);
+  tk = __builtin_amdgcn_readfirstlane(tk);
+  while ((uint64_t)tk < a.d.total_units) {
+    // the next ticket, taken now so its latency hides behind this one
+    uint32_t nxt = 0;
+    if (lane == 0) nxt = atomicAdd(a.ticket, UPW);
 
-    uint32_t x[CH], cnt[CH], n[CH];
-    uint16_t* stage[CH];
+    uint32_t x[CH], cnt[CH], n[CH], fl[CH];
+    uint32_t soff[CH];  // chain region, from wstage
     const uint4* tab[CH];
     uint64_t blk[CH];
     int kp[CH];
@@ -168,7 +140,8 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
       src[c] = pl.src;
       x[c] = kMinState;
       cnt[c] = 0;
-      stage[c] = wstage + c * CSTAGE + half * (CSTAGE / HALVES);
+      fl[c] = 0;
+      soff[c] = c * REG + (RING ? 0u : half * (CSTAGE / HALVES));
       tab[c] = s_tab + k * kNumSymbols;
     }
     // wave-uniform step counts over all chains and halves
@@ -187,8 +160,26 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
       n_min = __builtin_amdgcn_readfirstlane(n_min);
       n_max = __builtin_amdgcn_readfirstlane(n_max);
     }
-    const uint32_t full = n_min / L;
     const uint32_t all = (uint32_t)div_up(n_max, L);
+
+    // ring upkeep after a step: a chain with R/2 unspilled words moves the
+    // oldest R/2 (an aligned half of the ring) to its spill area; after the
+    // next step it has at most R/2 + 64 unspilled words, so no unspilled word
+    // is overwritten
+    auto spill_full = [&]() {
+      if constexpr (RING) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          if (cnt[c] - fl[c] >= H) {
+            if ((uint32_t)lane < H4) {
+              const uint4 q = ((const uint4*)(wstage + soff[c] + (fl[c] & MASK)))[lane];
+              ((uint4*)(spill + c * CSTAGE + fl[c]))[lane] = q;
+            }
+            fl[c] += H;
+          }
+        }
+      }
+    };
 
     {
       // One element per lane per step feeds all CH chains (byte c -> chain
@@ -208,12 +199,26 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) buf[j] = bp[j * (L * S / 4)];
       };
+      // The table entries of step j + 1 are read before step j is coded:
+      // they depend on the symbols only, not on the states.
       auto code = [&](const uint32_t (&buf)[kBatch]) {
+        uint4 e[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) e[c] = tab[c][__builtin_amdgcn_ubfe(buf[0], sh + 8u * c, 8u)];
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
+          uint4 en[CH];
+          if (j + 1 < kBatch) {
 #pragma unroll
-          for (int c = 0; c < CH; ++c)
-            enc_step<L>(x[c], cnt[c], stage[c], tab[c], __builtin_amdgcn_ubfe(buf[j], sh + 8u * c, 8u), true, half);
+            for (int c = 0; c < CH; ++c) en[c] = tab[c][__builtin_amdgcn_ubfe(buf[j + 1], sh + 8u * c, 8u)];
+          }
+#pragma unroll
+          for (int c = 0; c < CH; ++c) enc_step<L, MASK>(x[c], cnt[c], wstage, soff[c], doff, e[c], true, half);
+          spill_full();
+          if (j + 1 < kBatch) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) e[c] = en[c];
+          }
         }
       };
       // The prefetch is unconditional (the last batch reloads itself) so the
@@ -242,8 +247,10 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
         for (int c = 0; c < CH; ++c) {
           const bool v = i < n[c];
           const uint32_t sym = v ? (uint32_t)eb[(uint64_t)i * S + c] : 0u;
-          enc_step<L>(x[c], cnt[c], stage[c], tab[c], sym, v, half);
+          const uint4 e = tab[c][sym];
+          enc_step<L, MASK>(x[c], cnt[c], wstage, soff[c], doff, e, v, half);
         }
+        spill_full();
       }
     }
 
@@ -255,43 +262,14 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
     for (int c = 0; c < CH; ++c) {
       padded[c] = (uint32_t)round_up(cnt[c], 8);
       if (cnt[c] > CSTAGE / HALVES && lin == 0) atomicOr(a.err, kErrStageOverflow);
-      if (lin < padded[c] - cnt[c]) stage[c][cnt[c] + lin] = 0;
+      if (lin < padded[c] - cnt[c]) wstage[soff[c] + ((cnt[c] + lin) & MASK)] = 0;
       if (act[c]) ((uint32_t*)(a.out + s_pl[kp[c]].section + kSectionFixed))[blk[c] * L + lin] = x[c];
       a_lo[c] = __builtin_amdgcn_readlane(padded[c], 0);
       agg += a_lo[c];
       if (L == 32) agg += __builtin_amdgcn_readlane(padded[c], 32);
     }
 
-    // ---- decoupled look-back over tickets --------------------------------
-    // Each status word carries its flag and its value together, and no other
-    // memory is handed over through it, so relaxed device-scope atomics are
-    // enough.  (Release / acquire at agent scope would write back / invalidate
-    // the whole L2 of the XCD on every publish and poll.)
-    const uint64_t chain = tk / UPW;
-    uint64_t excl = 0;
-    if (lane == 0) {
-      uint64_t* st = a.status + chain;
-      if (chain == 0) {
-        __hip_atomic_store(st, kFlagPrefix | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        __hip_atomic_store(st, kFlagAgg | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint64_t j = chain - 1;
-        for (;;) {
-          const uint64_t v = __hip_atomic_load(a.status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const uint64_t f = v >> 62;
-          if (f == 0) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-          excl += v & kValueMask;
-          if (f == 2) break;
-          --j;
-        }
-        __hip_atomic_store(st, kFlagPrefix | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    excl = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(excl >> 32)) << 32) |
-           (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)excl);
+    const uint64_t excl = look_back(a.status, tk / UPW, agg, lane);
 
     // unit order inside the ticket: (half 0: chains 0..CH-1), (half 1: ...)
     uint64_t y = excl;
@@ -299,7 +277,9 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
 #pragma unroll
       for (int c = 0; c < CH; ++c) y += a_lo[c];
     }
-    __threadfence_block();
+    // spilled words are read back by the lanes that wrote them: make those
+    // stores complete first
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       if (act[c] && lin == 0) {
@@ -307,11 +287,18 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
         uint32_t* bw = (uint32_t*)(sec + kSectionFixed + 4ull * L * s_pl[kp[c]].nb + 8ull * blk[c]);
         bw[0] = (n[c] << 16) | cnt[c];
         bw[1] = (uint32_t)y;
-        atomicAdd((uint32_t*)(sec + 12), padded[c]);
       }
+#pragma unroll
+      for (int k = 0; k < kMaxPlanes; ++k) ptot[k] += (act[c] && kp[c] == k) ? padded[c] : 0u;
       uint4* dst = (uint4*)(a.out + a.d.data_off + 2ull * y);
-      const uint4* src4 = (const uint4*)stage[c];
-      for (uint32_t j = lin; j < padded[c] / 8; j += L) dst[j] = src4[j];
+      const uint4* ring4 = (const uint4*)(wstage + soff[c]);
+      const uint4* spill4 = RING ? (const uint4*)(spill + c * CSTAGE) : nullptr;
+      for (uint32_t j = lin; j < padded[c] / 8; j += L) {
+        uint4 q;
+        if (RING && 8 * j < fl[c]) q = spill4[j];
+        else q = ring4[((8 * j) & MASK) / 8];
+        dst[j] = q;
+      }
       y += padded[c];
     }
 
@@ -326,15 +313,24 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
       }
     }
     __threadfence_block();
+    tk = __builtin_amdgcn_readfirstlane(nxt);
+  }
+  // total coded words of each plane (the section headers start at 0)
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kMaxPlanes; ++k)
+      if (k < P && ptot[k]) atomicAdd((uint32_t*)(a.out + s_pl[k].section + 12), ptot[k]);
   }
 }
 
-template <int L, int PB, int S, int CH, int NW>
-hipError_t launch_t(const EncodeArgs& a, int cu, hipStream_t s) {
+template <int L, int PB, int S, int CH, int NW, int RB>
+hipError_t launch_t(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
-  const size_t lds = (size_t)a.d.num_planes * kNumSymbols * sizeof(uint4) + (size_t)NW * CH * CSTAGE * 2u;
+  constexpr uint32_t REG = RB > 0 ? 1u << RB : CSTAGE;
+  const size_t lds = (size_t)a.d.num_planes * kNumSymbols * sizeof(uint4) + (size_t)NW * (CH * REG + 128) * 2u;
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S, CH, NW>, 64 * NW, lds);
+  hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S, CH, NW, RB>, 64 * NW, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   constexpr uint32_t UPW = CH * (L == 32 ? 2u : 1u);
@@ -343,35 +339,54 @@ hipError_t launch_t(const EncodeArgs& a, int cu, hipStream_t s) {
   const uint64_t cap = (uint64_t)cu * (uint64_t)per_cu;
   if (grid > cap) grid = cap;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((encode_kernel<L, PB, S, CH, NW>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+  const uint64_t spill_bytes = RB > 0 ? grid * NW * CH * CSTAGE * 2ull : 0ull;
+  if (need_spill) {
+    *need_spill = spill_bytes;
+    return hipSuccess;
+  }
+  if (spill_bytes > a.spill_bytes) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((encode_kernel<L, PB, S, CH, NW, RB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
 template <int L, int PB>
-hipError_t launch_s(const EncodeArgs& a, int cu, hipStream_t s) {
+hipError_t launch_s(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
   const int P = a.d.num_planes;
   const uint32_t S = a.d.plane[0].stride;
-  if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 2>(a, cu, s);
-  if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 1>(a, cu, s);
-  if (S == 1) return launch_t<L, PB, 1, 1, 4>(a, cu, s);
+  if constexpr (L == 64) {
+    // rings of 1024 words (512 with four chains): 4-5 KiB of LDS per wave
+    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 4, 10>(a, cu, s, need_spill);
+    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 4, 9>(a, cu, s, need_spill);
+    if (S == 1) return launch_t<L, PB, 1, 1, 4, 10>(a, cu, s, need_spill);
+  } else {
+    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 2, 0>(a, cu, s, need_spill);
+    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 1, 0>(a, cu, s, need_spill);
+    if (S == 1) return launch_t<L, PB, 1, 1, 4, 0>(a, cu, s, need_spill);
+  }
   return hipErrorInvalidValue;
 }
 
 template <int L>
-hipError_t launch_p(const EncodeArgs& a, int cu, hipStream_t s) {
+hipError_t launch_p(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
   switch (a.d.prob_bits) {
-    case 9: return launch_s<L, 9>(a, cu, s);
-    case 10: return launch_s<L, 10>(a, cu, s);
-    case 11: return launch_s<L, 11>(a, cu, s);
+    case 9: return launch_s<L, 9>(a, cu, s, need_spill);
+    case 10: return launch_s<L, 10>(a, cu, s, need_spill);
+    case 11: return launch_s<L, 11>(a, cu, s, need_spill);
   }
   return hipErrorInvalidValue;
 }
 
 }  // namespace
 
+hipError_t encode_spill_bytes(const EncodeArgs& a, int cu, uint64_t* bytes) {
+  *bytes = 0;
+  if (a.d.total_units == 0) return hipSuccess;
+  return a.d.lanes == 64 ? launch_p<64>(a, cu, nullptr, bytes) : launch_p<32>(a, cu, nullptr, bytes);
+}
+
 hipError_t launch_encode(const EncodeArgs& a, int cu, hipStream_t s) {
   if (a.d.total_units == 0) return hipSuccess;
-  return a.d.lanes == 64 ? launch_p<64>(a, cu, s) : launch_p<32>(a, cu, s);
+  return a.d.lanes == 64 ? launch_p<64>(a, cu, s, nullptr) : launch_p<32>(a, cu, s, nullptr);
 }
 
 }  // namespace mans

@@ -157,6 +157,8 @@ struct EncodeArgs {
   uint32_t* d_out_size32;
   uint64_t* d_out_size64;
   int32_t frame;                 // 1: patch data_words / total_size in the frame header
+  uint16_t* spill;               // per-wave spill areas of the 64-lane encoder
+  uint64_t spill_bytes;          // their capacity
 };
 
 struct DecodeArgs {
@@ -197,8 +199,55 @@ struct AdmUnmapArgs {
 
 // Launchers (one per translation unit).  Each returns hipSuccess or the
 // launch error.  `cu` is the device's compute-unit count.
+// ---- batched byte-plane frames --------------------------------------------
+// One buffer of a batch: its frame layout is that of mans_frame_encode in
+// MANS_MODE_PLANES with the 64-lane format.
+struct BatchBuf {
+  const uint8_t* in;             // elements (16-byte aligned)
+  uint8_t* out;                  // frame (16-byte aligned)
+  uint64_t n;                    // elements
+  uint64_t nb;                   // blocks per plane
+  uint64_t section[kMaxPlanes];  // plane section offsets in the frame
+  uint64_t data_off;             // data region offset in the frame
+  uint32_t eb;                   // element bytes = byte planes (1, 2, 4)
+  uint32_t hist_first;           // first histogram item of the buffer
+  uint32_t nhist;                // histogram items (at least 1)
+  uint32_t tick_first;           // first encode ticket (ticket = 2 units)
+  uint32_t nticks;               // encode tickets
+  uint32_t pad_;
+  uint32_t frame_hdr[32];        // frame header (data_words / total_size set by the kernel)
+};
+
+struct BatchEncodeArgs {
+  const BatchBuf* bufs;
+  const uint32_t* item_buf;      // [nitems] item -> buffer
+  uint32_t nbufs;
+  uint32_t nhist_items;          // items [0, nhist_items) are histogram chunks
+  uint32_t nitems;               // then encode tickets
+  uint32_t* hist;                // [nbufs][4][256] accumulators (left zeroed)
+  uint32_t* done;                // [nbufs] histogram items finished (left zeroed)
+  uint32_t* ready;               // [nbufs] generation whose tables are ready
+  uint4* tables;                 // [nbufs][4][256] encoder table entries
+  uint64_t* status;              // [tickets] look-back words tagged with gen
+  uint32_t* ptot;                // [nbufs][4] words per plane (left zeroed)
+  uint32_t* fin;                 // [nbufs] encode tickets finished (left zeroed)
+  uint32_t* queue;               // [3] next item, exited wavefronts (both left
+                                 // zeroed), launch generation 1 .. kBatchGenMask
+                                 // (advanced by the last wavefront out)
+  uint32_t* err;
+  uint64_t* sizes;               // [nbufs] frame sizes out
+  uint16_t* spill;               // per-wave spill areas
+  uint64_t spill_bytes;
+};
+constexpr uint32_t kBatchGenMask = 0x3fffu;     // 14-bit generations
+constexpr uint32_t kBatchHistChunk = 1u << 18;  // bytes per histogram item
+
 hipError_t launch_hist(const HistArgs& a, int cu, hipStream_t s);
+// batched encode; with need_spill set, only reports the spill bytes needed
+hipError_t launch_batch_encode(const BatchEncodeArgs& a, int pb, int cu, hipStream_t s, uint64_t* need_spill);
 hipError_t launch_encode(const EncodeArgs& a, int cu, hipStream_t s);
+// bytes of spill area launch_encode needs for a (0 for the 32-lane format)
+hipError_t encode_spill_bytes(const EncodeArgs& a, int cu, uint64_t* bytes);
 hipError_t launch_decode(const DecodeArgs& a, int cu, hipStream_t s);
 hipError_t launch_adm_map(const AdmMapArgs& a, int cu, hipStream_t s);
 hipError_t launch_adm_unmap(const AdmUnmapArgs& a, hipStream_t s);

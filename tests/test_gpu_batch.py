@@ -1,0 +1,116 @@
+"""Batched encode (mans_batch_encode_*): every frame of a batch must be
+byte-identical to the single-buffer frame encode of that buffer (and to the
+oracle), decode back to the input, and stay identical over repeated and
+graph-replayed launches."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import mans_amd  # noqa: E402
+
+NAT = oracle.FORMAT_NATIVE
+DT = {1: np.uint8, 2: np.uint16, 4: np.uint32}
+VIEW = {1: np.uint8, 2: np.int16, 4: np.int32}
+
+
+@pytest.fixture(scope="module")
+def ws():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    w = mans_amd.Workspace(0)
+    yield w
+    w.close()
+
+
+def gen_buffer(rng, n: int, eb: int) -> np.ndarray:
+    bits = 8 * eb
+    if rng.random() < 0.5:
+        b = float(np.exp(rng.uniform(0, np.log(64))))
+        v = np.round(rng.laplace(0, b, n)).astype(np.int64) + (1 << (bits - 1))
+    else:
+        v = rng.zipf(1.1, n).astype(np.int64) * 2654435761
+    return (v % (1 << bits)).astype(DT[eb])
+
+
+def make_batch(seed: int, sizes: list[int]):
+    rng = np.random.default_rng(seed)
+    xs = []
+    for i, n in enumerate(sizes):
+        eb = int(rng.choice([1, 2, 4], p=[0.2, 0.6, 0.2]))
+        xs.append(gen_buffer(np.random.default_rng(1000 + seed * 10_000 + i), n, eb))
+    ts = [torch.from_numpy(x.view(VIEW[x.itemsize])).cuda() if x.size else
+          torch.empty(0, dtype=torch.from_numpy(x.view(VIEW[x.itemsize])).dtype, device="cuda") for x in xs]
+    return xs, ts
+
+
+EDGE_SIZES = [0, 1, 3, 15, 16, 17, 63, 64, 65, 8191, 8192, 8193, 16384, 3 * 8192 + 5,
+              (1 << 18) - 1, 1 << 18, (1 << 18) + 1, 100_000, 1_000_003]
+
+
+@pytest.mark.parametrize("p", [10, 11])
+def test_batch_frames_match_single_frames(ws, p):
+    rng = np.random.default_rng(p)
+    sizes = EDGE_SIZES + [int(v) for v in np.exp(rng.uniform(np.log(1 << 10), np.log(1 << 21), 40))]
+    xs, ts = make_batch(p, sizes)
+    print(f"batch p={p} buffers={len(xs)} bytes={sum(x.nbytes for x in xs)}", flush=True)
+    b = mans_amd.EncodeBatch(ts, mans_amd.params(p, 64), ws)
+    sizes_d = b.run()
+    got_sizes = sizes_d.cpu().numpy()
+    ws.check()
+    for i, (x, t) in enumerate(zip(xs, ts)):
+        q = mans_amd.params(p, 64, x.itemsize, mans_amd.MODE_PLANES)
+        ref, ref_size = mans_amd.frame_encode(t, q, ws)
+        assert got_sizes[i] == ref_size, (i, x.size, x.itemsize)
+        got = b.outs[i][:ref_size].cpu().numpy()
+        assert (got == ref[:ref_size].cpu().numpy()).all(), (i, x.size, x.itemsize)
+        if i % 7 == 0:
+            o = oracle.frame_encode(x, oracle.MODE_PLANES, p, NAT)
+            assert o.size == ref_size and (o == got).all(), (i, x.size)
+        back = mans_amd.frame_decode(b.outs[i], int(ref_size), t.dtype, x.size, ws)
+        assert (back.cpu().numpy().view(x.dtype) == x).all(), i
+    ws.check()
+    # a second launch of the same plan rewrites identical frames
+    for o in b.outs:
+        o.zero_()
+    b.run()
+    torch.cuda.synchronize()
+    assert (b.sizes.cpu().numpy() == got_sizes).all()
+    for i, x in enumerate(xs):
+        q = mans_amd.params(p, 64, x.itemsize, mans_amd.MODE_PLANES)
+        back = mans_amd.frame_decode(b.outs[i], int(got_sizes[i]), ts[i].dtype, x.size, ws)
+        assert (back.cpu().numpy().view(x.dtype) == x).all(), i
+    b.close()
+
+
+def test_batch_graph_replay(ws):
+    rng = np.random.default_rng(7)
+    sizes = [int(v) for v in np.exp(rng.uniform(np.log(4096), np.log(1 << 20), 64))]
+    xs, ts = make_batch(7, sizes)
+    b = mans_amd.EncodeBatch(ts, mans_amd.params(10, 64), ws)
+    b.run()
+    torch.cuda.synchronize()
+    eager = [o.clone() for o in b.outs]
+    eager_sizes = b.sizes.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        b.run(stream=s)
+    for rep in range(3):
+        for o in b.outs:
+            o.zero_()
+        b.sizes.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        print(f"graph replay {rep}", flush=True)
+        assert torch.equal(b.sizes, eager_sizes), rep
+        for i in range(len(xs)):
+            n = int(eager_sizes[i])
+            assert torch.equal(b.outs[i][:n], eager[i][:n]), (rep, i)
+    ws.check()
+    b.close()

@@ -253,6 +253,29 @@ def test_frame_planes_match_oracle(ws, lanes, dt):
         assert (back.view(x.dtype) == x).all()
 
 
+@pytest.mark.parametrize("dt,n", [(np.uint16, 40_000_003), (np.uint32, 12_000_001)])
+def test_frame_large_mixed_entropy(ws, dt, n):
+    """Many units per wavefront: blocks alternate between low-entropy
+    (Laplace) and near-uniform bytes, so the 64-lane coder's LDS rings spill
+    and refill on every chain, and look-back spans more than 64 tickets."""
+    rng = np.random.default_rng(41)
+    bits = 8 * np.dtype(dt).itemsize
+    lap = np.round(rng.laplace(0, 3, n)).astype(np.int64) + (1 << (bits - 1))
+    uni = rng.integers(0, 1 << bits, n, dtype=np.uint64).astype(np.int64)
+    block = (np.arange(n) // 8192) % 3 == 0
+    x = np.where(block, uni, lap).astype(dt)
+    print(f"frame large dt={np.dtype(dt).name} n={n}", flush=True)
+    xt = dev(x.view({2: np.int16, 4: np.int32}[x.itemsize]))
+    prm = mans_amd.params(10, 64, x.itemsize, mans_amd.MODE_PLANES)
+    out, size = mans_amd.frame_encode(xt, prm, ws)
+    ref = oracle.frame_encode(x, oracle.MODE_PLANES, 10, NAT, threads=8)
+    got = host(out[:size])
+    assert size == ref.size and (got == ref).all(), np.flatnonzero(got != ref)[:8]
+    back = host(mans_amd.frame_decode(out, size, xt.dtype, n, ws))
+    ws.check()
+    assert (back.view(x.dtype) == x).all()
+
+
 @pytest.mark.parametrize("lanes", [32, 64])
 def test_frame_adm_matches_oracle(ws, lanes):
     rng = np.random.default_rng(41)
