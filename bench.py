@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--prob-bits", type=int, default=10)
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096,
+                    help="buffers of the batched workload (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--dump", default=None,
                     help="write the generated codes to this file and exit (input of mans_profile)")
     return ap.parse_args()
@@ -125,6 +128,8 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(x, args)
+    del dec, out
+    batched = batched_bench(args) if args.batch_count > 0 else None
 
     line = {
         "metric": METRIC,
@@ -164,10 +169,158 @@ def main() -> int:
             "traffic": measured_traffic("encode") if default_workload else None,
         },
         "cpu_baseline": cpu,
+        "batched": batched,
         "gen_s": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)
     return 0 if ok else 1
+
+
+def batched_bench(args):
+    """The batched form on the 4096-buffer workload: every buffer coded as
+    its own byte-plane frame (64-lane format), once by the plain codec
+    (one mans_frame_encode per buffer, sizes left on the device) and once by
+    one batched launch (eager, then replayed from a captured graph).  Frames
+    are compared byte for byte."""
+    import torch
+
+    import mans_amd
+    from mans_amd import workloads
+
+    t0 = time.time()
+    xs, widths = workloads.batch_buffers(args.batch_count)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    total = sum(x.numel() * x.element_size() for x in xs)
+    p = mans_amd.params(args.prob_bits, 64)
+    ws = mans_amd.Workspace(0)
+    for eb in (1, 2, 4):
+        ws.reserve(16 << 20, mans_amd.params(args.prob_bits, 64, eb, mans_amd.MODE_PLANES))
+    caps = [mans_amd.frame_max_size(x.numel(), mans_amd.params(args.prob_bits, 64, x.element_size(),
+                                                                 mans_amd.MODE_PLANES)) for x in xs]
+    outs_plain = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    outs_batch = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    dsz = torch.zeros(len(xs), dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def plain_pass():
+        for i, x in enumerate(xs):
+            mans_amd.frame_encode(x, p, ws, out=outs_plain[i], sync_size=False, d_size=dsz[i:i + 1])
+
+    plain_pass()
+    torch.cuda.synchronize()
+    ws.check()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ws.set_profiling(True)
+    ws.kernel_times()
+    e0.record(stream)
+    for _ in range(args.batch_steps):
+        plain_pass()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    plain_ms = e0.elapsed_time(e1) / args.batch_steps
+    plain_kernel_ms = sum(ms for _, ms in ws.kernel_times()) / args.batch_steps
+    ws.set_profiling(False)
+
+    b = mans_amd.EncodeBatch(xs, p, ws, outs=outs_batch)
+    b.run()
+    torch.cuda.synchronize()
+    ws.check()
+    ok = bool(torch.equal(b.sizes, dsz))
+    if ok:
+        sz = dsz.cpu().tolist()
+        ok = all(torch.equal(outs_batch[i][:sz[i]], outs_plain[i][:sz[i]]) for i in range(len(xs)))
+    e0.record(stream)
+    for _ in range(args.batch_steps):
+        b.run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    batch_ms = e0.elapsed_time(e1) / args.batch_steps
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(stream)
+    with torch.cuda.graph(g, stream=s):
+        b.run(stream=s)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.batch_steps):
+        g.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    graph_ms = e0.elapsed_time(e1) / args.batch_steps
+    ok = ok and bool(torch.equal(b.sizes, dsz))
+    ws.check()
+    comp = int(dsz.sum().item())
+
+    # decode: one mans_frame_decode per frame (headers handed over from the
+    # host, so no call synchronises) against one batched launch
+    sz = dsz.cpu().tolist()
+    headers = [bytes(o[:128].cpu().numpy().tobytes()) for o in outs_plain]
+    dec_plain = [torch.empty(max(1, x.numel()), dtype=x.dtype, device="cuda") for x in xs]
+
+    def plain_decode():
+        for i, x in enumerate(xs):
+            mans_amd.frame_decode(outs_plain[i], sz[i], x.dtype, x.numel(), ws, out=dec_plain[i], header=headers[i])
+
+    plain_decode()
+    torch.cuda.synchronize()
+    ws.check()
+    ws.set_profiling(True)
+    ws.kernel_times()
+    e0.record(stream)
+    for _ in range(args.batch_steps):
+        plain_decode()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    plain_dec_ms = e0.elapsed_time(e1) / args.batch_steps
+    plain_dec_kernel_ms = sum(ms for _, ms in ws.kernel_times()) / args.batch_steps
+    ws.set_profiling(False)
+    d = mans_amd.DecodeBatch(outs_batch, sz, xs, p, ws)
+    outs = d.run()
+    torch.cuda.synchronize()
+    ws.check()
+    dec_ok = all(torch.equal(outs[i], xs[i]) and torch.equal(dec_plain[i][:xs[i].numel()], xs[i])
+                 for i in range(len(xs)))
+    gd = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gd, stream=s):
+        d.run(stream=s)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.batch_steps):
+        gd.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    graph_dec_ms = e0.elapsed_time(e1) / args.batch_steps
+    dec_ok = dec_ok and all(torch.equal(outs[i], xs[i]) for i in range(len(xs)))
+    ws.check()
+    d.close()
+    b.close()
+    ws.close()
+    return {
+        "workload": f"{len(xs)} buffers, sizes log-uniform in [4 KiB, 16 MiB], uint8/16/32 at 20/60/20 %, "
+                    f"Laplace or Zipf values 50/50, seeds 1000+i; byte-plane frames, 64 lanes, p={args.prob_bits}",
+        "bytes": total,
+        "compressed_bytes": comp,
+        "plain_ms": round(plain_ms, 3),
+        "plain_GBps": round(total / (plain_ms * 1e-3) / 1e9, 2),
+        "plain_kernel_ms": round(plain_kernel_ms, 3),
+        "batched_ms": round(batch_ms, 3),
+        "batched_GBps": round(total / (batch_ms * 1e-3) / 1e9, 2),
+        "graph_ms": round(graph_ms, 3),
+        "graph_GBps": round(total / (graph_ms * 1e-3) / 1e9, 2),
+        "speedup_vs_plain": round(plain_ms / graph_ms, 2),
+        "speedup_vs_plain_kernel_time": round(plain_kernel_ms / graph_ms, 2),
+        "bit_exact_vs_plain": ok,
+        "decode_plain_ms": round(plain_dec_ms, 3),
+        "decode_plain_GBps": round(total / (plain_dec_ms * 1e-3) / 1e9, 2),
+        "decode_plain_kernel_ms": round(plain_dec_kernel_ms, 3),
+        "decode_graph_ms": round(graph_dec_ms, 3),
+        "decode_graph_GBps": round(total / (graph_dec_ms * 1e-3) / 1e9, 2),
+        "decode_speedup_vs_plain": round(plain_dec_ms / graph_dec_ms, 2),
+        "decode_speedup_vs_plain_kernel_time": round(plain_dec_kernel_ms / graph_dec_ms, 2),
+        "decode_round_trip_bit_exact": dec_ok,
+        "gen_s": round(gen_s, 2),
+    }
 
 
 def measured_traffic(kernel: str):

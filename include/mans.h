@@ -150,10 +150,12 @@ mans_status mans_frame_info(const void* h_header, uint64_t header_bytes, uint64_
  * can be captured in a HIP graph.  Runs sharing a workspace must be
  * stream-ordered. */
 typedef struct mans_buffer {
-  const void* d_in;      /* elements, 16-byte aligned */
+  const void* d_in;      /* encode: elements; decode: frame (16-byte aligned) */
+  uint64_t in_bytes;     /* decode: bytes readable at d_in (the frame); encode: ignored */
+  void* d_out;           /* encode: frame; decode: elements (16-byte aligned) */
+  uint64_t out_capacity; /* encode: >= mans_frame_max_size(n_elems, {p, 64, elem_bytes});
+                            decode: >= n_elems * elem_bytes */
   uint64_t n_elems;
-  void* d_out;           /* frame, 16-byte aligned */
-  uint64_t out_capacity; /* >= mans_frame_max_size(n_elems, {p, 64, elem_bytes}) */
   int32_t elem_bytes;    /* 1, 2 or 4 */
   int32_t reserved;      /* 0 */
 } mans_buffer;
@@ -165,10 +167,65 @@ mans_status mans_batch_encode_create(mans_batch** b, const mans_params* p, const
                                      int32_t count, mans_workspace* ws);
 /* d_sizes: device array of count uint64 frame sizes, written by the launch. */
 mans_status mans_batch_encode_run(mans_batch* b, uint64_t* d_sizes, void* stream);
+/* Decodes frames written by the batched or the single-buffer frame encode
+ * (byte planes, 64 lanes, p->prob_bits) whose element count and width the
+ * caller knows.  A frame that does not match them, or whose streams are
+ * inconsistent, raises the sticky error read by mans_workspace_check
+ * (MANS_ERR_CORRUPT); its output is then undefined. */
+mans_status mans_batch_decode_create(mans_batch** b, const mans_params* p, const mans_buffer* h_bufs,
+                                     int32_t count, mans_workspace* ws);
+mans_status mans_batch_decode_run(mans_batch* b, void* stream);
 mans_status mans_batch_destroy(mans_batch* b);
 /* create + run + destroy (synchronises the stream before freeing). */
 mans_status mans_encode_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, uint64_t* d_sizes,
                               mans_workspace* ws, void* stream);
+mans_status mans_decode_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count,
+                              mans_workspace* ws, void* stream);
+
+/* ---- request server (continuous batching) ------------------------------ */
+/* No counterpart in the reference (its compress driver, examples/compress.cu,
+ * codes one file per process).  A server owns a device-memory arena of
+ * `memory_budget` bytes (less its own batch state) and a worker thread.
+ * Requests (device-resident input) are admitted in arrival order as arena
+ * memory frees up and batched continuously into the batched launches; a
+ * request's output stays in the arena until mans_server_release.  A
+ * compress request yields the frame mans_frame_encode writes for its input
+ * (byte planes, 64 lanes, the server's prob_bits), a decompress request the
+ * elements of such a frame.  All entry points are thread-safe. */
+enum { MANS_REQ_COMPRESS = 0, MANS_REQ_DECOMPRESS = 1 };
+
+typedef struct mans_request {
+  int32_t kind;       /* MANS_REQ_COMPRESS or MANS_REQ_DECOMPRESS */
+  int32_t elem_bytes; /* 1, 2 or 4 */
+  const void* d_in;   /* compress: elements; decompress: frame (device, 16-byte aligned) */
+  uint64_t in_bytes;  /* decompress: frame bytes */
+  uint64_t n_elems;   /* elements (both kinds) */
+} mans_request;
+
+typedef struct mans_result {
+  int32_t status;     /* MANS_OK, or the request's error */
+  int32_t reserved;
+  const void* d_out;  /* frame or elements, in the server's arena */
+  uint64_t out_bytes;
+  double queue_ms;    /* submit -> admitted into a launch */
+  double latency_ms;  /* submit -> output ready */
+} mans_result;
+
+typedef struct mans_server mans_server;
+
+/* p->lanes must be 64.  MANS_ERR_BUDGET when the budget does not cover the
+ * server's own state. */
+mans_status mans_server_create(mans_server** s, const mans_params* p, uint64_t memory_budget, int32_t device);
+/* MANS_ERR_BUDGET if the request's output can never fit the arena. */
+mans_status mans_server_submit(mans_server* s, const mans_request* r, uint64_t* id);
+/* Waits up to timeout_ms (< 0: no limit); MANS_ERR_BUSY if not done yet. */
+mans_status mans_server_wait(mans_server* s, uint64_t id, int32_t timeout_ms, mans_result* out);
+/* Frees a finished request's output. */
+mans_status mans_server_release(mans_server* s, uint64_t id);
+mans_status mans_server_stats(mans_server* s, uint64_t* arena_bytes, uint64_t* arena_used, uint64_t* batches,
+                              uint64_t* waiting);
+/* Finishes every submitted request, then frees everything. */
+mans_status mans_server_destroy(mans_server* s);
 
 /* ---- profiling --------------------------------------------------------- */
 /* When enabled, every call records HIP events on its stream around each of

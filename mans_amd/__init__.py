@@ -34,9 +34,23 @@ class Params(C.Structure):
 
 
 class Buffer(C.Structure):
-    _fields_ = [("d_in", C.c_void_p), ("n_elems", C.c_uint64), ("d_out", C.c_void_p),
-                ("out_capacity", C.c_uint64), ("elem_bytes", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("d_in", C.c_void_p), ("in_bytes", C.c_uint64), ("d_out", C.c_void_p),
+                ("out_capacity", C.c_uint64), ("n_elems", C.c_uint64), ("elem_bytes", C.c_int32),
+                ("reserved", C.c_int32)]
 
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("elem_bytes", C.c_int32), ("d_in", C.c_void_p), ("in_bytes", C.c_uint64),
+                ("n_elems", C.c_uint64)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("reserved", C.c_int32), ("d_out", C.c_void_p), ("out_bytes", C.c_uint64),
+                ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+REQ_COMPRESS = 0
+REQ_DECOMPRESS = 1
 
 _P = C.POINTER(Params)
 _B = C.POINTER(Buffer)
@@ -60,8 +74,17 @@ SIGNATURES = {
     "mans_frame_info": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64)]),
     "mans_batch_encode_create": (C.c_int, [C.POINTER(_vp), _P, _B, _i32, _vp]),
     "mans_batch_encode_run": (C.c_int, [_vp, _vp, _vp]),
+    "mans_batch_decode_create": (C.c_int, [C.POINTER(_vp), _P, _B, _i32, _vp]),
+    "mans_batch_decode_run": (C.c_int, [_vp, _vp]),
     "mans_batch_destroy": (C.c_int, [_vp]),
     "mans_encode_batch": (C.c_int, [_P, _B, _i32, _vp, _vp, _vp]),
+    "mans_decode_batch": (C.c_int, [_P, _B, _i32, _vp, _vp]),
+    "mans_server_create": (C.c_int, [C.POINTER(_vp), _P, _u64, _i32]),
+    "mans_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(_u64)]),
+    "mans_server_wait": (C.c_int, [_vp, _u64, _i32, C.POINTER(Result)]),
+    "mans_server_release": (C.c_int, [_vp, _u64]),
+    "mans_server_stats": (C.c_int, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
+    "mans_server_destroy": (C.c_int, [_vp]),
     "mans_set_profiling": (C.c_int, [_vp, _i32]),
     "mans_last_kernel_times": (_i32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_char_p), _i32]),
 }
@@ -239,15 +262,18 @@ def adm_unmap(offsets, centers, codes, bits, n: int, ws: Workspace, stream=None)
     return out[:n]
 
 
-def frame_encode(x, p: Params, ws: Workspace, out=None, stream=None, sync_size: bool = True):
-    """Encode a CUDA tensor of 1/2/4-byte elements into a MANS frame."""
+def frame_encode(x, p: Params, ws: Workspace, out=None, stream=None, sync_size: bool = True, d_size=None):
+    """Encode a CUDA tensor of 1/2/4-byte elements into a MANS frame.  The
+    size is returned on the host (sync_size) or left in `d_size` (a one-element
+    int64 CUDA tensor, allocated when not given)."""
     torch = _torch()
     n = x.numel()
     p = Params(p.prob_bits, p.lanes, x.element_size(), p.mode)
     cap = frame_max_size(n, p)
     if out is None:
         out = torch.empty(cap, dtype=torch.uint8, device=x.device)
-    d_size = torch.zeros(1, dtype=torch.int64, device=x.device)
+    if d_size is None:
+        d_size = torch.zeros(1, dtype=torch.int64, device=x.device)
     h_size = C.c_uint64(0)
     rc = lib().mans_frame_encode(C.byref(p), _ptr(x), n, out.data_ptr(), out.numel(), d_size.data_ptr(),
                                  C.byref(h_size) if sync_size else None, ws.handle, _stream(stream))
@@ -289,7 +315,7 @@ class EncodeBatch:
         self.outs = list(outs)
         arr = (Buffer * max(1, len(self.inputs)))()
         for i, (x, o) in enumerate(zip(self.inputs, self.outs)):
-            arr[i] = Buffer(_ptr(x), x.numel(), o.data_ptr(), o.numel(), x.element_size(), 0)
+            arr[i] = Buffer(_ptr(x), 0, o.data_ptr(), o.numel(), x.numel(), x.element_size(), 0)
         self.sizes = torch.zeros(max(1, len(self.inputs)), dtype=torch.int64,
                                  device=self.inputs[0].device if self.inputs else "cuda")
         h = C.c_void_p()
@@ -312,3 +338,113 @@ class EncodeBatch:
             self.close()
         except Exception:
             pass
+
+
+class DecodeBatch:
+    """A planned batch decode (mans_batch_decode_create) of byte-plane frames
+    (64 lanes): `frames` are uint8 CUDA tensors holding each frame (at least
+    its size in bytes), `like` gives each buffer's element count and dtype
+    (tensors, only their numel / dtype are used).  Outputs are allocated in
+    `outs`.  run() only launches."""
+
+    def __init__(self, frames, sizes, like, p: Params, ws: Workspace):
+        torch = _torch()
+        self.frames = list(frames)
+        self.ws = ws
+        self.outs = [torch.empty(max(1, t.numel()), dtype=t.dtype, device="cuda") for t in like]
+        arr = (Buffer * max(1, len(self.frames)))()
+        for i, (f, sz, t, o) in enumerate(zip(self.frames, sizes, like, self.outs)):
+            arr[i] = Buffer(f.data_ptr(), int(sz), o.data_ptr(), o.numel() * o.element_size(), t.numel(),
+                            t.element_size(), 0)
+        h = C.c_void_p()
+        _check(lib().mans_batch_decode_create(C.byref(h), C.byref(p), arr, len(self.frames), ws.handle),
+               "mans_batch_decode_create")
+        self.handle = h
+        self.like_n = [t.numel() for t in like]
+
+    def run(self, stream=None):
+        _check(lib().mans_batch_decode_run(self.handle, _stream(stream)), "mans_batch_decode_run")
+        return [o[:n] for o, n in zip(self.outs, self.like_n)]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().mans_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server (mans_server_*): submit compress / decompress requests
+    of CUDA tensors; results stay in the server's device arena until
+    released."""
+
+    def __init__(self, p: Params, memory_budget: int, device: int = 0):
+        _torch()
+        h = C.c_void_p()
+        _check(lib().mans_server_create(C.byref(h), C.byref(p), memory_budget, device), "mans_server_create")
+        self.handle = h
+
+    def submit_compress(self, x) -> int:
+        r = Request(REQ_COMPRESS, x.element_size(), _ptr(x), 0, x.numel())
+        rid = C.c_uint64(0)
+        _check(lib().mans_server_submit(self.handle, C.byref(r), C.byref(rid)), "mans_server_submit")
+        return rid.value
+
+    def submit_decompress(self, frame, frame_bytes: int, n: int, elem_bytes: int) -> int:
+        r = Request(REQ_DECOMPRESS, elem_bytes, frame.data_ptr(), frame_bytes, n)
+        rid = C.c_uint64(0)
+        _check(lib().mans_server_submit(self.handle, C.byref(r), C.byref(rid)), "mans_server_submit")
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1) -> Result | None:
+        res = Result()
+        rc = lib().mans_server_wait(self.handle, rid, timeout_ms, C.byref(res))
+        if rc == 11:  # MANS_ERR_BUSY
+            return None
+        _check(rc, "mans_server_wait")
+        return res
+
+    def fetch(self, res: Result, dtype=None):
+        """Copy a result out of the arena into a new CUDA tensor (uint8, or
+        viewed as dtype)."""
+        torch = _torch()
+        out = torch.empty(max(1, res.out_bytes), dtype=torch.uint8, device="cuda")
+        if res.out_bytes:
+            t = _torch_from_ptr(res.d_out, res.out_bytes)
+            out[:res.out_bytes].copy_(t)
+        out = out[:res.out_bytes]
+        return out.view(dtype) if dtype is not None else out
+
+    def release(self, rid: int) -> None:
+        _check(lib().mans_server_release(self.handle, rid), "mans_server_release")
+
+    def stats(self) -> dict:
+        v = [C.c_uint64(0) for _ in range(4)]
+        _check(lib().mans_server_stats(self.handle, *[C.byref(x) for x in v]), "mans_server_stats")
+        return {"arena_bytes": v[0].value, "arena_used": v[1].value, "batches": v[2].value, "waiting": v[3].value}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().mans_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _torch_from_ptr(ptr: int, nbytes: int):
+    """A uint8 CUDA tensor aliasing device memory owned by the library."""
+    torch = _torch()
+
+    class _Arr:
+        __cuda_array_interface__ = {"shape": (nbytes,), "typestr": "|u1", "data": (ptr, False), "version": 3}
+
+    return torch.as_tensor(_Arr(), device="cuda")

@@ -29,7 +29,7 @@ DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 
 HIP_SOURCES = ["hist.hip", "encode.hip", "decode.hip", "adm.hip", "batch.hip", "capi.hip"]
-HOST_SOURCES = ["gpu_ans_shim.cpp"]
+HOST_SOURCES = ["gpu_ans_shim.cpp", "serve.cpp"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -57,7 +57,7 @@ def build(verbose: bool = False) -> str:
         list(ex.map(_run, jobs))
     objs = [j[-1] for j in jobs]
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
     cli_tmp = CLI + ".tmp"
     _run([HIPCC, *HOST_FLAGS, os.path.join(CSRC, "compress_main.cpp"), "-o", cli_tmp,

@@ -450,6 +450,283 @@ hipError_t launch_pb(const BatchEncodeArgs& a, int cu, hipStream_t s, uint64_t* 
   return hipGetLastError();
 }
 
+// ---- batched decode --------------------------------------------------------
+constexpr uint32_t kDecRing = kR + 64;  // ring words + mirror of its head, per chain
+
+// LUT item: check buffer b's frame against its plan and build the packed LUT
+// (slot - cdf) << 20 | pdf << 8 | sym of every plane into global memory.
+template <int PB>
+__device__ void build_luts(const BatchBuf& B, uint32_t b, const BatchDecodeArgs& a, uint32_t gen, uint32_t* wl) {
+  constexpr uint32_t W = 1u << PB;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t P = B.eb;
+  bool bad = B.in_bytes < B.data_off;
+  if (!bad) {
+    const uint32_t* fh = (const uint32_t*)B.in;
+    const uint8_t* fb = B.in;
+    // magic, version, element width, mode 0, lanes, precision, planes, order,
+    // block bytes, n, data offset and section offsets must match the plan
+    bad = fh[0] != kFrameMagic || fb[4] != kFrameVersion || fb[5] != 0 || fb[6] != P || fb[7] != 0 ||
+          fb[8] != kL || fb[9] != PB || fb[10] != P || fb[11] != 0 || fh[3] != kBlock ||
+          ((const uint64_t*)B.in)[2] != B.n || ((const uint64_t*)B.in)[5] != B.data_off;
+    for (uint32_t k = 0; k < P && !bad; ++k) bad = ((const uint64_t*)B.in)[8 + k] != B.section[k];
+    const uint64_t data_words = ((const uint64_t*)B.in)[6];
+    bad = bad || B.data_off + 2ull * data_words > B.in_bytes;
+  }
+  uint32_t* cdf = wl;  // 257 words
+  for (uint32_t k = 0; k < P && !bad; ++k) {
+    const uint8_t* sec = B.in + B.section[k];
+    const uint32_t* h = (const uint32_t*)sec;
+    if ((h[0] >> 16) != kAnsMagic || (h[0] & 0xffffu) != 2u || (h[4] & 0xfu) != (uint32_t)PB || h[1] != B.nb ||
+        h[2] != (uint32_t)B.n) {
+      bad = true;
+      break;
+    }
+    const uint16_t* probs = (const uint16_t*)(sec + kAnsHeaderBytes);
+    uint32_t v[4], ls = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[j] = probs[4 * lane + j];
+      ls += v[j];
+    }
+    uint32_t inc = ls;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = __shfl_up(inc, o, 64);
+      if (lane >= (uint32_t)o) inc += u;
+    }
+    uint32_t run = inc - ls;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      cdf[4 * lane + j] = run;
+      run += v[j];
+    }
+    if (lane == 63) cdf[256] = run;
+    drain();
+    if (B.n > 0 && cdf[256] != W) {
+      bad = true;
+      break;
+    }
+    uint32_t* g = a.luts + ((uint64_t)b * 4u + k) * W;
+    for (uint32_t slot = 2 * lane; slot < W; slot += 128) {
+      uint32_t ent[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const uint32_t sl = slot + q;
+        uint32_t lo = 0, hi = kNumSymbols;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (cdf[mid] <= sl) lo = mid; else hi = mid;
+        }
+        const uint32_t c = cdf[lo], pdf = cdf[lo + 1] - c;
+        ent[q] = ((sl - c) << 20) | (pdf << 8) | lo;
+      }
+      atomic_st64((uint64_t*)(g + slot), ((uint64_t)ent[1] << 32) | ent[0]);
+    }
+    drain();
+  }
+  bad = __ballot(bad) != 0;
+  if (bad && lane == 0) atomicOr(a.err, kErrMagic);
+  drain();
+  if (lane == 0) atomic_st32(a.ready + b, gen | (bad ? kBatchBad : 0u));
+}
+
+// Decode ticket: units u0, u0 + 1 of buffer B with the 64-lane ring decoder.
+// SAME: two planes of the same elements (stored together as 16 bits at byte
+// kp[0] of each element); otherwise two blocks of a uint8 buffer.
+template <int PB, bool SAME>
+__device__ void decode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, const BatchDecodeArgs& a, uint32_t* wl) {
+  constexpr uint32_t W = 1u << PB;
+  constexpr uint32_t H4 = kH / 8;
+  constexpr uint32_t UNIT_CAP = kL * lane_max_words(kBlock / kL, PB);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t P = B.eb;
+  const uint64_t U = (uint64_t)P * B.nb;
+  const uint64_t u0 = 2ull * lt;
+  const uint64_t data_words = ((const uint64_t*)B.in)[6];
+  uint32_t* s_lut = wl;                                   // [2][W]
+  uint16_t* wstage = (uint16_t*)(wl + 2 * W);             // [2][R + 64]
+  uint32_t x[2], ptr[2], n[2], wds[2], taken[2], lo[2], kp[2];
+  bool act[2], pend[2];
+  uint64_t blk[2];
+  uint4 rv[2];
+  const uint4* src4[2];
+  const uint32_t* lut[2];
+  const uint16_t* stage[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const uint64_t u = u0 + c;
+    act[c] = u < U;
+    kp[c] = act[c] ? (uint32_t)(u % P) : 0u;
+    blk[c] = act[c] ? u / P : 0u;
+    const uint8_t* sec = B.in + B.section[kp[c]];
+    uint32_t nn = 0, words = 0, y = 0;
+    if (act[c]) {
+      const uint2 bw = *(const uint2*)(sec + kSectionFixed + 4ull * kL * B.nb + 8ull * blk[c]);
+      nn = bw.x >> 16;
+      words = bw.x & 0xffffu;
+      y = bw.y;
+      const uint64_t start = blk[c] * kBlock;
+      const uint64_t expect = (B.n - start) < kBlock ? (B.n - start) : kBlock;
+      if (nn != expect || words > UNIT_CAP || (uint64_t)y + round_up(words, 8) > data_words ||
+          B.data_off + 2ull * ((uint64_t)y + round_up(words, 8)) > B.in_bytes) {
+        if (lane == 0) atomicOr(a.err, kErrBlockWords);
+        act[c] = false;
+        nn = 0;
+        words = 0;
+      }
+    }
+    n[c] = nn;
+    words = wave_u32(words);
+    y = wave_u32(y);
+    // this plane's LUT
+    const uint64_t* g = (const uint64_t*)(a.luts + ((uint64_t)b * 4u + kp[c]) * W);
+    uint64_t* d = (uint64_t*)(s_lut + c * W);
+    for (uint32_t j = lane; j < W / 2; j += 64) d[j] = atomic_ld64(g + j);
+    lut[c] = s_lut + c * W;
+    src4[c] = (const uint4*)(B.in + B.data_off + 2ull * y);
+    uint16_t* reg = wstage + c * kDecRing;
+    stage[c] = reg;
+    lo[c] = 0;
+    pend[c] = false;
+    rv[c] = make_uint4(0, 0, 0, 0);
+    if (words > 0) {
+      const uint32_t jt = (words - 1) / kH;
+      const uint32_t j0 = jt > 0 ? jt - 1 : 0;
+      lo[c] = j0 * kH;
+      for (uint32_t j = j0; j <= jt; ++j) {
+        const uint32_t left = words - j * kH;
+        const uint32_t n4 = (uint32_t)div_up(left < kH ? left : kH, 8);
+        if (lane < n4) {
+          const uint4 q = src4[c][j * H4 + lane];
+          ((uint4*)(reg + (j & 1) * kH))[lane] = q;
+          if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + kR))[lane] = q;
+        }
+      }
+    }
+    x[c] = act[c] ? ((const uint32_t*)(sec + kSectionFixed))[blk[c] * kL + lane] : kMinState;
+    ptr[c] = words;
+    wds[c] = words;
+    taken[c] = 0;
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) __asm__ volatile("" : "+v"(x[c]));
+  drain();
+
+  auto refill = [&]() {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (!pend[c] && lo[c] > 0 && ptr[c] <= lo[c] + kH) {
+        const uint32_t j = lo[c] / kH - 1;
+        rv[c] = src4[c][j * H4 + lane];
+        pend[c] = true;
+      }
+      if (pend[c] && ptr[c] < lo[c] + 64u) {
+        const uint32_t j = lo[c] / kH - 1;
+        uint16_t* reg = wstage + c * kDecRing;
+        ((uint4*)(reg + (j & 1) * kH))[lane] = rv[c];
+        if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + kR))[lane] = rv[c];
+        lo[c] -= kH;
+        pend[c] = false;
+      }
+    }
+  };
+
+  const uint32_t n_min = wave_u32(n[0] < n[1] || !act[1] ? n[0] : n[1]);
+  const uint32_t n_max = wave_u32(n[0] > n[1] ? n[0] : n[1]);
+  const uint32_t full = act[1] ? n_min / kL : 0u;
+  const uint32_t all = (uint32_t)div_up(n_max, kL);
+  // element outputs: SAME stores 16 bits at byte kp[0] of each element
+  uint8_t* o0 = B.out + blk[0] * kBlock * P + kp[0];
+  uint8_t* o1 = B.out + blk[1] * kBlock * P + kp[1];
+  auto store = [&](uint32_t s, uint32_t el, bool v0, bool v1) {
+    const uint64_t i = (uint64_t)s * kL + lane;
+    if constexpr (SAME) {
+      if (v0) *(__attribute__((address_space(1))) uint16_t*)(o0 + i * P) = (uint16_t)el;
+    } else {
+      if (v0) *(__attribute__((address_space(1))) uint8_t*)(o0 + i) = (uint8_t)el;
+      if (v1) *(__attribute__((address_space(1))) uint8_t*)(o1 + i) = (uint8_t)(el >> 8);
+    }
+  };
+  for (uint32_t s = all; s > full;) {
+    --s;
+    const uint32_t i = s * kL + lane;
+    bool v[2] = {i < n[0], i < n[1]};
+    const uint32_t el = dec_steps<kL, PB, 2, kMask, uint32_t>(x, ptr, taken, lut, stage, v, 0u);
+    store(s, el, v[0], v[1]);
+    refill();
+  }
+  const bool vall[2] = {true, true};
+  for (uint32_t s = full; s > 0;) {
+    --s;
+    const uint32_t el = dec_steps<kL, PB, 2, kMask, uint32_t>(x, ptr, taken, lut, stage, vall, 0u);
+    store(s, el, true, true);
+    refill();
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const bool bad = act[c] && (taken[c] != wds[c] || x[c] != kMinState);
+    if (__ballot(bad) && lane == 0) atomicOr(a.err, kErrFinalState);
+  }
+}
+
+template <int PB>
+__global__ __launch_bounds__(64 * kNW) void batch_decode_kernel(BatchDecodeArgs a) {
+  constexpr uint32_t W = 1u << PB;
+  constexpr uint32_t WAVE_WORDS = 2 * W + kDecRing;  // two LUTs | two rings (u16)
+  extern __shared__ uint32_t lds[];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = wave_u32(threadIdx.x >> 6);
+  uint32_t* wl = lds + wave * WAVE_WORDS;
+  const uint32_t gen = wave_u32(lane == 0 ? atomic_ld32(a.queue + 2) : 0u);
+  uint32_t it = 0;
+  if (lane == 0) it = atomicAdd(a.queue, 1u);
+  it = wave_u32(it);
+  while (it < a.nitems) {
+    uint32_t nxt = 0;
+    if (lane == 0) nxt = atomicAdd(a.queue, 1u);
+    const uint32_t b = wave_u32(a.item_buf[it]);
+    const BatchBuf& B = a.bufs[b];
+    if (it < a.nbufs) {
+      build_luts<PB>(B, b, a, gen, wl);
+    } else {
+      uint32_t r = 0;
+      while (((r = wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u)) & ~kBatchBad) != gen)
+        __builtin_amdgcn_s_sleep(2);
+      if (!(r & kBatchBad)) {
+        const uint32_t lt = it - a.nbufs - B.tick_first;
+        if (B.eb == 1) decode_ticket<PB, false>(B, b, lt, a, wl);
+        else decode_ticket<PB, true>(B, b, lt, a, wl);
+      }
+    }
+    drain();
+    it = wave_u32(nxt);
+  }
+  if (lane == 0) {
+    const uint32_t total = gridDim.x * kNW;
+    if (atomicAdd(a.queue + 1, 1u) + 1 == total) {
+      atomic_st32(a.queue, 0u);
+      atomic_st32(a.queue + 1, 0u);
+      atomic_st32(a.queue + 2, gen % kBatchGenMask + 1u);
+    }
+  }
+}
+
+template <int PB>
+hipError_t launch_dec_pb(const BatchDecodeArgs& a, int cu, hipStream_t s) {
+  const size_t lds = (size_t)kNW * (2 * (1u << PB) + kDecRing) * 4u;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_decode_kernel<PB>, 64 * kNW, lds);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  uint64_t grid = (uint64_t)cu * (uint64_t)per_cu;
+  const uint64_t want = div_up(a.nitems, kNW);
+  if (want < grid) grid = want;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(batch_decode_kernel<PB>, dim3((uint32_t)grid), dim3(64 * kNW), lds, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_batch_encode(const BatchEncodeArgs& a, int pb, int cu, hipStream_t s, uint64_t* need_spill) {
@@ -461,4 +738,15 @@ hipError_t launch_batch_encode(const BatchEncodeArgs& a, int pb, int cu, hipStre
   return hipErrorInvalidValue;
 }
 
+}  // namespace mans
+
+namespace mans {
+hipError_t launch_batch_decode(const BatchDecodeArgs& a, int pb, int cu, hipStream_t s) {
+  switch (pb) {
+    case 9: return launch_dec_pb<9>(a, cu, s);
+    case 10: return launch_dec_pb<10>(a, cu, s);
+    case 11: return launch_dec_pb<11>(a, cu, s);
+  }
+  return hipErrorInvalidValue;
+}
 }  // namespace mans

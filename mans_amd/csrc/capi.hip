@@ -765,85 +765,61 @@ mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header
 struct mans_batch {
   mans_workspace* ws = nullptr;
   int pb = 10;
+  bool decode = false;
+  // capacity
+  uint32_t cap_bufs = 0;
+  uint64_t cap_items = 0, cap_ticks = 0;
+  // current plan
   uint32_t nbufs = 0, nhist = 0, nitems = 0, nticks = 0;
-  BatchBuf* bufs = nullptr;   // device
-  uint32_t* item_buf = nullptr;
-  uint32_t* hist = nullptr;   // [nbufs][4][256]
-  uint32_t* done = nullptr;   // [nbufs]
-  uint32_t* ready = nullptr;  // [nbufs]
-  uint4* tables = nullptr;    // [nbufs][4][256]
-  uint64_t* status = nullptr; // [max(nticks, 1)]
-  uint32_t* ptot = nullptr;   // [nbufs][4]
-  uint32_t* fin = nullptr;    // [nbufs]
+  BatchBuf* bufs = nullptr;   // device [cap_bufs]
+  uint32_t* item_buf = nullptr;  // device [cap_items]
+  uint32_t* hist = nullptr;   // [cap_bufs][4][256]
+  uint32_t* done = nullptr;   // [cap_bufs]
+  uint32_t* ready = nullptr;  // [cap_bufs]
+  uint4* tables = nullptr;    // [cap_bufs][4][256]
+  uint64_t* status = nullptr; // [cap_ticks]
+  uint32_t* ptot = nullptr;   // [cap_bufs][4]
+  uint32_t* fin = nullptr;    // [cap_bufs]
   uint32_t* queue = nullptr;  // [3] next item, exited waves, generation
+  uint32_t* luts = nullptr;   // decode: [cap_bufs][4][2^p]
+  BatchBuf* h_bufs = nullptr; // pinned staging for asynchronous plans (or null)
+  uint32_t* h_items = nullptr;
   uint64_t spill = 0;
 };
 
 namespace {
 
 void batch_free(mans_batch* b) {
-  if (b->bufs) (void)hipFree(b->bufs);
-  if (b->item_buf) (void)hipFree(b->item_buf);
-  if (b->hist) (void)hipFree(b->hist);
-  if (b->done) (void)hipFree(b->done);
-  if (b->ready) (void)hipFree(b->ready);
-  if (b->tables) (void)hipFree(b->tables);
-  if (b->status) (void)hipFree(b->status);
-  if (b->ptot) (void)hipFree(b->ptot);
-  if (b->fin) (void)hipFree(b->fin);
-  if (b->queue) (void)hipFree(b->queue);
+  void* dev[] = {b->bufs, b->item_buf, b->hist, b->done, b->ready, b->tables,
+                 b->status, b->ptot, b->fin, b->queue, b->luts};
+  for (void* q : dev)
+    if (q) (void)hipFree(q);
+  if (b->h_bufs) (void)hipHostFree(b->h_bufs);
+  if (b->h_items) (void)hipHostFree(b->h_items);
   delete b;
 }
 
-BatchEncodeArgs batch_args(const mans_batch* b, uint64_t* d_sizes) {
-  BatchEncodeArgs a;
-  memset(&a, 0, sizeof(a));
-  a.bufs = b->bufs;
-  a.item_buf = b->item_buf;
-  a.nbufs = b->nbufs;
-  a.nhist_items = b->nhist;
-  a.nitems = b->nitems;
-  a.hist = b->hist;
-  a.done = b->done;
-  a.ready = b->ready;
-  a.tables = b->tables;
-  a.status = b->status;
-  a.ptot = b->ptot;
-  a.fin = b->fin;
-  a.queue = b->queue;
-  a.err = b->ws->counters + 3;
-  a.sizes = d_sizes;
-  a.spill = b->ws->spill;
-  a.spill_bytes = b->ws->spill_cap;
-  return a;
-}
-
-}  // namespace
-
-extern "C" {
-
-mans_status mans_batch_encode_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs,
-                                     int32_t count, mans_workspace* ws) {
-  if (!out) return MANS_ERR_INVALID_ARG;
-  *out = nullptr;
-  mans_status st = check_params(p);
-  if (st != MANS_OK) return st;
-  if (!ws || count < 0 || (count > 0 && !h_bufs) || p->lanes != 64) return MANS_ERR_INVALID_ARG;
+// Lays out every buffer of a batch as the 64-lane byte-plane frame of
+// mans_frame_encode and counts histogram items and tickets (two units each).
+mans_status plan_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, bool decode,
+                       std::vector<BatchBuf>& bufs, uint64_t& nhist, uint64_t& nticks) {
   const Format f = format_for_lanes(64);
-  std::vector<BatchBuf> bufs((size_t)count);
-  std::vector<uint32_t> item_buf;
-  uint64_t nhist = 0, nticks = 0;
+  bufs.assign((size_t)count, BatchBuf{});
+  nhist = nticks = 0;
   for (int32_t i = 0; i < count; ++i) {
     const mans_buffer& m = h_bufs[i];
     const uint32_t eb = (uint32_t)m.elem_bytes;
     if (eb != 1 && eb != 2 && eb != 4) return MANS_ERR_INVALID_ARG;
-    if (!m.d_out || (m.n_elems > 0 && !m.d_in)) return MANS_ERR_INVALID_ARG;
+    if (m.reserved != 0) return MANS_ERR_INVALID_ARG;
+    if ((m.n_elems > 0 && !m.d_out) || (!decode && !m.d_out) || (m.n_elems > 0 && !m.d_in) || (decode && !m.d_in))
+      return MANS_ERR_INVALID_ARG;
     if (!aligned(m.d_in, 16) || !aligned(m.d_out, 16)) return MANS_ERR_ALIGNMENT;
     if (m.n_elems > 0xffffffffull) return MANS_ERR_TOO_LARGE;
     mans_params q = *p;
     q.elem_bytes = (int32_t)eb;
     q.mode = MANS_MODE_PLANES;
-    if (m.out_capacity < mans_frame_max_size(m.n_elems, &q)) return MANS_ERR_CAPACITY;
+    if (!decode && m.out_capacity < mans_frame_max_size(m.n_elems, &q)) return MANS_ERR_CAPACITY;
+    if (decode && m.out_capacity < m.n_elems * eb) return MANS_ERR_CAPACITY;
     CodecDesc d;
     memset(&d, 0, sizeof(d));
     d.prob_bits = p->prob_bits;
@@ -870,63 +846,196 @@ mans_status mans_batch_encode_create(mans_batch** out, const mans_params* p, con
     B.nhist = (uint32_t)std::max<uint64_t>(1, div_up(m.n_elems * eb, kBatchHistChunk));
     B.tick_first = (uint32_t)nticks;
     B.nticks = (uint32_t)div_up(eb * nb, 2);
+    B.in_bytes = decode ? m.in_bytes : 0;
     frame_header(B.frame_hdr, m.n_elems, eb, MANS_MODE_PLANES, f, p->prob_bits, d, 0, 0);
     put64(B.frame_hdr, 56, off);  // total size of an empty frame
     nhist += B.nhist;
     nticks += B.nticks;
-    if (nhist + nticks > 0x7fffffffull) return MANS_ERR_TOO_LARGE;
+    if (nhist + nticks + (uint64_t)count > 0x7fffffffull) return MANS_ERR_TOO_LARGE;
   }
-  item_buf.reserve(nhist + nticks);
-  for (int32_t i = 0; i < count; ++i)
-    for (uint32_t k = 0; k < bufs[(size_t)i].nhist; ++k) item_buf.push_back((uint32_t)i);
-  for (int32_t i = 0; i < count; ++i)
-    for (uint32_t k = 0; k < bufs[(size_t)i].nticks; ++k) item_buf.push_back((uint32_t)i);
+  return MANS_OK;
+}
 
+// Allocates a batch able to hold cap_bufs buffers, cap_items work items and
+// cap_ticks tickets; every self-cleaning array starts at zero and the
+// generation at 1.  pinned: keep host staging for asynchronous plans.
+mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, uint64_t cap_ticks, bool pinned) {
+  const size_t nb1 = std::max<size_t>(1, cap_bufs);
+  b->cap_bufs = cap_bufs;
+  b->cap_items = cap_items;
+  b->cap_ticks = cap_ticks;
+  HIP_TRY(hipMalloc(&b->bufs, nb1 * sizeof(BatchBuf)));
+  HIP_TRY(hipMalloc(&b->item_buf, std::max<size_t>(1, cap_items) * 4));
+  HIP_TRY(hipMalloc(&b->ready, nb1 * 4));
+  HIP_TRY(hipMalloc(&b->queue, 12));
+  HIP_TRY(hipMemset(b->ready, 0, nb1 * 4));
+  const uint32_t q0[3] = {0, 0, 1};  // next item, exited waves, first generation
+  HIP_TRY(hipMemcpy(b->queue, q0, 12, hipMemcpyHostToDevice));
+  if (b->decode) {
+    HIP_TRY(hipMalloc(&b->luts, nb1 * 4 * (4ull << b->pb)));
+  } else {
+    HIP_TRY(hipMalloc(&b->hist, nb1 * 4096));
+    HIP_TRY(hipMalloc(&b->done, nb1 * 4));
+    HIP_TRY(hipMalloc(&b->tables, nb1 * 4 * 256 * sizeof(uint4)));
+    HIP_TRY(hipMalloc(&b->status, std::max<uint64_t>(1, cap_ticks) * 8));
+    HIP_TRY(hipMalloc(&b->ptot, nb1 * 16));
+    HIP_TRY(hipMalloc(&b->fin, nb1 * 4));
+    HIP_TRY(hipMemset(b->hist, 0, nb1 * 4096));
+    HIP_TRY(hipMemset(b->done, 0, nb1 * 4));
+    HIP_TRY(hipMemset(b->status, 0, std::max<uint64_t>(1, cap_ticks) * 8));
+    HIP_TRY(hipMemset(b->ptot, 0, nb1 * 16));
+    HIP_TRY(hipMemset(b->fin, 0, nb1 * 4));
+  }
+  if (pinned) {
+    HIP_TRY(hipHostMalloc((void**)&b->h_bufs, nb1 * sizeof(BatchBuf), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&b->h_items, std::max<size_t>(1, cap_items) * 4, hipHostMallocDefault));
+  }
+  return MANS_OK;
+}
+
+// Plans `count` buffers into b (which must hold them) and uploads the
+// descriptors: on stream s from the pinned staging when b has one (the
+// staging must not be in use by an earlier upload), else synchronously.
+mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h_bufs, int32_t count, hipStream_t s) {
+  std::vector<BatchBuf> bufs;
+  uint64_t nhist = 0, nticks = 0;
+  mans_status st = plan_batch(p, h_bufs, count, b->decode, bufs, nhist, nticks);
+  if (st != MANS_OK) return st;
+  const uint64_t nitems = (b->decode ? (uint64_t)count : nhist) + nticks;
+  if ((uint32_t)count > b->cap_bufs || nitems > b->cap_items || nticks > b->cap_ticks) return MANS_ERR_TOO_LARGE;
+  std::vector<uint32_t> items;
+  uint32_t* dst = b->h_items;
+  if (!dst) {
+    items.resize(std::max<size_t>(1, nitems));
+    dst = items.data();
+  }
+  uint64_t k = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    const uint32_t reps = b->decode ? 1u : bufs[(size_t)i].nhist;
+    for (uint32_t r = 0; r < reps; ++r) dst[k++] = (uint32_t)i;
+  }
+  for (int32_t i = 0; i < count; ++i)
+    for (uint32_t r = 0; r < bufs[(size_t)i].nticks; ++r) dst[k++] = (uint32_t)i;
+  if (b->h_bufs) {
+    if (count > 0) memcpy(b->h_bufs, bufs.data(), bufs.size() * sizeof(BatchBuf));
+    if (count > 0) HIP_TRY(hipMemcpyAsync(b->bufs, b->h_bufs, bufs.size() * sizeof(BatchBuf), hipMemcpyHostToDevice, s));
+    if (nitems > 0) HIP_TRY(hipMemcpyAsync(b->item_buf, dst, nitems * 4, hipMemcpyHostToDevice, s));
+  } else {
+    if (count > 0) HIP_TRY(hipMemcpy(b->bufs, bufs.data(), bufs.size() * sizeof(BatchBuf), hipMemcpyHostToDevice));
+    if (nitems > 0) HIP_TRY(hipMemcpy(b->item_buf, dst, nitems * 4, hipMemcpyHostToDevice));
+  }
+  b->pb = p->prob_bits;
+  b->nbufs = (uint32_t)count;
+  b->nhist = b->decode ? 0u : (uint32_t)nhist;
+  b->nticks = (uint32_t)nticks;
+  b->nitems = (uint32_t)nitems;
+  return MANS_OK;
+}
+
+BatchEncodeArgs batch_args(const mans_batch* b, uint64_t* d_sizes) {
+  BatchEncodeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.bufs = b->bufs;
+  a.item_buf = b->item_buf;
+  a.nbufs = b->nbufs;
+  a.nhist_items = b->nhist;
+  a.nitems = b->nitems;
+  a.hist = b->hist;
+  a.done = b->done;
+  a.ready = b->ready;
+  a.tables = b->tables;
+  a.status = b->status;
+  a.ptot = b->ptot;
+  a.fin = b->fin;
+  a.queue = b->queue;
+  a.err = b->ws->counters + 3;
+  a.sizes = d_sizes;
+  a.spill = b->ws->spill;
+  a.spill_bytes = b->ws->spill_cap;
+  return a;
+}
+
+mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs, int32_t count,
+                         mans_workspace* ws, bool decode) {
+  if (!out) return MANS_ERR_INVALID_ARG;
+  *out = nullptr;
+  mans_status st = check_params(p);
+  if (st != MANS_OK) return st;
+  if (!ws || count < 0 || (count > 0 && !h_bufs) || p->lanes != 64) return MANS_ERR_INVALID_ARG;
+  std::vector<BatchBuf> bufs;
+  uint64_t nhist = 0, nticks = 0;
+  st = plan_batch(p, h_bufs, count, decode, bufs, nhist, nticks);
+  if (st != MANS_OK) return st;
   DeviceGuard g(ws->device);
   mans_batch* b = new mans_batch();
   b->ws = ws;
   b->pb = p->prob_bits;
-  b->nbufs = (uint32_t)count;
-  b->nhist = (uint32_t)nhist;
-  b->nticks = (uint32_t)nticks;
-  b->nitems = (uint32_t)(nhist + nticks);
-  const size_t nb1 = std::max<size_t>(1, (size_t)count);
-  auto fail = [&](mans_status s) {
+  b->decode = decode;
+  st = batch_alloc(b, (uint32_t)count, (decode ? (uint64_t)count : nhist) + nticks, nticks, false);
+  if (st == MANS_OK) st = batch_plan(b, p, h_bufs, count, nullptr);
+  if (st == MANS_OK && !decode && b->nitems > 0) {
+    if (launch_batch_encode(batch_args(b, nullptr), b->pb, ws->cu, nullptr, &b->spill) != hipSuccess) st = MANS_ERR_HIP;
+    if (st == MANS_OK) st = ensure_spill(ws, b->spill);
+  }
+  if (st != MANS_OK) {
     batch_free(b);
-    return s;
-  };
-  if (hipMalloc(&b->bufs, nb1 * sizeof(BatchBuf)) != hipSuccess ||
-      hipMalloc(&b->item_buf, std::max<size_t>(1, item_buf.size()) * 4) != hipSuccess ||
-      hipMalloc(&b->hist, nb1 * 4096) != hipSuccess || hipMalloc(&b->done, nb1 * 4) != hipSuccess ||
-      hipMalloc(&b->ready, nb1 * 4) != hipSuccess || hipMalloc(&b->tables, nb1 * 4 * 256 * sizeof(uint4)) != hipSuccess ||
-      hipMalloc(&b->status, std::max<uint64_t>(1, nticks) * 8) != hipSuccess || hipMalloc(&b->queue, 12) != hipSuccess ||
-      hipMalloc(&b->ptot, nb1 * 16) != hipSuccess || hipMalloc(&b->fin, nb1 * 4) != hipSuccess)
-    return fail(MANS_ERR_HIP);
-  if (count > 0 && hipMemcpy(b->bufs, bufs.data(), bufs.size() * sizeof(BatchBuf), hipMemcpyHostToDevice) != hipSuccess)
-    return fail(MANS_ERR_HIP);
-  if (!item_buf.empty() &&
-      hipMemcpy(b->item_buf, item_buf.data(), item_buf.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-    return fail(MANS_ERR_HIP);
-  if (hipMemset(b->hist, 0, nb1 * 4096) != hipSuccess || hipMemset(b->done, 0, nb1 * 4) != hipSuccess ||
-      hipMemset(b->ready, 0, nb1 * 4) != hipSuccess ||
-      hipMemset(b->status, 0, std::max<uint64_t>(1, nticks) * 8) != hipSuccess ||
-      hipMemset(b->queue, 0, 12) != hipSuccess || hipMemset(b->ptot, 0, nb1 * 16) != hipSuccess ||
-      hipMemset(b->fin, 0, nb1 * 4) != hipSuccess)
-    return fail(MANS_ERR_HIP);
-  const uint32_t gen1 = 1;  // generation of the first launch (0 marks nothing published)
-  if (hipMemcpy(b->queue + 2, &gen1, 4, hipMemcpyHostToDevice) != hipSuccess) return fail(MANS_ERR_HIP);
-  if (b->nitems > 0) {
-    if (launch_batch_encode(batch_args(b, nullptr), b->pb, ws->cu, nullptr, &b->spill) != hipSuccess)
-      return fail(MANS_ERR_HIP);
-    st = ensure_spill(ws, b->spill);
-    if (st != MANS_OK) return fail(st);
+    return st;
   }
   *out = b;
   return MANS_OK;
 }
 
+}  // namespace
+
+// ---- batches for the request server (mans_amd/csrc/serve.cpp) ---------------
+mans_status batch_slot_create(mans_batch** out, const mans_params* p, bool decode, uint32_t cap_bufs,
+                              uint64_t cap_bytes, mans_workspace* ws) {
+  if (!out || !ws) return MANS_ERR_INVALID_ARG;
+  *out = nullptr;
+  mans_status st = check_params(p);
+  if (st != MANS_OK) return st;
+  if (p->lanes != 64) return MANS_ERR_INVALID_ARG;
+  DeviceGuard g(ws->device);
+  mans_batch* b = new mans_batch();
+  b->ws = ws;
+  b->pb = p->prob_bits;
+  b->decode = decode;
+  // units of cap_bytes in 8 KiB blocks, plus a partial block per plane and
+  // buffer; tickets hold two units
+  const uint64_t units = cap_bytes / 8192 + 4ull * cap_bufs;
+  const uint64_t ticks = units / 2 + cap_bufs;
+  const uint64_t hist = cap_bytes / kBatchHistChunk + cap_bufs;
+  st = batch_alloc(b, cap_bufs, (decode ? cap_bufs : hist) + ticks, ticks, true);
+  if (st == MANS_OK && !decode) {
+    BatchEncodeArgs a = batch_args(b, nullptr);
+    a.nitems = 0x7fffffffu;
+    if (launch_batch_encode(a, b->pb, ws->cu, nullptr, &b->spill) != hipSuccess) st = MANS_ERR_HIP;
+    if (st == MANS_OK) st = ensure_spill(ws, b->spill);
+  }
+  if (st != MANS_OK) {
+    batch_free(b);
+    return st;
+  }
+  *out = b;
+  return MANS_OK;
+}
+
+mans_status batch_slot_plan(mans_batch* b, const mans_params* p, const mans_buffer* bufs, int32_t count,
+                            hipStream_t s) {
+  if (!b) return MANS_ERR_INVALID_ARG;
+  DeviceGuard g(b->ws->device);
+  return batch_plan(b, p, bufs, count, s);
+}
+
+extern "C" {
+
+mans_status mans_batch_encode_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs,
+                                     int32_t count, mans_workspace* ws) {
+  return batch_create(out, p, h_bufs, count, ws, false);
+}
+
 mans_status mans_batch_encode_run(mans_batch* b, uint64_t* d_sizes, void* stream) {
-  if (!b || (b->nbufs > 0 && !d_sizes)) return MANS_ERR_INVALID_ARG;
+  if (!b || b->decode || (b->nbufs > 0 && !d_sizes)) return MANS_ERR_INVALID_ARG;
   if (b->nitems == 0) return MANS_OK;
   mans_workspace* ws = b->ws;
   DeviceGuard g(ws->device);
@@ -934,6 +1043,33 @@ mans_status mans_batch_encode_run(mans_batch* b, uint64_t* d_sizes, void* stream
   if (b->spill > ws->spill_cap) return MANS_ERR_INVALID_ARG;  // the workspace was replaced under the batch
   prof_mark(ws, s, "batch_encode");
   HIP_TRY(launch_batch_encode(batch_args(b, d_sizes), b->pb, ws->cu, s, nullptr));
+  prof_mark(ws, s, nullptr);
+  return MANS_OK;
+}
+
+mans_status mans_batch_decode_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs,
+                                     int32_t count, mans_workspace* ws) {
+  return batch_create(out, p, h_bufs, count, ws, true);
+}
+
+mans_status mans_batch_decode_run(mans_batch* b, void* stream) {
+  if (!b || !b->decode) return MANS_ERR_INVALID_ARG;
+  if (b->nitems == 0) return MANS_OK;
+  mans_workspace* ws = b->ws;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  BatchDecodeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.bufs = b->bufs;
+  a.item_buf = b->item_buf;
+  a.nbufs = b->nbufs;
+  a.nitems = b->nitems;
+  a.ready = b->ready;
+  a.luts = b->luts;
+  a.queue = b->queue;
+  a.err = ws->counters + 3;
+  prof_mark(ws, s, "batch_decode");
+  HIP_TRY(launch_batch_decode(a, b->pb, ws->cu, s));
   prof_mark(ws, s, nullptr);
   return MANS_OK;
 }
@@ -952,6 +1088,17 @@ mans_status mans_encode_batch(const mans_params* p, const mans_buffer* h_bufs, i
   mans_status st = mans_batch_encode_create(&b, p, h_bufs, count, ws);
   if (st != MANS_OK) return st;
   st = mans_batch_encode_run(b, d_sizes, stream);
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess && st == MANS_OK) st = MANS_ERR_HIP;
+  mans_batch_destroy(b);
+  return st;
+}
+
+mans_status mans_decode_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, mans_workspace* ws,
+                              void* stream) {
+  mans_batch* b = nullptr;
+  mans_status st = mans_batch_decode_create(&b, p, h_bufs, count, ws);
+  if (st != MANS_OK) return st;
+  st = mans_batch_decode_run(b, stream);
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess && st == MANS_OK) st = MANS_ERR_HIP;
   mans_batch_destroy(b);
   return st;

@@ -116,20 +116,32 @@ __device__ __forceinline__ uint64_t look_back(uint64_t* status, uint64_t chain, 
 // ring's mirror covers.  The three phases (LUT reads; state update and word
 // reads; selects) each run over all chains, so one chain's LDS latency
 // overlaps the others' work.
-template <int L, int PB, int CH, uint32_t MASK>
+// Decoder LUT entries.  uint2 {pdf | sym << 24, slot - cdf}: the 24-bit
+// multiply ignores the symbol byte, so the update is one shift and one
+// multiply-add.  uint32_t (slot - cdf) << 20 | pdf << 8 | sym: half the LDS,
+// two more operations per symbol.
+template <int PB>
+__device__ __forceinline__ uint32_t lut_next(uint2 e, uint32_t x) { return __umul24(e.x, x >> PB) + e.y; }
+template <int PB>
+__device__ __forceinline__ uint32_t lut_next(uint32_t e, uint32_t x) {
+  return __umul24((e >> 8) & 0xfffu, x >> PB) + (e >> 20);
+}
+__device__ __forceinline__ uint32_t lut_sym(uint2 e) { return e.x >> 24; }
+__device__ __forceinline__ uint32_t lut_sym(uint32_t e) { return e & 0xffu; }
+
+template <int L, int PB, int CH, uint32_t MASK, typename LE = uint2>
 __device__ __forceinline__ uint32_t dec_steps(uint32_t (&x)[CH], uint32_t (&ptr)[CH], uint32_t (&taken)[CH],
-                                              const uint2* const (&lut)[CH], const uint16_t* const (&stage)[CH],
+                                              const LE* const (&lut)[CH], const uint16_t* const (&stage)[CH],
                                               const bool (&v)[CH], uint32_t half) {
   constexpr uint32_t W = 1u << PB;
-  uint2 e[CH];
+  LE e[CH];
   uint32_t word[CH];
   bool r[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) e[c] = lut[c][x[c] & (W - 1u)];
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
-    // the 24-bit multiply ignores the symbol in bits 24..31 of e.x
-    const uint32_t xn = __umul24(e[c].x, x[c] >> PB) + e[c].y;
+    const uint32_t xn = lut_next<PB>(e[c], x[c]);
     x[c] = v[c] ? xn : x[c];
     r[c] = v[c] && (x[c] < kMinState);
     const uint64_t m = __ballot(r[c]);
@@ -158,7 +170,7 @@ __device__ __forceinline__ uint32_t dec_steps(uint32_t (&x)[CH], uint32_t (&ptr)
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     x[c] = r[c] ? (x[c] << 16) | word[c] : x[c];
-    el |= (e[c].x >> 24) << (8 * c);
+    el |= lut_sym(e[c]) << (8 * c);
   }
   return el;
 }

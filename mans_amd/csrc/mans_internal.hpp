@@ -213,8 +213,9 @@ struct BatchBuf {
   uint32_t hist_first;           // first histogram item of the buffer
   uint32_t nhist;                // histogram items (at least 1)
   uint32_t tick_first;           // first encode ticket (ticket = 2 units)
-  uint32_t nticks;               // encode tickets
+  uint32_t nticks;               // encode (decode) tickets
   uint32_t pad_;
+  uint64_t in_bytes;             // decode: bytes readable at `in` (the frame)
   uint32_t frame_hdr[32];        // frame header (data_words / total_size set by the kernel)
 };
 
@@ -239,12 +240,26 @@ struct BatchEncodeArgs {
   uint16_t* spill;               // per-wave spill areas
   uint64_t spill_bytes;
 };
+// Batched decode: BatchBuf.in is the frame, .out the elements; items
+// [0, nbufs) build buffer b's LUTs, then come the decode tickets.
+struct BatchDecodeArgs {
+  const BatchBuf* bufs;
+  const uint32_t* item_buf;      // [nitems] item -> buffer
+  uint32_t nbufs;
+  uint32_t nitems;
+  uint32_t* ready;               // [nbufs] generation whose LUTs are built (| kBatchBad: frame rejected)
+  uint32_t* luts;                // [nbufs][4][2^p] packed LUT entries
+  uint32_t* queue;               // [3] as BatchEncodeArgs::queue
+  uint32_t* err;
+};
+constexpr uint32_t kBatchBad = 1u << 31;
 constexpr uint32_t kBatchGenMask = 0x3fffu;     // 14-bit generations
 constexpr uint32_t kBatchHistChunk = 1u << 18;  // bytes per histogram item
 
 hipError_t launch_hist(const HistArgs& a, int cu, hipStream_t s);
 // batched encode; with need_spill set, only reports the spill bytes needed
 hipError_t launch_batch_encode(const BatchEncodeArgs& a, int pb, int cu, hipStream_t s, uint64_t* need_spill);
+hipError_t launch_batch_decode(const BatchDecodeArgs& a, int pb, int cu, hipStream_t s);
 hipError_t launch_encode(const EncodeArgs& a, int cu, hipStream_t s);
 // bytes of spill area launch_encode needs for a (0 for the 32-lane format)
 hipError_t encode_spill_bytes(const EncodeArgs& a, int cu, uint64_t* bytes);

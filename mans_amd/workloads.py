@@ -12,6 +12,14 @@
 * zipf_u32         -- Zipf(s) ranks over 2^20 values mapped through a seeded
                       permutation to 32-bit words with the high byte in [0, 15].
 * detector_u16     -- frames of round(N(1000, 8)) + 40 * Poisson(0.05).
+* batch_buffers    -- independent buffers: sizes log-uniform in [4 KiB, 16 MiB],
+                      element width uint8 / uint16 / uint32 at 20 / 60 / 20 %,
+                      values 50 / 50 from a Laplace source (b log-uniform in
+                      [1, 64], centred at half the range, clipped) or the Zipf
+                      source (s = 1.1 ranks over 2^20 values through a seeded
+                      permutation to words with the high byte in [0, 15], low
+                      bytes kept for narrower widths); buffer i from seed
+                      1000 + i.  Generated on the GPU.
 """
 from __future__ import annotations
 
@@ -81,3 +89,40 @@ def detector_u16(frames: int = 64, side: int = 2048, seed: int = 3) -> np.ndarra
     ped = np.round(rng.normal(1000.0, 8.0, n))
     photons = rng.poisson(0.05, n) * 40
     return np.clip(ped + photons, 0, 65535).astype(np.uint16)
+
+
+def batch_buffers(count: int = 4096, seed_base: int = 1000, lo: int = 4096, hi: int = 16 << 20,
+                  device: str = "cuda"):
+    """List of `count` CUDA tensors (int8/int16/int32 views of the unsigned
+    values) and their element widths."""
+    import torch
+
+    # Zipf source shared by all buffers: inverse-CDF table over 2^20 ranks
+    zr = np.random.default_rng(2)
+    words = torch.from_numpy(zr.integers(0, 1 << 28, 1 << 20, dtype=np.int64)).to(device)
+    w = torch.arange(1, (1 << 20) + 1, device=device, dtype=torch.float64).pow(-1.1)
+    cdf = torch.cumsum(w, 0)
+    cdf /= cdf[-1].clone()
+    out, widths = [], []
+    dt = {1: torch.int8, 2: torch.int16, 4: torch.int32}
+    for i in range(count):
+        rng = np.random.default_rng(seed_base + i)
+        eb = int(rng.choice([1, 2, 4], p=[0.2, 0.6, 0.2]))
+        nbytes = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        n = max(1, nbytes // eb)
+        g = torch.Generator(device=device)
+        g.manual_seed(seed_base + i)
+        bits = 8 * eb
+        if rng.random() < 0.5:
+            b = float(np.exp(rng.uniform(0.0, np.log(64.0))))
+            u = torch.rand(n, generator=g, device=device, dtype=torch.float64) - 0.5
+            lap = -b * torch.sign(u) * torch.log1p(-2.0 * u.abs())
+            v = torch.clamp(torch.round(lap).to(torch.int64) + (1 << (bits - 1)), 0, (1 << bits) - 1)
+        else:
+            u = torch.rand(n, generator=g, device=device, dtype=torch.float64)
+            r = torch.clamp(torch.searchsorted(cdf, u), max=(1 << 20) - 1)
+            v = words[r] & ((1 << bits) - 1)
+        v = torch.where(v >= (1 << (bits - 1)), v - (1 << bits), v).to(dt[eb])
+        out.append(v)
+        widths.append(eb)
+    return out, widths

@@ -114,3 +114,60 @@ def test_batch_graph_replay(ws):
             assert torch.equal(b.outs[i][:n], eager[i][:n]), (rep, i)
     ws.check()
     b.close()
+
+
+def test_batch_decode_round_trip(ws):
+    rng = np.random.default_rng(11)
+    sizes = EDGE_SIZES + [int(v) for v in np.exp(rng.uniform(np.log(1 << 10), np.log(1 << 21), 30))]
+    xs, ts = make_batch(11, sizes)
+    p = mans_amd.params(10, 64)
+    enc = mans_amd.EncodeBatch(ts, p, ws)
+    sizes_h = enc.run().cpu().tolist()
+    ws.check()
+    print(f"batch decode buffers={len(xs)}", flush=True)
+    dec = mans_amd.DecodeBatch(enc.outs, sizes_h, ts, p, ws)
+    outs = dec.run()
+    torch.cuda.synchronize()
+    ws.check()
+    for i, x in enumerate(xs):
+        assert (outs[i].cpu().numpy().view(x.dtype) == x).all(), (i, x.size, x.itemsize)
+    # again, from a captured graph
+    for o in dec.outs:
+        o.zero_()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        dec.run(stream=s)
+    for rep in range(2):
+        for o in dec.outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for i, x in enumerate(xs):
+            got = dec.outs[i][:x.size].cpu().numpy().view(x.dtype)
+            assert (got == x).all(), (rep, i)
+    ws.check()
+    dec.close()
+    enc.close()
+
+
+def test_batch_decode_flags_a_corrupt_frame(ws):
+    xs, ts = make_batch(12, [50_000, 70_001, 9_000])
+    p = mans_amd.params(10, 64)
+    enc = mans_amd.EncodeBatch(ts, p, ws)
+    sizes_h = enc.run().cpu().tolist()
+    ws.check()
+    bad = enc.outs[1].clone()
+    bad[0] ^= 0xFF  # frame magic
+    frames = [enc.outs[0], bad, enc.outs[2]]
+    dec = mans_amd.DecodeBatch(frames, sizes_h, ts, p, ws)
+    outs = dec.run()
+    torch.cuda.synchronize()
+    with pytest.raises(mans_amd.MansError):
+        ws.check()
+    for i in (0, 2):
+        assert (outs[i].cpu().numpy().view(xs[i].dtype) == xs[i]).all(), i
+    dec.close()
+    enc.close()
