@@ -288,8 +288,19 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
         bw[0] = (n[c] << 16) | cnt[c];
         bw[1] = (uint32_t)y;
       }
+      {
+        // this chain's unit per half-wave (two units in the 32-lane format,
+        // possibly of different planes when planes are concatenated)
+        const uint32_t pa = act[c] ? padded[c] : 0u;
+        const uint32_t k0 = __builtin_amdgcn_readlane((uint32_t)kp[c], 0), a0 = __builtin_amdgcn_readlane(pa, 0);
 #pragma unroll
-      for (int k = 0; k < kMaxPlanes; ++k) ptot[k] += (act[c] && kp[c] == k) ? padded[c] : 0u;
+        for (int k = 0; k < kMaxPlanes; ++k) ptot[k] += k0 == (uint32_t)k ? a0 : 0u;
+        if (L == 32) {
+          const uint32_t k1 = __builtin_amdgcn_readlane((uint32_t)kp[c], 32), a1 = __builtin_amdgcn_readlane(pa, 32);
+#pragma unroll
+          for (int k = 0; k < kMaxPlanes; ++k) ptot[k] += k1 == (uint32_t)k ? a1 : 0u;
+        }
+      }
       uint4* dst = (uint4*)(a.out + a.d.data_off + 2ull * y);
       const uint4* ring4 = (const uint4*)(wstage + soff[c]);
       const uint4* spill4 = RING ? (const uint4*)(spill + c * CSTAGE) : nullptr;
