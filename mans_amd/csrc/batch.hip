@@ -73,24 +73,11 @@ __device__ uint64_t look_back_gen(uint64_t* status, uint64_t idx, uint64_t first
     return 0;
   }
   if (lane == 0) atomic_st64(status + idx, kFlagAgg | G | agg);
-  uint64_t excl = 0;
-  int64_t j = (int64_t)idx - 1;
-  for (;;) {
-    const int64_t i = j - lane;
-    uint64_t v = kFlagPrefix | G;  // before the buffer's first ticket: a zero prefix
-    if (i >= (int64_t)first) v = atomic_ld64(status + i);
-    uint32_t f = (uint32_t)(v >> 62);
-    if (((v >> kGenShift) & kBatchGenMask) != gen) f = 0;
-    const uint64_t pm = __ballot(f == 2), nm = __ballot(f == 0);
-    const uint64_t upto = pm ? (pm & (0 - pm)) * 2u - 1u : ~0ull;
-    if (nm & upto) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    excl += wave_sum(((upto >> lane) & 1u) ? (v & kValue48) : 0u);
-    if (pm) break;
-    j -= 64;
-  }
+  const uint64_t excl = look_back_window<8>(status, (int64_t)idx - 1, (int64_t)first, kFlagPrefix | G, lane,
+                                            [gen](uint64_t v, uint32_t& f, uint64_t& val) {
+                                              f = ((v >> kGenShift) & kBatchGenMask) == gen ? (uint32_t)(v >> 62) : 0u;
+                                              val = v & kValue48;
+                                            });
   if (lane == 0) atomic_st64(status + idx, kFlagPrefix | G | (excl + agg));
   return excl;
 }
@@ -391,8 +378,6 @@ __global__ __launch_bounds__(64 * kNW) void batch_encode_kernel(BatchEncodeArgs 
   if (lane == 0) it = atomicAdd(a.queue, 1u);
   it = wave_u32(it);
   while (it < a.nitems) {
-    uint32_t nxt = 0;
-    if (lane == 0) nxt = atomicAdd(a.queue, 1u);
     const uint32_t b = wave_u32(a.item_buf[it]);
     const BatchBuf& B = a.bufs[b];
     if (it < a.nhist_items) {
@@ -414,6 +399,9 @@ __global__ __launch_bounds__(64 * kNW) void batch_encode_kernel(BatchEncodeArgs 
       for (uint32_t i = lane; i < kTabWords; i += 64) wl[i] = 0;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    // the next item only once this one is done (items start in order)
+    uint32_t nxt = 0;
+    if (lane == 0) nxt = atomicAdd(a.queue, 1u);
     it = wave_u32(nxt);
   }
   // the last wavefront out resets the queue and advances the generation for
@@ -683,8 +671,6 @@ __global__ __launch_bounds__(64 * kNW) void batch_decode_kernel(BatchDecodeArgs 
   if (lane == 0) it = atomicAdd(a.queue, 1u);
   it = wave_u32(it);
   while (it < a.nitems) {
-    uint32_t nxt = 0;
-    if (lane == 0) nxt = atomicAdd(a.queue, 1u);
     const uint32_t b = wave_u32(a.item_buf[it]);
     const BatchBuf& B = a.bufs[b];
     if (it < a.nbufs) {
@@ -700,6 +686,8 @@ __global__ __launch_bounds__(64 * kNW) void batch_decode_kernel(BatchDecodeArgs 
       }
     }
     drain();
+    uint32_t nxt = 0;
+    if (lane == 0) nxt = atomicAdd(a.queue, 1u);
     it = wave_u32(nxt);
   }
   if (lane == 0) {

@@ -60,45 +60,108 @@ __device__ __forceinline__ void enc_step(uint32_t& x, uint32_t& cnt, uint16_t* w
   x = v ? xn : x;
 }
 
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+
+// LDS byte address of a generic pointer into shared memory.
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// One symbol of one chain in the 64-lane format with a linear stage: an
+// emitted word goes to LDS byte address sbase + 2 * rank, where sbase (wave
+// uniform) is where the chain's next word goes and rank the number of
+// emitting lanes below this one; a lane that emits nothing writes its own
+// dummy slot at byte address dummy.  sbase advances by two bytes per emitted
+// word.  Same arithmetic as enc_step.
+__device__ __forceinline__ void enc_step64(uint32_t& x, uint32_t& sbase, uint32_t dummy, const uint4& e,
+                                           bool v = true) {
+  const bool w = v && x >= e.x;
+  const uint64_t m = __ballot(w);
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  // sbase as one scalar and the emit address computed by every lane: one
+  // v_lshl_add and one v_cndmask, no exec-mask branch around them
+  __asm__ volatile("" : "+s"(sbase));
+  uint32_t emit = sbase + 2u * rank;
+  __asm__ volatile("" : "+v"(emit));
+  uint32_t addr = w ? emit : dummy;
+  // an opaque address keeps one unconditional store (no branch)
+  __asm__ volatile("" : "+v"(addr));
+  *(lds_u16*)(uintptr_t)addr = (uint16_t)x;
+  x = w ? x >> 16 : x;
+  sbase += 2u * (uint32_t)__popcll(m);
+  const uint32_t q = (__umulhi(x, e.y) + x) >> (e.z & 31u);
+  const uint32_t xn = __umul24(q, e.w) + x + (e.z >> 8);
+  x = v ? xn : x;
+}
+
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
+// Sum of the aggregates of status entries j, j - 1, ... back to and including
+// the nearest inclusive prefix (entries below `first` read as a zero prefix).
+// Lane l reads entries j - l - 64 k, k < K: 64 K entries per round trip.
+// Groups of 64 are taken in order; a group with an unpublished entry before
+// its nearest prefix ends the round (the groups before it are kept) and the
+// wave sleeps briefly before reading again.  dec(raw, flag, value) splits a
+// status word (flag 0: not yet published, 1: aggregate, 2: inclusive prefix).
+template <int K, typename Dec>
+__device__ __forceinline__ uint64_t look_back_window(const uint64_t* status, int64_t j, int64_t first,
+                                                     uint64_t zero_prefix, int lane, Dec dec) {
+  uint64_t excl = 0;
+  for (;;) {
+    uint64_t raw[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t i = j - lane - 64 * k;
+      raw[k] = i >= first ? __hip_atomic_load(status + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : zero_prefix;
+    }
+    uint64_t part = 0;
+    int groups = 0;
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      uint32_t f;
+      uint64_t val;
+      dec(raw[k], f, val);
+      const uint64_t pm = __ballot(f == 2), nm = __ballot(f == 0);
+      // lanes up to and including the nearest prefix (all 64 if none)
+      const uint64_t upto = pm ? (pm & (0 - pm)) * 2u - 1u : ~0ull;
+      if (nm & upto) break;
+      part += ((upto >> lane) & 1u) ? val : 0u;
+      ++groups;
+      if (pm) {
+        found = true;
+        break;
+      }
+    }
+    excl += wave_sum(part);
+    if (found) return excl;
+    j -= 64 * groups;
+    if (groups < K) __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 // Decoupled look-back for ticket group `chain` with aggregate agg (all
-// lanes): publishes the aggregate, sums predecessors 64 at a time (lane l
-// reads status[j - l]) back to the nearest inclusive prefix, publishes this
-// group's inclusive prefix and returns the exclusive one.  Each status word
-// holds its flag and value together and carries nothing else, so relaxed
-// device-scope atomics suffice (release / acquire at agent scope would write
-// back / invalidate the XCD's whole L2 on every publish and poll).
+// lanes): publishes the aggregate, sums predecessors back to the nearest
+// inclusive prefix (512 per round trip), publishes this group's inclusive
+// prefix and returns the exclusive one.  Each status word holds its flag and
+// value together and carries nothing else, so relaxed device-scope atomics
+// suffice (release / acquire at agent scope would write back / invalidate
+// the XCD's whole L2 on every publish and poll).
 __device__ __forceinline__ uint64_t look_back(uint64_t* status, uint64_t chain, uint64_t agg, int lane) {
   if (chain == 0) {
     if (lane == 0) __hip_atomic_store(status, kFlagPrefix | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return 0;
   }
   if (lane == 0) __hip_atomic_store(status + chain, kFlagAgg | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t excl = 0;
-  int64_t j = (int64_t)chain - 1;
-  for (;;) {
-    const int64_t idx = j - lane;
-    // below group 0 (never needed: group 0 is always a prefix) reads as a
-    // zero prefix
-    uint64_t v = kFlagPrefix;
-    if (idx >= 0) v = __hip_atomic_load(status + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t f = (uint32_t)(v >> 62);
-    const uint64_t pm = __ballot(f == 2), nm = __ballot(f == 0);
-    // lanes up to and including the nearest prefix (all 64 if none)
-    const uint64_t upto = pm ? (pm & (0 - pm)) * 2u - 1u : ~0ull;
-    if (nm & upto) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    excl += wave_sum(((upto >> lane) & 1u) ? (v & kValueMask) : 0u);
-    if (pm) break;
-    j -= 64;
-  }
+  const uint64_t excl = look_back_window<8>(status, (int64_t)chain - 1, 0, kFlagPrefix, lane,
+                                            [](uint64_t v, uint32_t& f, uint64_t& val) {
+                                              f = (uint32_t)(v >> 62);
+                                              val = v & kValueMask;
+                                            });
   if (lane == 0)
     __hip_atomic_store(status + chain, kFlagPrefix | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return excl;
@@ -129,7 +192,11 @@ __device__ __forceinline__ uint32_t lut_next(uint32_t e, uint32_t x) {
 __device__ __forceinline__ uint32_t lut_sym(uint2 e) { return e.x >> 24; }
 __device__ __forceinline__ uint32_t lut_sym(uint32_t e) { return e & 0xffu; }
 
-template <int L, int PB, int CH, uint32_t MASK, typename LE = uint2>
+// CLAMP: ptr stops at 0 (whole-unit stages, where a read past the unit's
+// words must stay inside the stage); without it (rings, every read masked
+// into the ring) ptr goes below 0 on a corrupt stream and stays there, and
+// `taken` is not kept.
+template <int L, int PB, int CH, uint32_t MASK, typename LE = uint2, bool CLAMP = true>
 __device__ __forceinline__ uint32_t dec_steps(uint32_t (&x)[CH], uint32_t (&ptr)[CH], uint32_t (&taken)[CH],
                                               const LE* const (&lut)[CH], const uint16_t* const (&stage)[CH],
                                               const bool (&v)[CH], uint32_t half) {
@@ -154,10 +221,15 @@ __device__ __forceinline__ uint32_t dec_steps(uint32_t (&x)[CH], uint32_t (&ptr)
       cnt = half ? (uint32_t)__popc(mhi) : (uint32_t)__popc(mlo);
       below = half ? (uint32_t)__popc(mlo) : 0u;
     }
-    taken[c] += cnt;
-    // ptr < 2^16 and cnt <= 64: a signed max keeps this scalar (s_max_i32)
-    const int32_t bd = (int32_t)ptr[c] - (int32_t)cnt;
-    const uint32_t base = (uint32_t)(bd > 0 ? bd : 0);
+    uint32_t base;
+    if constexpr (CLAMP) {
+      taken[c] += cnt;
+      // ptr < 2^16 and cnt <= 64: a signed max keeps this scalar (s_max_i32)
+      const int32_t bd = (int32_t)ptr[c] - (int32_t)cnt;
+      base = (uint32_t)(bd > 0 ? bd : 0);
+    } else {
+      base = ptr[c] - cnt;
+    }
     const uint32_t rank = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u)) - below;
     word[c] = (stage[c] + (base & MASK))[rank];
     ptr[c] = base;

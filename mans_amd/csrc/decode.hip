@@ -285,19 +285,20 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
     for (int c = 0; c < CH; ++c) __asm__ volatile("" : "+v"(x[c]));
     __threadfence_block();
 
-    // ring upkeep after a step: start loading the chunk below once the one
-    // above the lowest is used up, and write it to LDS once fewer than 64
-    // words remain above lo (the next step may take up to 64)
+    // ring upkeep, at most 2 steps apart: start loading the chunk below once
+    // the one above the lowest is used up, and write it to LDS once fewer
+    // than 128 words remain above lo (the next two steps may take up to 64
+    // each).  (ptr is signed here: below 0 only on a corrupt stream.)
     auto refill = [&]() {
       if constexpr (RING) {
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-          if (!pend[c] && lo[c] > 0 && ptr[c] <= lo[c] + H) {
+          if (!pend[c] && lo[c] > 0 && (int32_t)ptr[c] <= (int32_t)(lo[c] + H)) {
             const uint32_t j = lo[c] / H - 1;
             if ((uint32_t)lane < H4) rv[c] = src4[c][j * H4 + lane];
             pend[c] = true;
           }
-          if (pend[c] && ptr[c] < lo[c] + 64u) {
+          if (pend[c] && (int32_t)ptr[c] < (int32_t)(lo[c] + 128u)) {
             const uint32_t j = lo[c] / H - 1;
             uint16_t* reg = wstage + c * REG;
             if ((uint32_t)lane < H4) ((uint4*)(reg + (j & 1) * H))[lane] = rv[c];
@@ -338,22 +339,32 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
         v[c] = i < n[c];
         any |= v[c];
       }
-      const uint32_t el = dec_steps<L, PB, CH, MASK>(x, ptr, taken, lut, stage, v, half);
+      const uint32_t el = dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, v, half);
       if (any) ep[(uint64_t)s * L] = (T)el;
       refill();
     }
     bool vall[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) vall[c] = true;
-    for (uint32_t s = full; s > 0;) {
+    // two steps between ring checks (full is a multiple of two for whole
+    // blocks; an odd remainder takes a single step first)
+    uint32_t s = full;
+    if (s & 1u) {
       --s;
-      ep[(uint64_t)s * L] = (T)dec_steps<L, PB, CH, MASK>(x, ptr, taken, lut, stage, vall, half);
+      ep[(uint64_t)s * L] = (T)dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half);
+      refill();
+    }
+    while (s > 0) {
+      s -= 2;
+      ep[(uint64_t)(s + 1) * L] = (T)dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half);
+      if constexpr (!RING) refill();
+      ep[(uint64_t)s * L] = (T)dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half);
       refill();
     }
     // a valid block uses every word and returns each lane to 2^15
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      const bool bad = act[c] && (taken[c] != wds[c] || x[c] != kMinState);
+      const bool bad = act[c] && ((RING ? ptr[c] != 0u : taken[c] != wds[c]) || x[c] != kMinState);
       if (__ballot(bad) && lin == 0 && act[c]) atomicOr(a.err, kErrFinalState);
     }
     __threadfence_block();

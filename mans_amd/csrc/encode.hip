@@ -8,21 +8,22 @@
 // final container.  Here one persistent launch does all of it:
 //  * a wavefront takes a ticket (atomic counter) for a group of units: the
 //    CH planes of one element block (two blocks, one per half-wave, in the
-//    32-lane reference format), and takes its next ticket while it codes this
-//    one.  Each lane runs CH independent rANS chains, one per byte plane, fed
+//    32-lane reference format), taking the next one when it is done.  Each
+//    lane runs CH independent rANS chains, one per byte plane, fed
 //    by a single element load per step, so the serial per-chain dependency is
 //    interleaved CH ways;
 //  * element loads are software-pipelined: the next 16 steps are in flight
 //    while the current 16 are coded, and the table entries of a step are read
 //    one step ahead;
 //  * emitted 16-bit words are compacted in lane order within each step
-//    (64-bit ballot + mbcnt) into LDS.  In the 64-lane format a chain keeps a
-//    ring of R words: every R/2 words go to the wave's own spill area (cache
-//    resident: each wave reuses it for every unit), so a wave needs a few KiB
+//    (64-bit ballot + mbcnt) into LDS.  In the 64-lane format a chain keeps
+//    a linear stage of H + 256 words: at most every 4 steps, once H words are
+//    waiting, they go to the wave's own spill area (each wave reuses it for
+//    every unit) and the rest moves to the front, so a wave needs a few KiB
 //    of LDS and many waves stay resident.  The 32-lane format stages units
 //    whole (bounded by lane_max_words);
 //  * the wave publishes its padded word count and resolves its output offset
-//    by decoupled look-back over earlier tickets, 64 of them per round trip
+//    by decoupled look-back over earlier tickets, 512 of them per round trip
 //    (tickets are taken by running waves, so every wave waited on is resident
 //    and finishes);
 //  * it then writes states, block words and the unit's words straight to
@@ -48,24 +49,22 @@ struct LanePlane {  // per-plane fields copied to LDS (dynamic index by lane)
 };
 
 // L lanes per unit, PB probability bits, S element bytes, CH chains per lane,
-// NW wavefronts per group, RB log2 of the LDS ring in words (64-lane format;
-// 0 = whole-unit staging).  CH == S: the CH chains of a lane code the CH byte
+// NW wavefronts per group, HB log2 of the spill chunk in words (64-lane
+// format; 0 = whole-unit staging).  CH == S: the CH chains of a lane code the CH byte
 // planes of the same elements (unit order 0 with P == S).  CH == 1, S == 1:
 // one chain per lane on any unit (single streams, concatenated ADM planes).
-template <int L, int PB, int S, int CH, int NW, int RB>
+template <int L, int PB, int S, int CH, int NW, int HB>
 __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
   constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
   constexpr uint32_t HALVES = L == 32 ? 2u : 1u;
   constexpr uint32_t UPW = CH * HALVES;  // units per ticket
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
-  constexpr bool RING = RB > 0;
-  constexpr uint32_t R = RING ? 1u << RB : 0u;   // ring words
-  constexpr uint32_t H = RING ? R / 2 : 8u;      // spill chunk words
-  constexpr uint32_t H4 = H / 8;                 // spill chunk uint4s
-  constexpr uint32_t MASK = RING ? R - 1u : 0xffffffffu;
-  constexpr uint32_t REG = RING ? R : CSTAGE;    // LDS words per chain
+  constexpr bool LINEAR = HB > 0;                  // 64-lane: linear stage + spills
+  constexpr uint32_t H = LINEAR ? 1u << HB : 8u;  // spill chunk words
+  constexpr uint32_t H4 = H / 8;                   // spill chunk uint4s
+  constexpr uint32_t REG = LINEAR ? H + 256u : CSTAGE;  // LDS words per chain
   static_assert(CH == S, "chain c codes byte c of each element");
-  static_assert(!RING || (L == 64 && H >= 128 && H4 <= 64), "ring: 64 lanes, 128 <= R/2 <= 512 words");
+  static_assert(!LINEAR || (L == 64 && H >= 256 && H4 <= 64), "linear stage: 64 lanes, 256 <= R/2 <= 512 words");
   extern __shared__ uint4 smem[];
   __shared__ LanePlane s_pl[kMaxPlanes];
   __shared__ uint64_t s_ubase[kMaxPlanes + 1];
@@ -89,13 +88,15 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
 
   const uint32_t half = L == 32 ? (uint32_t)(lane >> 5) : 0u;
   const uint32_t lin = (uint32_t)lane & (L - 1);
-  // staging: per wave, CH regions of REG words (a ring, or [half] whole
-  // units), then one dword-spaced dummy slot per lane
+  // staging: per wave, CH regions of REG words (64 lanes: the unit's words
+  // not yet spilled, from position 0; 32 lanes: [half] whole units), then one
+  // dword-spaced dummy slot per lane
   const uint32_t wave_u = __builtin_amdgcn_readfirstlane((uint32_t)wave);
   uint16_t* wstage = (uint16_t*)(smem + P * kNumSymbols) + wave_u * (CH * REG + 128);
   const uint32_t doff = CH * REG + 2u * (uint32_t)lane;  // dummy slot, from wstage
+  const uint32_t dummy_addr = lds_addr(wstage + doff);
   // this wave's spill area: CH regions of CSTAGE words
-  uint16_t* const spill = RING ? a.spill + ((uint64_t)blockIdx.x * NW + wave_u) * (CH * CSTAGE) : nullptr;
+  uint16_t* const spill = LINEAR ? a.spill + ((uint64_t)blockIdx.x * NW + wave_u) * (CH * CSTAGE) : nullptr;
   // words coded per plane by this wave (added to the section headers once,
   // at the end)
   uint32_t ptot[kMaxPlanes];
@@ -106,12 +107,10 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
   if (lane == 0) tk = atomicAdd(a.ticket, UPW);
   tk = __builtin_amdgcn_readfirstlane(tk);
   while ((uint64_t)tk < a.d.total_units) {
-    // the next ticket, taken now so its latency hides behind this one
-    uint32_t nxt = 0;
-    if (lane == 0) nxt = atomicAdd(a.ticket, UPW);
 
     uint32_t x[CH], cnt[CH], n[CH], fl[CH];
     uint32_t soff[CH];  // chain region, from wstage
+    uint32_t cb[CH], sb[CH];  // 64 lanes: LDS byte address of the region and of the next word
     const uint4* tab[CH];
     uint64_t blk[CH];
     int kp[CH];
@@ -141,7 +140,9 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
       x[c] = kMinState;
       cnt[c] = 0;
       fl[c] = 0;
-      soff[c] = c * REG + (RING ? 0u : half * (CSTAGE / HALVES));
+      soff[c] = c * REG + (LINEAR ? 0u : half * (CSTAGE / HALVES));
+      cb[c] = lds_addr(wstage + soff[c]);
+      sb[c] = cb[c];
       tab[c] = s_tab + k * kNumSymbols;
     }
     // wave-uniform step counts over all chains and halves
@@ -162,23 +163,28 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
     }
     const uint32_t all = (uint32_t)div_up(n_max, L);
 
-    // ring upkeep after a step: a chain with R/2 unspilled words moves the
-    // oldest R/2 (an aligned half of the ring) to its spill area; after the
-    // next step it has at most R/2 + 64 unspilled words, so no unspilled word
-    // is overwritten
+    // stage upkeep (64 lanes), at most 4 steps apart: a chain with H or more
+    // unspilled words moves its first H to its spill area and the rest
+    // (fewer than 256: at most 64 words per step) to the front.  The stage
+    // holds H + 256 words, the most a chain can have between two checks.
     auto spill_full = [&]() {
-      if constexpr (RING) {
+      if constexpr (LINEAR) {
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-          if (cnt[c] - fl[c] >= H) {
-            if ((uint32_t)lane < H4) {
-              const uint4 q = ((const uint4*)(wstage + soff[c] + (fl[c] & MASK)))[lane];
-              ((uint4*)(spill + c * CSTAGE + fl[c]))[lane] = q;
-            }
+          const uint32_t un = (sb[c] - cb[c]) >> 1;  // unspilled words
+          if (un >= H) {
+            uint16_t* st = wstage + soff[c];
+            if ((uint32_t)lane < H4) ((uint4*)(spill + c * CSTAGE + fl[c]))[lane] = ((const uint4*)st)[lane];
+            if (4u * (uint32_t)lane < un - H) ((uint2*)st)[lane] = ((const uint2*)(st + H))[lane];
             fl[c] += H;
+            sb[c] -= 2u * H;
           }
         }
       }
+    };
+    auto step = [&](int c, const uint4& e, bool v) {
+      if constexpr (LINEAR) enc_step64(x[c], sb[c], dummy_addr, e);
+      else enc_step<L, 0xffffffffu>(x[c], cnt[c], wstage, soff[c], doff, e, v, half);
     };
 
     {
@@ -213,8 +219,8 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
             for (int c = 0; c < CH; ++c) en[c] = tab[c][__builtin_amdgcn_ubfe(buf[j + 1], sh + 8u * c, 8u)];
           }
 #pragma unroll
-          for (int c = 0; c < CH; ++c) enc_step<L, MASK>(x[c], cnt[c], wstage, soff[c], doff, e[c], true, half);
-          spill_full();
+          for (int c = 0; c < CH; ++c) step(c, e[c], true);
+          if ((j & 3) == 3) spill_full();
           if (j + 1 < kBatch) {
 #pragma unroll
             for (int c = 0; c < CH; ++c) e[c] = en[c];
@@ -248,7 +254,8 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
           const bool v = i < n[c];
           const uint32_t sym = v ? (uint32_t)eb[(uint64_t)i * S + c] : 0u;
           const uint4 e = tab[c][sym];
-          enc_step<L, MASK>(x[c], cnt[c], wstage, soff[c], doff, e, v, half);
+          if constexpr (LINEAR) enc_step64(x[c], sb[c], dummy_addr, e, v);
+          else step(c, e, v);
         }
         spill_full();
       }
@@ -260,9 +267,10 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
     uint32_t a_lo[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
+      if constexpr (LINEAR) cnt[c] = fl[c] + ((sb[c] - cb[c]) >> 1);
       padded[c] = (uint32_t)round_up(cnt[c], 8);
       if (cnt[c] > CSTAGE / HALVES && lin == 0) atomicOr(a.err, kErrStageOverflow);
-      if (lin < padded[c] - cnt[c]) wstage[soff[c] + ((cnt[c] + lin) & MASK)] = 0;
+      if (lin < padded[c] - cnt[c]) wstage[soff[c] + (cnt[c] - fl[c]) + lin] = 0;
       if (act[c]) ((uint32_t*)(a.out + s_pl[kp[c]].section + kSectionFixed))[blk[c] * L + lin] = x[c];
       a_lo[c] = __builtin_amdgcn_readlane(padded[c], 0);
       agg += a_lo[c];
@@ -302,12 +310,12 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
         }
       }
       uint4* dst = (uint4*)(a.out + a.d.data_off + 2ull * y);
-      const uint4* ring4 = (const uint4*)(wstage + soff[c]);
-      const uint4* spill4 = RING ? (const uint4*)(spill + c * CSTAGE) : nullptr;
+      const uint4* st4 = (const uint4*)(wstage + soff[c]);
+      const uint4* spill4 = LINEAR ? (const uint4*)(spill + c * CSTAGE) : nullptr;
       for (uint32_t j = lin; j < padded[c] / 8; j += L) {
         uint4 q;
-        if (RING && 8 * j < fl[c]) q = spill4[j];
-        else q = ring4[((8 * j) & MASK) / 8];
+        if (LINEAR && 8 * j < fl[c]) q = spill4[j];
+        else q = st4[(8 * j - fl[c]) / 8];
         dst[j] = q;
       }
       y += padded[c];
@@ -324,7 +332,10 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
       }
     }
     __threadfence_block();
-    tk = __builtin_amdgcn_readfirstlane(nxt);
+    // the next ticket only once this one is done: tickets start in ticket
+    // order, so the tickets a look-back waits on are already being coded
+    if (lane == 0) tk = atomicAdd(a.ticket, UPW);
+    tk = __builtin_amdgcn_readfirstlane(tk);
   }
   // total coded words of each plane (the section headers start at 0)
   if (lane == 0) {
@@ -334,14 +345,14 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
   }
 }
 
-template <int L, int PB, int S, int CH, int NW, int RB>
+template <int L, int PB, int S, int CH, int NW, int HB>
 hipError_t launch_t(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
-  constexpr uint32_t REG = RB > 0 ? 1u << RB : CSTAGE;
+  constexpr uint32_t REG = HB > 0 ? (1u << HB) + 256u : CSTAGE;
   const size_t lds = (size_t)a.d.num_planes * kNumSymbols * sizeof(uint4) + (size_t)NW * (CH * REG + 128) * 2u;
   int per_cu = 0;
   hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S, CH, NW, RB>, 64 * NW, lds);
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S, CH, NW, HB>, 64 * NW, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   constexpr uint32_t UPW = CH * (L == 32 ? 2u : 1u);
@@ -350,13 +361,13 @@ hipError_t launch_t(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_s
   const uint64_t cap = (uint64_t)cu * (uint64_t)per_cu;
   if (grid > cap) grid = cap;
   if (grid < 1) grid = 1;
-  const uint64_t spill_bytes = RB > 0 ? grid * NW * CH * CSTAGE * 2ull : 0ull;
+  const uint64_t spill_bytes = HB > 0 ? grid * NW * CH * CSTAGE * 2ull : 0ull;
   if (need_spill) {
     *need_spill = spill_bytes;
     return hipSuccess;
   }
   if (spill_bytes > a.spill_bytes) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((encode_kernel<L, PB, S, CH, NW, RB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+  hipLaunchKernelGGL((encode_kernel<L, PB, S, CH, NW, HB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
@@ -365,10 +376,11 @@ hipError_t launch_s(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_s
   const int P = a.d.num_planes;
   const uint32_t S = a.d.plane[0].stride;
   if constexpr (L == 64) {
-    // rings of 1024 words (512 with four chains): 4-5 KiB of LDS per wave
-    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 4, 10>(a, cu, s, need_spill);
-    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 4, 9>(a, cu, s, need_spill);
-    if (S == 1) return launch_t<L, PB, 1, 1, 4, 10>(a, cu, s, need_spill);
+    // linear stages of 512 + 256 words (256 + 256 with four chains): 3-4 KiB
+    // of LDS per wave
+    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 4, 9>(a, cu, s, need_spill);
+    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 4, 8>(a, cu, s, need_spill);
+    if (S == 1) return launch_t<L, PB, 1, 1, 4, 9>(a, cu, s, need_spill);
   } else {
     if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 2, 0>(a, cu, s, need_spill);
     if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 1, 0>(a, cu, s, need_spill);
