@@ -24,7 +24,10 @@ ARCH = os.environ.get("MANS_OFFLOAD_ARCH", "gfx950")
 # -packed-fp32-ops: hipcc must not form v_pk_*_f32 (none of the codec's
 # arithmetic is floating point except the one fp32 division in the
 # normalisation, which must stay a plain correctly rounded v_div sequence).
-DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
+# -fno-strict-aliasing: the coders write 16-bit words into LDS stages and
+# spill areas and move them as uint2 / uint4; those accesses must stay in
+# program order.
+DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-fno-strict-aliasing",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 

@@ -67,15 +67,17 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
 
-// One symbol of one chain in the 64-lane format with a linear stage: an
-// emitted word goes to LDS byte address sbase + 2 * rank, where sbase (wave
-// uniform) is where the chain's next word goes and rank the number of
-// emitting lanes below this one; a lane that emits nothing writes its own
-// dummy slot at byte address dummy.  sbase advances by two bytes per emitted
-// word.  Same arithmetic as enc_step.
+// One symbol of one chain in the 64-lane format with a linear stage, table
+// entry e = {pdf << (31 - p) | cdf, magic, shift, 2^p - pdf} (p <= 15, so
+// the threshold pdf << (31 - p) has no bits below bit 16 and cdf fits under
+// it).  An emitted word goes to LDS byte address sbase + 2 * rank, where
+// sbase (wave uniform) is where the chain's next word goes and rank the
+// number of emitting lanes below this one; a lane that emits nothing writes
+// its own dummy slot at byte address dummy.  sbase advances by two bytes per
+// emitted word.  Lanes with v false keep their state and emit nothing.
 __device__ __forceinline__ void enc_step64(uint32_t& x, uint32_t& sbase, uint32_t dummy, const uint4& e,
                                            bool v = true) {
-  const bool w = v && x >= e.x;
+  const bool w = v && (x >> 16) >= (e.x >> 16);
   const uint64_t m = __ballot(w);
   const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
   // sbase as one scalar and the emit address computed by every lane: one
@@ -90,8 +92,55 @@ __device__ __forceinline__ void enc_step64(uint32_t& x, uint32_t& sbase, uint32_
   x = w ? x >> 16 : x;
   sbase += 2u * (uint32_t)__popcll(m);
   const uint32_t q = (__umulhi(x, e.y) + x) >> (e.z & 31u);
-  const uint32_t xn = __umul24(q, e.w) + x + (e.z >> 8);
+  const uint32_t xn = __umul24(q, e.w) + x + (e.x & 0xffffu);
   x = v ? xn : x;
+}
+
+// enc_step64 for a step where every lane codes a symbol, written with
+// sub-dword operand selects: the threshold compare reads the high halves of
+// x and e.x, the renormalising shift is a select of x's high half, and cdf
+// is added from e.x's low half (three VALU operations fewer per symbol).
+__device__ __forceinline__ void enc_step64f(uint32_t& x, uint32_t& sbase, uint32_t dummy, const uint4& e) {
+  uint64_t m;
+  uint32_t xr;
+  __asm__(
+      "v_cmp_ge_u32_sdwa vcc, %2, %3 src0_sel:WORD_1 src1_sel:WORD_1\n\t"
+      "v_cndmask_b32_sdwa %1, %2, %2, vcc dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1\n\t"
+      "s_mov_b64 %0, vcc"
+      : "=s"(m), "=&v"(xr)
+      : "v"(x), "v"(e.x)
+      : "vcc");
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  __asm__ volatile("" : "+s"(sbase));
+  const uint32_t emit = sbase + 2u * rank;
+  uint32_t addr;
+  __asm__("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(addr) : "v"(dummy), "v"(emit), "s"(m));
+  *(lds_u16*)(uintptr_t)addr = (uint16_t)x;
+  sbase += 2u * (uint32_t)__popcll(m);
+  const uint32_t q = (__umulhi(xr, e.y) + xr) >> (e.z & 31u);
+  const uint32_t xn = __umul24(q, e.w) + xr;
+  __asm__("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+          : "=v"(x)
+          : "v"(xn), "v"(e.x));
+}
+
+// 16 * byte B of w: a byte offset into a table of 16-byte entries.
+template <int B>
+__device__ __forceinline__ uint32_t byte_x16(uint32_t w) {
+  uint32_t r;
+  if constexpr (B == 0)
+    __asm__("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+            : "=v"(r) : "v"(w));
+  else if constexpr (B == 1)
+    __asm__("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1"
+            : "=v"(r) : "v"(w));
+  else if constexpr (B == 2)
+    __asm__("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2"
+            : "=v"(r) : "v"(w));
+  else
+    __asm__("v_lshlrev_b32_sdwa %0, 4, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3"
+            : "=v"(r) : "v"(w));
+  return r;
 }
 
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
@@ -144,6 +193,29 @@ __device__ __forceinline__ uint64_t look_back_window(const uint64_t* status, int
   }
 }
 
+// First half of a look-back: publish entry `chain`'s own count (the first
+// entry's is already its inclusive prefix).
+__device__ __forceinline__ void publish_aggregate(uint64_t* status, uint64_t chain, uint64_t agg, int lane) {
+  if (lane == 0)
+    __hip_atomic_store(status + chain, (chain == 0 ? kFlagPrefix : kFlagAgg) | agg, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+// Second half: sum the predecessors back to the nearest published prefix,
+// publish this entry's inclusive prefix and return the exclusive one.  Any
+// time may pass between the two halves; the predecessors only need to have
+// published their aggregates.  K groups of 64 entries per round trip.
+template <int K = 8>
+__device__ __forceinline__ uint64_t resolve_prefix(uint64_t* status, uint64_t chain, uint64_t agg, int lane) {
+  if (chain == 0) return 0;
+  const uint64_t excl = look_back_window<K>(status, (int64_t)chain - 1, 0, kFlagPrefix, lane,
+                                            [](uint64_t v, uint32_t& f, uint64_t& val) {
+                                              f = (uint32_t)(v >> 62);
+                                              val = v & kValueMask;
+                                            });
+  if (lane == 0)
+    __hip_atomic_store(status + chain, kFlagPrefix | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
 // Decoupled look-back for ticket group `chain` with aggregate agg (all
 // lanes): publishes the aggregate, sums predecessors back to the nearest
 // inclusive prefix (512 per round trip), publishes this group's inclusive
@@ -152,19 +224,8 @@ __device__ __forceinline__ uint64_t look_back_window(const uint64_t* status, int
 // suffice (release / acquire at agent scope would write back / invalidate
 // the XCD's whole L2 on every publish and poll).
 __device__ __forceinline__ uint64_t look_back(uint64_t* status, uint64_t chain, uint64_t agg, int lane) {
-  if (chain == 0) {
-    if (lane == 0) __hip_atomic_store(status, kFlagPrefix | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return 0;
-  }
-  if (lane == 0) __hip_atomic_store(status + chain, kFlagAgg | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint64_t excl = look_back_window<8>(status, (int64_t)chain - 1, 0, kFlagPrefix, lane,
-                                            [](uint64_t v, uint32_t& f, uint64_t& val) {
-                                              f = (uint32_t)(v >> 62);
-                                              val = v & kValueMask;
-                                            });
-  if (lane == 0)
-    __hip_atomic_store(status + chain, kFlagPrefix | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return excl;
+  publish_aggregate(status, chain, agg, lane);
+  return resolve_prefix(status, chain, agg, lane);
 }
 
 // One symbol of each of the CH chains; returns the CH decoded bytes packed

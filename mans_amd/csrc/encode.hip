@@ -18,14 +18,15 @@
 //  * emitted 16-bit words are compacted in lane order within each step
 //    (64-bit ballot + mbcnt) into LDS.  In the 64-lane format a chain keeps
 //    a linear stage of H + 256 words: at most every 4 steps, once H words are
-//    waiting, they go to the wave's own spill area (each wave reuses it for
-//    every unit) and the rest moves to the front, so a wave needs a few KiB
-//    of LDS and many waves stay resident.  The 32-lane format stages units
-//    whole (bounded by lane_max_words);
+//    waiting, they go to one of the wave's two spill slots in HBM and the
+//    rest moves to the front, so a wave needs a few KiB of LDS and many waves
+//    stay resident.  The 32-lane format stages units whole (bounded by
+//    lane_max_words);
 //  * the wave publishes its padded word count and resolves its output offset
 //    by decoupled look-back over earlier tickets, 512 of them per round trip
 //    (tickets are taken by running waves, so every wave waited on is resident
-//    and finishes);
+//    and finishes).  In the 64-lane format the look-back of a ticket runs
+//    only after the wave has coded its next ticket, so it rarely waits;
 //  * it then writes states, block words and the unit's words straight to
 //    their final place, 16 bytes per lane per store.
 // The arithmetic per symbol is the reference's: emit x & 0xffff when
@@ -40,6 +41,8 @@ constexpr int kBatch = 16;  // steps per prefetch batch
 
 template <typename T>
 using gptr = const __attribute__((address_space(1))) T*;
+// 16 bytes as a native vector: arrays of it stay in registers
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 struct LanePlane {  // per-plane fields copied to LDS (dynamic index by lane)
   const uint8_t* src;
@@ -48,23 +51,19 @@ struct LanePlane {  // per-plane fields copied to LDS (dynamic index by lane)
   uint64_t nb;
 };
 
-// L lanes per unit, PB probability bits, S element bytes, CH chains per lane,
-// NW wavefronts per group, HB log2 of the spill chunk in words (64-lane
-// format; 0 = whole-unit staging).  CH == S: the CH chains of a lane code the CH byte
+// The 32-lane reference format: L lanes per unit, PB probability bits, S
+// element bytes, CH chains per lane, NW wavefronts per group; units are
+// staged whole in LDS.  CH == S: the CH chains of a lane code the CH byte
 // planes of the same elements (unit order 0 with P == S).  CH == 1, S == 1:
 // one chain per lane on any unit (single streams, concatenated ADM planes).
-template <int L, int PB, int S, int CH, int NW, int HB>
+template <int L, int PB, int S, int CH, int NW>
 __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
   constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
   constexpr uint32_t HALVES = L == 32 ? 2u : 1u;
   constexpr uint32_t UPW = CH * HALVES;  // units per ticket
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
-  constexpr bool LINEAR = HB > 0;                  // 64-lane: linear stage + spills
-  constexpr uint32_t H = LINEAR ? 1u << HB : 8u;  // spill chunk words
-  constexpr uint32_t H4 = H / 8;                   // spill chunk uint4s
-  constexpr uint32_t REG = LINEAR ? H + 256u : CSTAGE;  // LDS words per chain
+  constexpr uint32_t REG = CSTAGE;  // LDS words per chain
   static_assert(CH == S, "chain c codes byte c of each element");
-  static_assert(!LINEAR || (L == 64 && H >= 256 && H4 <= 64), "linear stage: 64 lanes, 256 <= R/2 <= 512 words");
   extern __shared__ uint4 smem[];
   __shared__ LanePlane s_pl[kMaxPlanes];
   __shared__ uint64_t s_ubase[kMaxPlanes + 1];
@@ -88,15 +87,11 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
 
   const uint32_t half = L == 32 ? (uint32_t)(lane >> 5) : 0u;
   const uint32_t lin = (uint32_t)lane & (L - 1);
-  // staging: per wave, CH regions of REG words (64 lanes: the unit's words
-  // not yet spilled, from position 0; 32 lanes: [half] whole units), then one
-  // dword-spaced dummy slot per lane
+  // staging: per wave, CH regions of REG words ([half] whole units), then
+  // one dword-spaced dummy slot per lane
   const uint32_t wave_u = __builtin_amdgcn_readfirstlane((uint32_t)wave);
   uint16_t* wstage = (uint16_t*)(smem + P * kNumSymbols) + wave_u * (CH * REG + 128);
   const uint32_t doff = CH * REG + 2u * (uint32_t)lane;  // dummy slot, from wstage
-  const uint32_t dummy_addr = lds_addr(wstage + doff);
-  // this wave's spill area: CH regions of CSTAGE words
-  uint16_t* const spill = LINEAR ? a.spill + ((uint64_t)blockIdx.x * NW + wave_u) * (CH * CSTAGE) : nullptr;
   // words coded per plane by this wave (added to the section headers once,
   // at the end)
   uint32_t ptot[kMaxPlanes];
@@ -108,9 +103,8 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
   tk = __builtin_amdgcn_readfirstlane(tk);
   while ((uint64_t)tk < a.d.total_units) {
 
-    uint32_t x[CH], cnt[CH], n[CH], fl[CH];
+    uint32_t x[CH], cnt[CH], n[CH];
     uint32_t soff[CH];  // chain region, from wstage
-    uint32_t cb[CH], sb[CH];  // 64 lanes: LDS byte address of the region and of the next word
     const uint4* tab[CH];
     uint64_t blk[CH];
     int kp[CH];
@@ -139,10 +133,7 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
       src[c] = pl.src;
       x[c] = kMinState;
       cnt[c] = 0;
-      fl[c] = 0;
-      soff[c] = c * REG + (LINEAR ? 0u : half * (CSTAGE / HALVES));
-      cb[c] = lds_addr(wstage + soff[c]);
-      sb[c] = cb[c];
+      soff[c] = c * REG + half * (CSTAGE / HALVES);
       tab[c] = s_tab + k * kNumSymbols;
     }
     // wave-uniform step counts over all chains and halves
@@ -163,28 +154,8 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
     }
     const uint32_t all = (uint32_t)div_up(n_max, L);
 
-    // stage upkeep (64 lanes), at most 4 steps apart: a chain with H or more
-    // unspilled words moves its first H to its spill area and the rest
-    // (fewer than 256: at most 64 words per step) to the front.  The stage
-    // holds H + 256 words, the most a chain can have between two checks.
-    auto spill_full = [&]() {
-      if constexpr (LINEAR) {
-#pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          const uint32_t un = (sb[c] - cb[c]) >> 1;  // unspilled words
-          if (un >= H) {
-            uint16_t* st = wstage + soff[c];
-            if ((uint32_t)lane < H4) ((uint4*)(spill + c * CSTAGE + fl[c]))[lane] = ((const uint4*)st)[lane];
-            if (4u * (uint32_t)lane < un - H) ((uint2*)st)[lane] = ((const uint2*)(st + H))[lane];
-            fl[c] += H;
-            sb[c] -= 2u * H;
-          }
-        }
-      }
-    };
     auto step = [&](int c, const uint4& e, bool v) {
-      if constexpr (LINEAR) enc_step64(x[c], sb[c], dummy_addr, e);
-      else enc_step<L, 0xffffffffu>(x[c], cnt[c], wstage, soff[c], doff, e, v, half);
+      enc_step<L, 0xffffffffu>(x[c], cnt[c], wstage, soff[c], doff, e, v, half);
     };
 
     {
@@ -220,7 +191,6 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
           }
 #pragma unroll
           for (int c = 0; c < CH; ++c) step(c, e[c], true);
-          if ((j & 3) == 3) spill_full();
           if (j + 1 < kBatch) {
 #pragma unroll
             for (int c = 0; c < CH; ++c) e[c] = en[c];
@@ -253,11 +223,8 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
         for (int c = 0; c < CH; ++c) {
           const bool v = i < n[c];
           const uint32_t sym = v ? (uint32_t)eb[(uint64_t)i * S + c] : 0u;
-          const uint4 e = tab[c][sym];
-          if constexpr (LINEAR) enc_step64(x[c], sb[c], dummy_addr, e, v);
-          else step(c, e, v);
+          step(c, tab[c][sym], v);
         }
-        spill_full();
       }
     }
 
@@ -267,10 +234,9 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
     uint32_t a_lo[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      if constexpr (LINEAR) cnt[c] = fl[c] + ((sb[c] - cb[c]) >> 1);
       padded[c] = (uint32_t)round_up(cnt[c], 8);
       if (cnt[c] > CSTAGE / HALVES && lin == 0) atomicOr(a.err, kErrStageOverflow);
-      if (lin < padded[c] - cnt[c]) wstage[soff[c] + (cnt[c] - fl[c]) + lin] = 0;
+      if (lin < padded[c] - cnt[c]) wstage[soff[c] + cnt[c] + lin] = 0;
       if (act[c]) ((uint32_t*)(a.out + s_pl[kp[c]].section + kSectionFixed))[blk[c] * L + lin] = x[c];
       a_lo[c] = __builtin_amdgcn_readlane(padded[c], 0);
       agg += a_lo[c];
@@ -285,9 +251,6 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
 #pragma unroll
       for (int c = 0; c < CH; ++c) y += a_lo[c];
     }
-    // spilled words are read back by the lanes that wrote them: make those
-    // stores complete first
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       if (act[c] && lin == 0) {
@@ -311,13 +274,7 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
       }
       uint4* dst = (uint4*)(a.out + a.d.data_off + 2ull * y);
       const uint4* st4 = (const uint4*)(wstage + soff[c]);
-      const uint4* spill4 = LINEAR ? (const uint4*)(spill + c * CSTAGE) : nullptr;
-      for (uint32_t j = lin; j < padded[c] / 8; j += L) {
-        uint4 q;
-        if (LINEAR && 8 * j < fl[c]) q = spill4[j];
-        else q = st4[(8 * j - fl[c]) / 8];
-        dst[j] = q;
-      }
+      for (uint32_t j = lin; j < padded[c] / 8; j += L) dst[j] = st4[j];
       y += padded[c];
     }
 
@@ -345,14 +302,353 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
   }
 }
 
+// The 64-lane format: a ticket is BPT consecutive element blocks (CH = S *
+// BPT consecutive units: the S planes of each block), one chain per unit per
+// lane, so a lane runs several interleaved chains whatever the element
+// width.  A chain's words go to a linear LDS stage of
+// H + 256 words; once H are waiting (checked at most 4 steps apart) they go
+// to the wave's spill slot in HBM and the rest moves to the front.  When the
+// ticket is coded, the stage's remainder follows them, so the slot holds all
+// of the ticket's words, padded, and the wave publishes its aggregate.  The
+// offset is resolved only after the NEXT ticket has been coded (two slots per
+// wave, used in turn): by then the predecessors have had a whole ticket's
+// time to publish their own aggregates, so the look-back sums published
+// entries instead of waiting for the slowest wave still coding.  The copy to
+// the final place issues a batch of loads before any store.
+template <int PB, int S, int BPT, int NW, int HB>
+__global__ __launch_bounds__(64 * NW) void encode64_kernel(EncodeArgs a) {
+  constexpr int CH = S * BPT;       // chains per lane
+  constexpr int KB = kBatch / BPT;  // steps per prefetch batch
+  constexpr uint32_t L = 64, BLOCK = 8192u;
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  constexpr uint32_t H = 1u << HB;  // spill chunk words
+  constexpr uint32_t H4 = H / 8;    // spill chunk uint4s
+  constexpr uint32_t REG = H + 256u;  // LDS stage words per chain
+  constexpr uint32_t SLOT = CH * CSTAGE;  // spill slot words
+  constexpr int CB = 8;  // copy batch: uint4 loads per lane in flight
+  static_assert(BPT == 1 || (S < 4 && BLOCK / L % KB == 0), "blocks per ticket");
+  static_assert(H >= 256 && H4 <= 64, "256 <= spill chunk <= 512 words");
+  static_assert(CSTAGE % 8 == 0 && REG % 8 == 0, "uint4-aligned areas");
+  extern __shared__ uint4 smem[];
+  __shared__ LanePlane s_pl[kMaxPlanes];
+  __shared__ uint64_t s_ubase[kMaxPlanes + 1];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int P = a.d.num_planes;
+
+  // [P][256] {pdf << (31 - PB) | cdf, magic, shift, 2^PB - pdf}, at a fixed
+  // LDS address so each chain's base is an immediate offset of its reads
+  __shared__ uint4 s_tab[(S > 1 ? S : kMaxPlanes) * kNumSymbols];
+  for (int i = t; i < P * (int)kNumSymbols; i += 64 * NW) {
+    uint4 e = a.table[i];
+    s_tab[i] = make_uint4((e.x << (31 - PB)) | e.y, e.z, e.w, (1u << PB) - e.x);
+  }
+  if (t < P) {
+    s_pl[t].src = a.d.plane[t].src;
+    s_pl[t].nsym = a.d.plane[t].nsym;
+    s_pl[t].section = a.d.plane[t].section;
+    s_pl[t].nb = a.d.plane[t].nb;
+    s_ubase[t] = a.d.plane[t].unit_base;
+  }
+  if (t == 0) s_ubase[P] = a.d.total_units;
+  __syncthreads();
+
+  // staging: per wave, CH stages of REG words, then one dword-spaced dummy
+  // slot per lane
+  const uint32_t wave_u = __builtin_amdgcn_readfirstlane((uint32_t)wave);
+  uint16_t* wstage = (uint16_t*)smem + wave_u * (CH * REG + 128);
+  const uint32_t dummy_addr = lds_addr(wstage + CH * REG + 2u * (uint32_t)lane);
+  // this wave's two spill slots of CH chains of CSTAGE words
+  uint16_t* const spill = a.spill + ((uint64_t)blockIdx.x * NW + wave_u) * (2 * SLOT);
+  uint32_t ptot[kMaxPlanes];  // words coded per plane by this wave
+#pragma unroll
+  for (int k = 0; k < kMaxPlanes; ++k) ptot[k] = 0;
+
+  // the ticket coded last, whose offset is not resolved yet (wave-uniform)
+  bool pend = false;
+  uint32_t p_tk = 0, p_slot = 0;
+  uint64_t p_agg = 0;
+  uint32_t p_pad[CH];
+  uint64_t p_bw[CH];  // byte offset of the unit's block-word pair, 0: inactive
+#pragma unroll
+  for (int c = 0; c < CH; ++c) p_pad[c] = 0, p_bw[c] = 0;
+
+  // resolve the pending ticket's offset, write its units' word offsets and
+  // copy its words from its slot to their final place
+  auto finish_pending = [&]() {
+    uint32_t qb[CH + 1];  // first uint4 of each chain in the ticket's output
+    qb[0] = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) qb[c + 1] = qb[c] + p_pad[c] / 8;
+    const uint32_t nq = qb[CH];
+    // the slot was written by other lanes of this wave: their stores first
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    const u32x4* s4 = (const u32x4*)(spill + p_slot * SLOT);
+    auto load = [&](u32x4 (&q)[CB], uint32_t j0) {
+#pragma unroll
+      for (int u = 0; u < CB; ++u) {
+        if (j0 + u * L < nq) {  // wave-uniform
+          // lanes past the end load entry 0 (always in the slot)
+          const uint32_t j = j0 + u * L + (uint32_t)lane;
+          const uint32_t jj = j < nq ? j : 0u;
+          // chain k's words sit at k * CSTAGE / 8 in the slot, at qb[k] here
+          uint32_t d = 0;
+#pragma unroll
+          for (int k = 1; k < CH; ++k) d = jj >= qb[k] ? k * (CSTAGE / 8) - qb[k] : d;
+          q[u] = s4[jj + d];
+        }
+      }
+    };
+    auto store = [&](const u32x4 (&q)[CB], u32x4* dst, uint32_t j0) {
+#pragma unroll
+      for (int u = 0; u < CB; ++u) {
+        const uint32_t j = j0 + u * L + (uint32_t)lane;
+        if (j < nq) dst[j] = q[u];
+      }
+    };
+    // the first batch is in flight during the look-back
+    u32x4 q[CB];
+    load(q, 0);
+    const uint64_t excl = resolve_prefix<4>(a.status, p_tk / CH, p_agg, lane);
+    uint64_t y = excl;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      if (p_bw[c] && lane == 0) ((uint32_t*)(a.out + p_bw[c]))[1] = (uint32_t)y;
+      y += p_pad[c];
+    }
+    u32x4* dst = (u32x4*)(a.out + a.d.data_off + 2ull * excl);
+    store(q, dst, 0);
+    for (uint32_t j0 = CB * L; j0 < nq; j0 += CB * L) {
+      load(q, j0);
+      store(q, dst, j0);
+    }
+    if ((uint64_t)p_tk + CH >= a.d.total_units && lane == 0) {
+      const uint64_t total_words = excl + p_agg;
+      const uint64_t size = a.d.data_off + 2ull * total_words;
+      if (a.d_out_size32) *a.d_out_size32 = (uint32_t)size;
+      if (a.d_out_size64) *a.d_out_size64 = size;
+      if (a.frame) {
+        ((uint64_t*)a.out)[6] = total_words;  // data_words at byte 48
+        ((uint64_t*)a.out)[7] = size;         // total_size at byte 56
+      }
+    }
+  };
+
+  uint32_t slot = 0;
+  uint32_t tk = 0;
+  if (lane == 0) tk = atomicAdd(a.ticket, CH);
+  tk = __builtin_amdgcn_readfirstlane(tk);
+  while ((uint64_t)tk < a.d.total_units) {
+    uint32_t x[CH], n[CH], fl[CH];
+    uint32_t cb[CH], sb[CH];  // LDS byte address of the stage and of the next word
+    const uint4* tab[CH];
+    uint64_t blk[CH];
+    int kp[CH];
+    bool act[CH];
+    const uint8_t* src[CH];
+    uint16_t* const wslot = spill + slot * SLOT;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const uint64_t unit = (uint64_t)tk + c;
+      act[c] = unit < a.d.total_units;
+      int k = 0;
+      uint64_t b = 0;
+      if (act[c]) {
+        if (a.d.order == 0) {  // planes of one element array: P is 1, 2 or 4
+          k = (int)((uint32_t)unit & (uint32_t)(P - 1));
+          b = unit >> (P == 4 ? 2 : (P == 2 ? 1 : 0));
+        } else {
+          while (k + 1 < P && unit >= s_ubase[k + 1]) ++k;
+          b = unit - s_ubase[k];
+        }
+      }
+      kp[c] = k;
+      blk[c] = b;
+      const LanePlane pl = s_pl[k];
+      const uint64_t start = b * BLOCK;
+      n[c] = act[c] ? (uint32_t)((pl.nsym - start) < BLOCK ? (pl.nsym - start) : BLOCK) : 0u;
+      n[c] = __builtin_amdgcn_readfirstlane(n[c]);
+      src[c] = pl.src;
+      x[c] = kMinState;
+      fl[c] = 0;
+      cb[c] = lds_addr(wstage + c * REG);
+      sb[c] = cb[c];
+      // S > 1 only on element arrays of S planes: chain c is plane c % S
+      tab[c] = s_tab + (S > 1 ? c % S : k) * kNumSymbols;
+    }
+    uint32_t n_min = n[0], n_max = n[0];
+#pragma unroll
+    for (int c = 1; c < CH; ++c) {
+      n_min = n[c] < n_min ? n[c] : n_min;
+      n_max = n[c] > n_max ? n[c] : n_max;
+    }
+    const uint32_t all = (uint32_t)div_up(n_max, L);
+
+    // stage upkeep: a chain with H or more unspilled words moves its first H
+    // to the slot and the rest (fewer than 256: at most 64 words per step,
+    // at most 4 steps apart) to the front
+    auto spill_full = [&]() {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const uint32_t un = (sb[c] - cb[c]) >> 1;  // unspilled words
+        if (un >= H) {
+          uint16_t* st = wstage + c * REG;
+          if ((uint32_t)lane < H4) ((uint4*)(wslot + c * CSTAGE + fl[c]))[lane] = ((const uint4*)st)[lane];
+          if (4u * (uint32_t)lane < un - H) ((uint2*)st)[lane] = ((const uint2*)(st + H))[lane];
+          fl[c] += H;
+          sb[c] -= 2u * H;
+        }
+      }
+    };
+
+    {
+      // Per step each lane takes one element of each of the ticket's BPT
+      // blocks; byte b of block d's element feeds chain d * S + b.  With
+      // S > 1 the first chain of a block is its plane 0, whose pointer is the
+      // element array; with S == 1 each chain has its own byte stream
+      // (single streams, concatenated planes).
+      const uint8_t* eb[BPT];
+#pragma unroll
+      for (int d = 0; d < BPT; ++d) eb[d] = src[d * S] + blk[d * S] * BLOCK * S;
+      uint32_t s = 0;
+      // Full blocks: each lane loads its element of each block
+      // (zero-extended: byte b of the register is byte b of the element).
+      // Two register buffers: while one batch is coded the next is loading.
+      using EL = typename ElemOf<S>::T;
+      auto load = [&](uint32_t (&buf)[BPT][KB], uint32_t s0) {
+#pragma unroll
+        for (int d = 0; d < BPT; ++d) {
+          const gptr<EL> bp = (gptr<EL>)eb[d] + lane + (uint64_t)s0 * L;
+#pragma unroll
+          for (int j = 0; j < KB; ++j) buf[d][j] = bp[j * L];
+        }
+      };
+      // table entries of the chains for step j of a batch
+      auto lookup = [&](uint4 (&e)[CH], const uint32_t (&buf)[BPT][KB], int j) {
+#pragma unroll
+        for (int d = 0; d < BPT; ++d) {
+          const uint32_t w = buf[d][j];
+          e[d * S] = *(const uint4*)((const char*)tab[d * S] + byte_x16<0>(w));
+          if constexpr (S > 1) e[d * S + 1] = *(const uint4*)((const char*)tab[d * S + 1] + byte_x16<1>(w));
+          if constexpr (S > 2) {
+            e[d * S + 2] = *(const uint4*)((const char*)tab[d * S + 2] + byte_x16<2>(w));
+            e[d * S + 3] = *(const uint4*)((const char*)tab[d * S + 3] + byte_x16<3>(w));
+          }
+        }
+      };
+      // The table entries of step j + 1 are read before step j is coded:
+      // they depend on the symbols only, not on the states.  The stage is
+      // checked every 4 steps: at most 4 * 64 words arrive in between.
+      auto code = [&](const uint32_t (&buf)[BPT][KB]) {
+        uint4 e[CH];
+        lookup(e, buf, 0);
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+          uint4 en[CH];
+          if (j + 1 < KB) lookup(en, buf, j + 1);
+#pragma unroll
+          for (int c = 0; c < CH; ++c) enc_step64f(x[c], sb[c], dummy_addr, e[c]);
+          if ((j & 3) == 3) spill_full();
+          if (j + 1 < KB) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) e[c] = en[c];
+          }
+        }
+      };
+      static_assert(KB % 4 == 0, "stage checks at batch-relative steps 3, 7, ...");
+      // The prefetch is unconditional (the last batch reloads itself) so the
+      // wait before each batch only covers that batch's loads.  Partial blocks
+      // (one per plane) take the byte path below, which never reads past the
+      // end of the input.
+      const uint32_t nbatch = n_min == BLOCK ? BLOCK / L / KB : 0u;
+      if (nbatch > 0) {
+        const uint32_t last = (nbatch - 1) * KB;
+        uint32_t bufA[BPT][KB], bufB[BPT][KB];
+        load(bufA, 0);
+        for (;;) {
+          load(bufB, s + KB <= last ? s + KB : last);
+          code(bufA);
+          s += KB;
+          if (s > last) break;
+          load(bufA, s + KB <= last ? s + KB : last);
+          code(bufB);
+          s += KB;
+          if (s > last) break;
+        }
+      }
+      for (; s < all; ++s) {
+        const uint32_t i = s * L + (uint32_t)lane;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          const bool v = i < n[c];
+          const uint32_t sym = v ? (uint32_t)eb[c / S][(uint64_t)i * S + (c % S)] : 0u;
+          enc_step64(x[c], sb[c], dummy_addr, tab[c][sym], v);
+        }
+        spill_full();
+      }
+    }
+
+    // ---- per chain: pad, move the stage's remainder to the slot, states and
+    // word count; ticket aggregate
+    uint64_t agg = 0;
+    uint32_t pad[CH];
+    uint64_t bwo[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const uint32_t rem = (sb[c] - cb[c]) >> 1;  // words still in the stage
+      const uint32_t cnt = fl[c] + rem;
+      const uint32_t padded = (uint32_t)round_up(cnt, 8);
+      if (cnt > CSTAGE && lane == 0) atomicOr(a.err, kErrStageOverflow);
+      uint16_t* st = wstage + c * REG;
+      if ((uint32_t)lane < padded - cnt) st[rem + lane] = 0;
+      const uint32_t nr = (padded - fl[c]) / 8;
+      for (uint32_t j = lane; j < nr; j += L) ((uint4*)(wslot + c * CSTAGE + fl[c]))[j] = ((const uint4*)st)[j];
+      const LanePlane& pl = s_pl[kp[c]];
+      if (act[c]) ((uint32_t*)(a.out + pl.section + kSectionFixed))[blk[c] * L + lane] = x[c];
+      bwo[c] = act[c] ? pl.section + kSectionFixed + 4ull * L * pl.nb + 8ull * blk[c] : 0ull;
+      if (act[c] && lane == 0) ((uint32_t*)(a.out + bwo[c]))[0] = (n[c] << 16) | cnt;
+      pad[c] = act[c] ? padded : 0u;
+      agg += pad[c];
+#pragma unroll
+      for (int k = 0; k < kMaxPlanes; ++k) ptot[k] += kp[c] == k ? pad[c] : 0u;
+    }
+    publish_aggregate(a.status, tk / CH, agg, lane);
+    if (pend) finish_pending();
+    pend = true;
+    p_tk = tk;
+    p_slot = slot;
+    p_agg = agg;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) p_pad[c] = pad[c], p_bw[c] = bwo[c];
+    slot ^= 1u;
+    // the next ticket only once this one is published: tickets start in
+    // ticket order, and every ticket a look-back waits on belongs to a wave
+    // that is coding it
+    __threadfence_block();
+    if (lane == 0) tk = atomicAdd(a.ticket, CH);
+    tk = __builtin_amdgcn_readfirstlane(tk);
+  }
+  if (pend) finish_pending();
+  // total coded words of each plane (the section headers start at 0)
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kMaxPlanes; ++k)
+      if (k < P && ptot[k]) atomicAdd((uint32_t*)(a.out + s_pl[k].section + 12), ptot[k]);
+  }
+}
+
 template <int L, int PB, int S, int CH, int NW, int HB>
 hipError_t launch_t(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
-  constexpr uint32_t REG = HB > 0 ? (1u << HB) + 256u : CSTAGE;
-  const size_t lds = (size_t)a.d.num_planes * kNumSymbols * sizeof(uint4) + (size_t)NW * (CH * REG + 128) * 2u;
+  constexpr uint32_t REG = L == 64 ? (1u << HB) + 256u : CSTAGE;
+  // dynamic LDS: the staging areas, after the tables (static in the 64-lane kernel)
+  const size_t lds = (L == 64 ? 0 : (size_t)a.d.num_planes * kNumSymbols * sizeof(uint4)) +
+                     (size_t)NW * (CH * REG + 128) * 2u;
   int per_cu = 0;
-  hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S, CH, NW, HB>, 64 * NW, lds);
+  hipError_t e;
+  if constexpr (L == 64)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode64_kernel<PB, S, CH / S, NW, HB>, 64 * NW, lds);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S, CH, NW>, 64 * NW, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   constexpr uint32_t UPW = CH * (L == 32 ? 2u : 1u);
@@ -361,13 +657,17 @@ hipError_t launch_t(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_s
   const uint64_t cap = (uint64_t)cu * (uint64_t)per_cu;
   if (grid > cap) grid = cap;
   if (grid < 1) grid = 1;
-  const uint64_t spill_bytes = HB > 0 ? grid * NW * CH * CSTAGE * 2ull : 0ull;
+  // two slots per wave of CH chains (64 lanes)
+  const uint64_t spill_bytes = L == 64 ? grid * NW * 2ull * CH * CSTAGE * 2ull : 0ull;
   if (need_spill) {
     *need_spill = spill_bytes;
     return hipSuccess;
   }
   if (spill_bytes > a.spill_bytes) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((encode_kernel<L, PB, S, CH, NW, HB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+  if constexpr (L == 64)
+    hipLaunchKernelGGL((encode64_kernel<PB, S, CH / S, NW, HB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+  else
+    hipLaunchKernelGGL((encode_kernel<L, PB, S, CH, NW>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
@@ -378,7 +678,7 @@ hipError_t launch_s(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_s
   if constexpr (L == 64) {
     // linear stages of 512 + 256 words (256 + 256 with four chains): 3-4 KiB
     // of LDS per wave
-    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 4, 9>(a, cu, s, need_spill);
+    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 4, 4, 8>(a, cu, s, need_spill);
     if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 4, 8>(a, cu, s, need_spill);
     if (S == 1) return launch_t<L, PB, 1, 1, 4, 9>(a, cu, s, need_spill);
   } else {
