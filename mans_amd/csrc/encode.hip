@@ -315,8 +315,9 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
 // time to publish their own aggregates, so the look-back sums published
 // entries instead of waiting for the slowest wave still coding.  The copy to
 // the final place issues a batch of loads before any store.
+// (byte streams: at least 4 waves per SIMD, a few loop-invariant spills)
 template <int PB, int S, int BPT, int NW, int HB>
-__global__ __launch_bounds__(64 * NW) void encode64_kernel(EncodeArgs a) {
+__global__ __launch_bounds__(64 * NW, S == 1 ? 4 : 1) void encode64_kernel(EncodeArgs a) {
   constexpr int CH = S * BPT;       // chains per lane
   constexpr int KB = kBatch / BPT;  // steps per prefetch batch
   constexpr uint32_t L = 64, BLOCK = 8192u;
@@ -676,11 +677,11 @@ hipError_t launch_s(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_s
   const int P = a.d.num_planes;
   const uint32_t S = a.d.plane[0].stride;
   if constexpr (L == 64) {
-    // linear stages of 512 + 256 words (256 + 256 with four chains): 3-4 KiB
-    // of LDS per wave
+    // four chains per lane, linear stages of 256 + 256 words: 4.3 KiB of
+    // LDS per wave
     if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 4, 4, 8>(a, cu, s, need_spill);
     if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 4, 8>(a, cu, s, need_spill);
-    if (S == 1) return launch_t<L, PB, 1, 1, 4, 9>(a, cu, s, need_spill);
+    if (S == 1) return launch_t<L, PB, 1, 4, 4, 8>(a, cu, s, need_spill);
   } else {
     if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 2, 0>(a, cu, s, need_spill);
     if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 1, 0>(a, cu, s, need_spill);
