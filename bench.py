@@ -13,6 +13,9 @@ bytes 2N + C, decode bytes C + N) are reported beside it; `roofline` is the
 dominant kernel's (the encode launch: reads N, writes C) from HIP events
 recorded around every launch in the timed region.  `cpu_baseline` is the CPU
 port of the same frame codec (oracle/, OpenMP) on a sample of the same codes.
+`batched` is the batched form on 4096 uneven buffers against one plain call
+per buffer; `served` is the request server on a compress / decompress mix
+(burst throughput and open-loop p50 / p99 latency) against plain calls.
 """
 from __future__ import annotations
 
@@ -45,6 +48,9 @@ def parse():
     ap.add_argument("--batch-count", type=int, default=4096,
                     help="buffers of the batched workload (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve-count", type=int, default=1024,
+                    help="requests of the served workload (0 skips it)")
+    ap.add_argument("--serve-budget-gib", type=int, default=8, help="server memory budget")
     ap.add_argument("--dump", default=None,
                     help="write the generated codes to this file and exit (input of mans_profile)")
     return ap.parse_args()
@@ -130,6 +136,7 @@ def main() -> int:
         cpu = cpu_baseline(x, args)
     del dec, out
     batched = batched_bench(args) if args.batch_count > 0 else None
+    served = served_bench(args) if args.serve_count > 0 else None
 
     line = {
         "metric": METRIC,
@@ -170,6 +177,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
         "gen_s": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)
@@ -319,6 +327,133 @@ def batched_bench(args):
         "decode_speedup_vs_plain": round(plain_dec_ms / graph_dec_ms, 2),
         "decode_speedup_vs_plain_kernel_time": round(plain_dec_kernel_ms / graph_dec_ms, 2),
         "decode_round_trip_bit_exact": dec_ok,
+        "gen_s": round(gen_s, 2),
+    }
+
+
+def served_bench(args):
+    """The served form: compress and decompress requests (every other one a
+    decompress of that buffer's frame) sent to the request server
+    (mans_server_*), which admits them against its device arena and codes
+    them in batched launches while the previous batch runs.  Measured on the
+    same requests three ways: plain (each request alone: one
+    mans_frame_encode / mans_frame_decode call, then a synchronise; its
+    latency is that call's), the server under a burst (all requests submitted
+    at once: aggregate throughput = request bytes / makespan), and the server
+    under open-loop Poisson arrivals at 50 % and 80 % of the burst rate
+    (latency = submit to completion, queueing included, as the server reports
+    it).  Every served result is compared byte for byte with the plain one."""
+    import torch
+
+    import mans_amd
+    from mans_amd import workloads
+
+    t0 = time.time()
+    xs, widths = workloads.batch_buffers(args.serve_count, seed_base=50_000, lo=16 << 10, hi=4 << 20)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    p = mans_amd.params(args.prob_bits, 64)
+    ws = mans_amd.Workspace(0)
+    for eb in (1, 2, 4):
+        ws.reserve(4 << 20, mans_amd.params(args.prob_bits, 64, eb, mans_amd.MODE_PLANES))
+    kinds = ["c" if i % 2 == 0 else "d" for i in range(len(xs))]
+    frames, sizes, headers = [], [], []
+    for x in xs:
+        f, n = mans_amd.frame_encode(x, p, ws)
+        frames.append(f)
+        sizes.append(int(n))
+        headers.append(bytes(f[:128].cpu().numpy().tobytes()))
+    req_bytes = [x.numel() * x.element_size() for x in xs]
+    total = sum(req_bytes)
+
+    # plain: each request alone
+    def run_plain(i):
+        x = xs[i]
+        if kinds[i] == "c":
+            return mans_amd.frame_encode(x, p, ws)[0]
+        return mans_amd.frame_decode(frames[i], sizes[i], x.dtype, x.numel(), ws, header=headers[i])
+
+    for i in range(min(32, len(xs))):
+        run_plain(i)
+    torch.cuda.synchronize()
+    lat_plain = []
+    t0 = time.perf_counter()
+    for i in range(len(xs)):
+        a = time.perf_counter()
+        run_plain(i)
+        torch.cuda.synchronize()
+        lat_plain.append((time.perf_counter() - a) * 1e3)
+    plain_s = time.perf_counter() - t0
+    ws.check()
+
+    srv = mans_amd.Server(p, args.serve_budget_gib << 30)
+
+    def submit(i):
+        if kinds[i] == "c":
+            return srv.submit_compress(xs[i])
+        return srv.submit_decompress(frames[i], sizes[i], xs[i].numel(), widths[i])
+
+    def collect(ids, check):
+        res = [srv.wait(rid, 120_000) for rid in ids]
+        ok = all(r is not None and r.status == 0 for r in res)
+        if ok and check:
+            for i, r in enumerate(res):
+                if kinds[i] == "c":
+                    ok = ok and torch.equal(srv.fetch(r), frames[i][:sizes[i]])
+                else:
+                    ok = ok and torch.equal(srv.fetch(r, xs[i].dtype), xs[i])
+        lat = [r.latency_ms for r in res if r is not None]
+        for rid in ids:
+            srv.release(rid)
+        return ok, lat
+
+    collect([submit(i) for i in range(min(32, len(xs)))], False)
+    t0 = time.perf_counter()
+    ids = [submit(i) for i in range(len(xs))]
+    for rid in ids:
+        srv.wait(rid, 120_000)
+    burst_s = time.perf_counter() - t0
+    ok, lat_burst = collect(ids, True)
+
+    # open loop: Poisson arrivals (fixed seed) at a fraction of the burst rate
+    rate = len(xs) / burst_s
+    loads = {}
+    for frac in (0.5, 0.8):
+        gaps = np.random.default_rng(7).exponential(1.0 / (frac * rate), len(xs))
+        due = np.cumsum(gaps)
+        ids = []
+        t0 = time.perf_counter()
+        for i in range(len(xs)):
+            while time.perf_counter() - t0 < due[i]:
+                pass
+            ids.append(submit(i))
+        ok_i, lat = collect(ids, False)
+        ok = ok and ok_i
+        loads[f"load_{int(frac * 100)}"] = {
+            "offered_req_per_s": round(frac * rate, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        }
+    st = srv.stats()
+    srv.close()
+    ws.close()
+    return {
+        "workload": f"{len(xs)} requests, sizes log-uniform in [16 KiB, 4 MiB], uint8/16/32 at 20/60/20 %, "
+                    f"Laplace or Zipf values, seeds 50000+i; even requests compress, odd ones decompress "
+                    f"that buffer's frame; byte-plane frames, 64 lanes, p={args.prob_bits}",
+        "request_bytes": total,
+        "plain_s": round(plain_s, 4),
+        "plain_GBps": round(total / plain_s / 1e9, 2),
+        "plain_p50_ms": round(float(np.percentile(lat_plain, 50)), 3),
+        "plain_p99_ms": round(float(np.percentile(lat_plain, 99)), 3),
+        "served_burst_s": round(burst_s, 4),
+        "served_GBps": round(total / burst_s / 1e9, 2),
+        "speedup_vs_plain": round(plain_s / burst_s, 2),
+        "served_burst_p50_ms": round(float(np.percentile(lat_burst, 50)), 3),
+        "served_burst_p99_ms": round(float(np.percentile(lat_burst, 99)), 3),
+        **loads,
+        "server_batches": st["batches"],
+        "bit_exact_vs_plain": ok,
         "gen_s": round(gen_s, 2),
     }
 

@@ -39,11 +39,13 @@ struct DPlane {
 };
 
 // L lanes per unit, PB probability bits, S element bytes, CH chains per lane
-// (CH == S: chain c decodes byte c of each element), NW wavefronts per group,
-// RB: log2 of the ring in words (64-lane format), 0 = whole-unit staging.
+// (CH / S consecutive element blocks, 64-lane format only: chain d * S + b
+// decodes byte b of block d's elements), NW wavefronts per group, RB: log2 of
+// the ring in words (64-lane format), 0 = whole-unit staging.
 template <int L, int PB, int S, int CH, int NW, int RB>
 __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
   using T = typename ElemOf<S>::T;
+  constexpr int BPT = CH / S;  // element blocks per wave iteration
   constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
   constexpr uint32_t HALVES = L == 32 ? 2u : 1u;
   constexpr uint32_t UPW = CH * HALVES;
@@ -57,7 +59,7 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
   constexpr uint32_t HSTAGE = CSTAGE / HALVES;
   constexpr uint32_t REG = RING ? R + 64u : CSTAGE;  // LDS words per chain
   constexpr uint32_t UNIT_CAP = L * lane_max_words(BLOCK / L, PB);
-  static_assert(CH == S, "chain c decodes byte c of each element");
+  static_assert(CH % S == 0 && (L == 64 || CH == S) && CH * 8 <= 32, "chain d * S + b decodes byte b of block d");
   static_assert(!RING || (L == 64 && H >= 128 && H4 <= 64), "ring: 64 lanes, 128 <= R/2 <= 512 words");
   extern __shared__ uint4 smem[];
   __shared__ DPlane s_pl[kMaxPlanes];
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
     bool pend[CH];
     uint4 rv[CH];
     const uint4* src4[CH];
-    uint8_t* dst0 = nullptr;
+    uint8_t* dstp[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const uint64_t unit = wu * UPW + half * CH + c;
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
       }
       blk[c] = b;
       const DPlane pl = s_pl[k];
-      if (c == 0) dst0 = pl.dst;
+      dstp[c] = pl.dst;
       const uint8_t* sec = a.in + pl.section;
       const uint64_t start = b * BLOCK;
       uint32_t nn = 0, words = 0, y = 0;
@@ -325,22 +327,27 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
       n_min = __builtin_amdgcn_readfirstlane(n_min);
       n_max = __builtin_amdgcn_readfirstlane(n_max);
     }
-    // chain 0 of a group is plane 0 whenever CH > 1: its destination is the
-    // element array
-    __attribute__((address_space(1))) T* ep = (__attribute__((address_space(1))) T*)dst0 + blk[0] * BLOCK + lin;
+    // the first chain of each block is its plane 0 whenever S > 1: its
+    // destination is the element array
+    __attribute__((address_space(1))) T* ep[BPT];
+#pragma unroll
+    for (int d = 0; d < BPT; ++d)
+      ep[d] = (__attribute__((address_space(1))) T*)dstp[d * S] + blk[d * S] * BLOCK + lin;
+    // block d's element of step s: bytes d * S .. d * S + S - 1 of el
+    auto put = [&](uint32_t s, uint32_t el, const bool (&v)[CH]) {
+#pragma unroll
+      for (int d = 0; d < BPT; ++d)
+        if (v[d * S]) ep[d][(uint64_t)s * L] = (T)(el >> (8 * S * d));
+    };
     const uint32_t full = n_min / L;
     const uint32_t all = (uint32_t)div_up(n_max, L);
     for (uint32_t s = all; s > full;) {
       --s;
       const uint32_t i = s * L + lin;
-      bool v[CH], any = false;
+      bool v[CH];
 #pragma unroll
-      for (int c = 0; c < CH; ++c) {
-        v[c] = i < n[c];
-        any |= v[c];
-      }
-      const uint32_t el = dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, v, half);
-      if (any) ep[(uint64_t)s * L] = (T)el;
+      for (int c = 0; c < CH; ++c) v[c] = i < n[c];
+      put(s, dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, v, half), v);
       refill();
     }
     bool vall[CH];
@@ -351,14 +358,14 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
     uint32_t s = full;
     if (s & 1u) {
       --s;
-      ep[(uint64_t)s * L] = (T)dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half);
+      put(s, dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half), vall);
       refill();
     }
     while (s > 0) {
       s -= 2;
-      ep[(uint64_t)(s + 1) * L] = (T)dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half);
+      put(s + 1, dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half), vall);
       if constexpr (!RING) refill();
-      ep[(uint64_t)s * L] = (T)dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half);
+      put(s, dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half), vall);
       refill();
     }
     // a valid block uses every word and returns each lane to 2^15
@@ -398,7 +405,9 @@ hipError_t launch_s(const DecodeArgs& a, int cu, hipStream_t s) {
   const int P = a.d.num_planes;
   const uint32_t S = a.d.plane[0].stride;
   if constexpr (L == 64) {
-    // rings of 1024 words (512 with four chains): 4-9 KiB of LDS per wave
+    // one chain per byte plane of one element block, rings of 1024 words
+    // (512 with four chains): 2-9 KiB of LDS per wave.  (Two blocks per
+    // wave, four chains, measured slower for uint16 and uint8 elements.)
     if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 8, 10>(a, cu, s);
     if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 8, 9>(a, cu, s);
     if (S == 1) return launch_t<L, PB, 1, 1, 8, 10>(a, cu, s);
