@@ -291,8 +291,10 @@ __device__ __forceinline__ uint32_t dec_steps(uint32_t (&x)[CH], uint32_t (&ptr)
     } else {
       base = ptr[c] - cnt;
     }
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u)) - below;
-    word[c] = (stage[c] + (base & MASK))[rank];
+    // slot of this lane's word: (base & MASK) + rank, the base taken in as
+    // mbcnt's accumulator (no scalar shift / add for the address)
+    const uint32_t slot = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, base & MASK)) - below;
+    word[c] = stage[c][slot];
     ptr[c] = base;
   }
   // keep the reads unconditional: without this the compiler moves each under

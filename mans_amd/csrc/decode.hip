@@ -58,6 +58,8 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
   constexpr uint32_t HSTAGE = CSTAGE / HALVES;
   constexpr uint32_t REG = RING ? R + 64u : CSTAGE;  // LDS words per chain
+  constexpr uint32_t RK = H >= 512 ? 4u : 2u;          // steps between ring checks
+  static_assert(!RING || 2u * 64u * RK <= H, "a chunk's load starts a check before its commit");
   constexpr uint32_t UNIT_CAP = L * lane_max_words(BLOCK / L, PB);
   static_assert(CH % S == 0 && (L == 64 || CH == S) && CH * 8 <= 32, "chain d * S + b decodes byte b of block d");
   static_assert(!RING || (L == 64 && H >= 128 && H4 <= 64), "ring: 64 lanes, 128 <= R/2 <= 512 words");
@@ -287,10 +289,12 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
     for (int c = 0; c < CH; ++c) __asm__ volatile("" : "+v"(x[c]));
     __threadfence_block();
 
-    // ring upkeep, at most 2 steps apart: start loading the chunk below once
+    // ring upkeep, at most RK steps apart: start loading the chunk below once
     // the one above the lowest is used up, and write it to LDS once fewer
-    // than 128 words remain above lo (the next two steps may take up to 64
-    // each).  (ptr is signed here: below 0 only on a corrupt stream.)
+    // than 64 RK words remain above lo (the next RK steps may take up to 64
+    // each; the load was started at least one check earlier, since RK steps
+    // take at most 64 RK <= H - 64 RK words).  (ptr is signed here: below 0
+    // only on a corrupt stream.)
     auto refill = [&]() {
       if constexpr (RING) {
 #pragma unroll
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
             if ((uint32_t)lane < H4) rv[c] = src4[c][j * H4 + lane];
             pend[c] = true;
           }
-          if (pend[c] && (int32_t)ptr[c] < (int32_t)(lo[c] + 128u)) {
+          if (pend[c] && (int32_t)ptr[c] < (int32_t)(lo[c] + 64u * RK)) {
             const uint32_t j = lo[c] / H - 1;
             uint16_t* reg = wstage + c * REG;
             if ((uint32_t)lane < H4) ((uint4*)(reg + (j & 1) * H))[lane] = rv[c];
@@ -353,19 +357,19 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
     bool vall[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) vall[c] = true;
-    // two steps between ring checks (full is a multiple of two for whole
-    // blocks; an odd remainder takes a single step first)
+    // RK steps between ring checks (full is a multiple of RK for whole
+    // blocks; a remainder takes single steps first)
     uint32_t s = full;
-    if (s & 1u) {
+    while (s % RK) {
       --s;
       put(s, dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half), vall);
       refill();
     }
     while (s > 0) {
-      s -= 2;
-      put(s + 1, dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half), vall);
-      if constexpr (!RING) refill();
-      put(s, dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half), vall);
+      s -= RK;
+#pragma unroll
+      for (int k = (int)RK - 1; k >= 0; --k)
+        put(s + k, dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half), vall);
       refill();
     }
     // a valid block uses every word and returns each lane to 2^15
