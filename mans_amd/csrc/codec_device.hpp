@@ -113,9 +113,17 @@ __device__ __forceinline__ void enc_step64f(uint32_t& x, uint32_t& sbase, uint32
   const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
   __asm__ volatile("" : "+s"(sbase));
   const uint32_t emit = sbase + 2u * rank;
-  uint32_t addr;
-  __asm__("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(addr) : "v"(dummy), "v"(emit), "s"(m));
-  *(lds_u16*)(uintptr_t)addr = (uint16_t)x;
+  // only the emitting lanes store: exec is narrowed to m for the store and
+  // restored inside the same block (s_and_saveexec also writes SCC)
+  uint64_t saved;
+  __asm__ volatile(
+      "s_and_saveexec_b64 %0, %1\n\t"
+      "ds_write_b16 %2, %3\n\t"
+      "s_mov_b64 exec, %0"
+      : "=&s"(saved)
+      : "s"(m), "v"(emit), "v"(x)
+      : "memory", "scc");
+  (void)dummy;
   sbase += 2u * (uint32_t)__popcll(m);
   const uint32_t q = (__umulhi(xr, e.y) + xr) >> (e.z & 31u);
   const uint32_t xn = __umul24(q, e.w) + xr;
