@@ -189,7 +189,10 @@ def batched_bench(args):
     its own byte-plane frame (64-lane format), once by the plain codec
     (one mans_frame_encode per buffer, sizes left on the device) and once by
     one batched launch (eager, then replayed from a captured graph).  Frames
-    are compared byte for byte."""
+    are compared byte for byte.  Before the batched launches are timed the
+    autotuner (mans_batch_tune) picks their shape from workgroup size
+    {256, 512, 1024} x symbols per lane (encode {4, 8, 16}, decode
+    {1, 4, 8, 16}) at 64 lanes; its table is reported."""
     import torch
 
     import mans_amd
@@ -238,6 +241,9 @@ def batched_bench(args):
     if ok:
         sz = dsz.cpu().tolist()
         ok = all(torch.equal(outs_batch[i][:sz[i]], outs_plain[i][:sz[i]]) for i in range(len(xs)))
+    enc_default = b.shape
+    enc_tune = b.tune(reps=args.batch_steps)
+    ws.check()
     e0.record(stream)
     for _ in range(args.batch_steps):
         b.run()
@@ -258,6 +264,8 @@ def batched_bench(args):
     graph_ms = e0.elapsed_time(e1) / args.batch_steps
     ok = ok and bool(torch.equal(b.sizes, dsz))
     ws.check()
+    if ok:
+        ok = all(torch.equal(outs_batch[i][:sz[i]], outs_plain[i][:sz[i]]) for i in range(len(xs)))
     comp = int(dsz.sum().item())
 
     # decode: one mans_frame_decode per frame (headers handed over from the
@@ -289,6 +297,11 @@ def batched_bench(args):
     ws.check()
     dec_ok = all(torch.equal(outs[i], xs[i]) and torch.equal(dec_plain[i][:xs[i].numel()], xs[i])
                  for i in range(len(xs)))
+    dec_default = d.shape
+    dec_tune = d.tune(reps=args.batch_steps)
+    ws.check()
+    for o in d.outs:
+        o.zero_()
     gd = torch.cuda.CUDAGraph()
     with torch.cuda.graph(gd, stream=s):
         d.run(stream=s)
@@ -301,6 +314,7 @@ def batched_bench(args):
     graph_dec_ms = e0.elapsed_time(e1) / args.batch_steps
     dec_ok = dec_ok and all(torch.equal(outs[i], xs[i]) for i in range(len(xs)))
     ws.check()
+    enc_picked, dec_picked = b.shape, d.shape
     d.close()
     b.close()
     ws.close()
@@ -327,6 +341,14 @@ def batched_bench(args):
         "decode_speedup_vs_plain": round(plain_dec_ms / graph_dec_ms, 2),
         "decode_speedup_vs_plain_kernel_time": round(plain_dec_kernel_ms / graph_dec_ms, 2),
         "decode_round_trip_bit_exact": dec_ok,
+        "autotune": {
+            "candidates": "wg_threads {256, 512, 1024} x syms_per_lane (encode {4, 8, 16}, decode {1, 4, 8, 16}), "
+                          "64 lanes; ms < 0: does not fit the 160 KiB LDS",
+            "encode": {"default": list(enc_default), "picked": list(enc_picked),
+                       "ms": [[r["wg_threads"], r["syms_per_lane"], round(r["ms"], 3)] for r in enc_tune]},
+            "decode": {"default": list(dec_default), "picked": list(dec_picked),
+                       "ms": [[r["wg_threads"], r["syms_per_lane"], round(r["ms"], 3)] for r in dec_tune]},
+        },
         "gen_s": round(gen_s, 2),
     }
 
@@ -339,7 +361,8 @@ def served_bench(args):
     same requests three ways: plain (each request alone: one
     mans_frame_encode / mans_frame_decode call, then a synchronise; its
     latency is that call's), the server under a burst (all requests submitted
-    at once: aggregate throughput = request bytes / makespan), and the server
+    in one mans_server_submit_many call: aggregate throughput = request
+    bytes / makespan), and the server
     under open-loop Poisson arrivals at 50 % and 80 % of the burst rate
     (latency = submit to completion, queueing included, as the server reports
     it).  Every served result is compared byte for byte with the plain one."""
@@ -388,10 +411,12 @@ def served_bench(args):
 
     srv = mans_amd.Server(p, args.serve_budget_gib << 30)
 
+    reqs = [srv.compress_request(xs[i]) if kinds[i] == "c"
+            else srv.decompress_request(frames[i], sizes[i], xs[i].numel(), widths[i])
+            for i in range(len(xs))]
+
     def submit(i):
-        if kinds[i] == "c":
-            return srv.submit_compress(xs[i])
-        return srv.submit_decompress(frames[i], sizes[i], xs[i].numel(), widths[i])
+        return srv.submit(reqs[i])
 
     def collect(ids, check):
         res = [srv.wait(rid, 120_000) for rid in ids]
@@ -409,7 +434,7 @@ def served_bench(args):
 
     collect([submit(i) for i in range(min(32, len(xs)))], False)
     t0 = time.perf_counter()
-    ids = [submit(i) for i in range(len(xs))]
+    ids = srv.submit_many(reqs)
     for rid in ids:
         srv.wait(rid, 120_000)
     burst_s = time.perf_counter() - t0

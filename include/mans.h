@@ -176,6 +176,41 @@ mans_status mans_batch_decode_create(mans_batch** b, const mans_params* p, const
                                      int32_t count, mans_workspace* ws);
 mans_status mans_batch_decode_run(mans_batch* b, void* stream);
 mans_status mans_batch_destroy(mans_batch* b);
+/* Launch shape of a batch's kernels.  No shape changes an output bit; the
+ * reference's counterparts are compile-time constants (warp 32,
+ * DeviceUtils.h:49; unroll 8, GpuANSEncode.h:126; 256 threads per block of
+ * the encode launch, GpuANSEncode.h:329-338). */
+typedef struct mans_batch_shape {
+  int32_t wg_threads;    /* threads per workgroup: 256, 512 or 1024 */
+  int32_t syms_per_lane; /* symbols each lane codes per batch.  Encode: 4, 8 or 16
+                            elements loaded ahead; decode: 1, 4, 8 or 16 steps
+                            between two upkeeps of the word ring */
+  int32_t lanes;         /* interleave width: 64 (the batched frames' format) */
+  int32_t reserved;      /* 0 */
+} mans_batch_shape;
+
+typedef struct mans_tune_entry {
+  mans_batch_shape shape;
+  double ms;             /* mean time of one run; < 0 if the shape does not fit the LDS */
+} mans_tune_entry;
+
+/* The shape the batch runs with: the tuned one for its key (a previous
+ * mans_batch_tune in this process, or the file named by the environment
+ * variable MANS_TUNE_CACHE) or else the default. */
+mans_status mans_batch_get_shape(const mans_batch* b, mans_batch_shape* shape);
+/* MANS_ERR_INVALID_ARG if the shape is not a candidate or does not fit. */
+mans_status mans_batch_set_shape(mans_batch* b, const mans_batch_shape* shape);
+/* Autotuner: runs the batch `reps` times (after one warm-up run) with every
+ * candidate shape (wg_threads x syms_per_lane, lanes 64: 9 for encode, 12
+ * for decode; at most 16) on `stream`, keeps
+ * the fastest in b and under b's key in the process-wide cache (appended to
+ * MANS_TUNE_CACHE when set), and writes up to max_entries timings to
+ * h_entries (*n_entries = candidates).  Synchronises; an encode batch's
+ * frames are written by the runs (identically for every shape).  The key is
+ * (direction, prob_bits, buffer count and total bytes rounded to powers of
+ * two, element widths present, compute units). */
+mans_status mans_batch_tune(mans_batch* b, void* stream, int32_t reps, mans_tune_entry* h_entries,
+                            int32_t max_entries, int32_t* n_entries);
 /* create + run + destroy (synchronises the stream before freeing). */
 mans_status mans_encode_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, uint64_t* d_sizes,
                               mans_workspace* ws, void* stream);
@@ -218,6 +253,9 @@ typedef struct mans_server mans_server;
 mans_status mans_server_create(mans_server** s, const mans_params* p, uint64_t memory_budget, int32_t device);
 /* MANS_ERR_BUDGET if the request's output can never fit the arena. */
 mans_status mans_server_submit(mans_server* s, const mans_request* r, uint64_t* id);
+/* count requests queued together, in order (one worker wake-up); ids[i] is
+ * request i's id.  All are checked first: on any error none is queued. */
+mans_status mans_server_submit_many(mans_server* s, const mans_request* r, int32_t count, uint64_t* ids);
 /* Waits up to timeout_ms (< 0: no limit); MANS_ERR_BUSY if not done yet. */
 mans_status mans_server_wait(mans_server* s, uint64_t id, int32_t timeout_ms, mans_result* out);
 /* Frees a finished request's output. */

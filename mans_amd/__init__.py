@@ -49,6 +49,15 @@ class Result(C.Structure):
                 ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
 
 
+class BatchShape(C.Structure):
+    _fields_ = [("wg_threads", C.c_int32), ("syms_per_lane", C.c_int32), ("lanes", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("shape", BatchShape), ("ms", C.c_double)]
+
+
 REQ_COMPRESS = 0
 REQ_DECOMPRESS = 1
 
@@ -77,10 +86,14 @@ SIGNATURES = {
     "mans_batch_decode_create": (C.c_int, [C.POINTER(_vp), _P, _B, _i32, _vp]),
     "mans_batch_decode_run": (C.c_int, [_vp, _vp]),
     "mans_batch_destroy": (C.c_int, [_vp]),
+    "mans_batch_get_shape": (C.c_int, [_vp, C.POINTER(BatchShape)]),
+    "mans_batch_set_shape": (C.c_int, [_vp, C.POINTER(BatchShape)]),
+    "mans_batch_tune": (C.c_int, [_vp, _vp, _i32, C.POINTER(TuneEntry), _i32, C.POINTER(_i32)]),
     "mans_encode_batch": (C.c_int, [_P, _B, _i32, _vp, _vp, _vp]),
     "mans_decode_batch": (C.c_int, [_P, _B, _i32, _vp, _vp]),
     "mans_server_create": (C.c_int, [C.POINTER(_vp), _P, _u64, _i32]),
     "mans_server_submit": (C.c_int, [_vp, C.POINTER(Request), C.POINTER(_u64)]),
+    "mans_server_submit_many": (C.c_int, [_vp, C.POINTER(Request), _i32, C.POINTER(_u64)]),
     "mans_server_wait": (C.c_int, [_vp, _u64, _i32, C.POINTER(Result)]),
     "mans_server_release": (C.c_int, [_vp, _u64]),
     "mans_server_stats": (C.c_int, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
@@ -296,7 +309,32 @@ def frame_decode(comp, size: int, dtype, n: int, ws: Workspace, out=None, stream
     return out[:n]
 
 
-class EncodeBatch:
+class _Tunable:
+    """Launch shape and autotuner of a planned batch (mans_batch_*_shape,
+    mans_batch_tune)."""
+
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        """(threads per workgroup, symbols per lane, lanes)."""
+        sh = BatchShape()
+        _check(lib().mans_batch_get_shape(self.handle, C.byref(sh)), "mans_batch_get_shape")
+        return sh.wg_threads, sh.syms_per_lane, sh.lanes
+
+    def set_shape(self, wg_threads: int, syms_per_lane: int, lanes: int = 64) -> None:
+        sh = BatchShape(wg_threads, syms_per_lane, lanes, 0)
+        _check(lib().mans_batch_set_shape(self.handle, C.byref(sh)), "mans_batch_set_shape")
+
+    def tune(self, reps: int = 3, stream=None) -> list[dict]:
+        """Times every candidate shape on this batch and keeps the fastest;
+        returns one row per candidate (ms < 0: does not fit the LDS)."""
+        ent = (TuneEntry * 16)()
+        n = C.c_int32(0)
+        _check(lib().mans_batch_tune(self.handle, _stream(stream), reps, ent, 16, C.byref(n)), "mans_batch_tune")
+        return [{"wg_threads": e.shape.wg_threads, "syms_per_lane": e.shape.syms_per_lane,
+                 "lanes": e.shape.lanes, "ms": e.ms} for e in ent[:n.value]]
+
+
+class EncodeBatch(_Tunable):
     """A planned batch of independent buffers, each coded as a byte-plane
     frame (mans_batch_encode_create).  `inputs`: CUDA tensors of 1/2/4-byte
     elements (16-byte aligned storage); frames go to `outs` (allocated at
@@ -340,7 +378,7 @@ class EncodeBatch:
             pass
 
 
-class DecodeBatch:
+class DecodeBatch(_Tunable):
     """A planned batch decode (mans_batch_decode_create) of byte-plane frames
     (64 lanes): `frames` are uint8 CUDA tensors holding each frame (at least
     its size in bytes), `like` gives each buffer's element count and dtype
@@ -389,17 +427,33 @@ class Server:
         _check(lib().mans_server_create(C.byref(h), C.byref(p), memory_budget, device), "mans_server_create")
         self.handle = h
 
-    def submit_compress(self, x) -> int:
-        r = Request(REQ_COMPRESS, x.element_size(), _ptr(x), 0, x.numel())
+    @staticmethod
+    def compress_request(x) -> Request:
+        return Request(REQ_COMPRESS, x.element_size(), _ptr(x), 0, x.numel())
+
+    @staticmethod
+    def decompress_request(frame, frame_bytes: int, n: int, elem_bytes: int) -> Request:
+        return Request(REQ_DECOMPRESS, elem_bytes, frame.data_ptr(), frame_bytes, n)
+
+    def submit(self, r: Request) -> int:
         rid = C.c_uint64(0)
         _check(lib().mans_server_submit(self.handle, C.byref(r), C.byref(rid)), "mans_server_submit")
         return rid.value
 
+    def submit_compress(self, x) -> int:
+        return self.submit(self.compress_request(x))
+
     def submit_decompress(self, frame, frame_bytes: int, n: int, elem_bytes: int) -> int:
-        r = Request(REQ_DECOMPRESS, elem_bytes, frame.data_ptr(), frame_bytes, n)
-        rid = C.c_uint64(0)
-        _check(lib().mans_server_submit(self.handle, C.byref(r), C.byref(rid)), "mans_server_submit")
-        return rid.value
+        return self.submit(self.decompress_request(frame, frame_bytes, n, elem_bytes))
+
+    def submit_many(self, reqs) -> list[int]:
+        """Queue a list of Requests at once (one call, one worker wake-up);
+        returns their ids in order.  On an error none is queued."""
+        n = len(reqs)
+        arr = (Request * max(1, n))(*reqs)
+        ids = (C.c_uint64 * max(1, n))()
+        _check(lib().mans_server_submit_many(self.handle, arr, n, ids), "mans_server_submit_many")
+        return [ids[i] for i in range(n)]
 
     def wait(self, rid: int, timeout_ms: int = -1) -> Result | None:
         res = Result()

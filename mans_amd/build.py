@@ -31,7 +31,8 @@ DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 
-HIP_SOURCES = ["hist.hip", "encode.hip", "decode.hip", "adm.hip", "batch.hip", "capi.hip"]
+HIP_SOURCES = ["hist.hip", "encode.hip", "decode.hip", "adm.hip", "batch.hip", "batch_p9.hip", "batch_p10.hip",
+               "batch_p11.hip", "capi.hip"]
 HOST_SOURCES = ["gpu_ans_shim.cpp", "serve.cpp"]
 
 

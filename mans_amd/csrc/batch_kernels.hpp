@@ -1,0 +1,836 @@
+// batch_kernels.hpp -- many independent buffers, each coded as a byte-plane
+// frame, in one persistent launch.  Included by one translation unit per
+// probability precision (batch_p9.hip, batch_p10.hip, batch_p11.hip), each
+// instantiating the kernels of every launch shape (BatchShape) for it.
+//
+// Every buffer's frame is byte-identical to what mans_frame_encode writes for
+// it alone (MANS_MODE_PLANES, 64-lane format): the same histogram and
+// normalisation (GpuANSStatistics.h:23-151, :186-406), the same rANS
+// arithmetic and container (GpuANSEncode.h:24-164, :180-284).  What changes
+// is the schedule.  One queue of work items (an atomic counter) holds first
+// the histogram chunks of every buffer, then the encode tickets of every
+// buffer in order; every wavefront takes items one at a time:
+//  * a histogram item counts one chunk of one buffer (all its byte planes)
+//    into LDS and adds the counts to the buffer's accumulators; the wavefront
+//    that finishes a buffer's last chunk normalises its planes (one wavefront
+//    per buffer), writes the frame and plane headers and publishes the
+//    encoder tables, then marks the buffer ready;
+//  * an encode ticket covers two consecutive units of one buffer (both byte
+//    planes of a uint16 block, two planes of a uint32 block, or two blocks of
+//    a uint8 buffer).  It waits until its buffer is ready, codes the two
+//    chains with the single-buffer coder's step and LDS ring, and resolves
+//    its output offset by decoupled look-back within its buffer.
+// A ticket only waits on items taken before it (its buffer's histogram
+// chunks and earlier tickets of its buffer), all held by running
+// wavefronts, so the launch always drains.  Data published inside the launch
+// (accumulators, tables, ready flags, look-back words) goes through
+// device-scope atomics; look-back words and ready flags carry a 14-bit launch
+// generation, so nothing needs clearing between launches.
+#pragma once
+#include "codec_device.hpp"
+
+namespace mans {
+namespace {
+
+constexpr uint32_t kL = 64;            // lanes per unit
+constexpr uint32_t kBlock = 8192;      // symbols per unit
+constexpr uint32_t kRB = 10;           // ring: 1024 words per chain
+constexpr uint32_t kR = 1u << kRB, kH = kR / 2, kMask = kR - 1;
+constexpr uint32_t kGenShift = 48;
+constexpr uint64_t kValue48 = (1ull << kGenShift) - 1;
+// per-wave LDS in 32-bit words: two encoder tables (or the histogram
+// counters, or the normalisation keys) | two rings | dummy slots
+constexpr uint32_t kTabWords = 2 * 256 * 4;
+constexpr uint32_t kRingWords = 2 * kR / 2;
+constexpr uint32_t kWaveWords = kTabWords + kRingWords + 64;
+
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+__device__ __forceinline__ uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint64_t atomic_ld64(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_st64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_ld32(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_st32(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// orders this wavefront's LDS accesses (a workgroup-scope fence emits no
+// wait on global memory operations)
+__device__ __forceinline__ void drain() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+// Precedes each flag or counter that tells another wavefront it may read
+// data this wavefront published.  All such data is written with
+// device-scope atomics (stores carry the device scope bit, adds are
+// performed where every XCD sees them) and read with device-scope atomic
+// loads, so it is enough that those writes have completed: wait for every
+// outstanding vector memory operation of the wavefront.  (A device-scope
+// release fence would also write back the XCD's L2, which holds none of
+// this data and costs a writeback per item.)
+__device__ __forceinline__ void publish() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Look-back within one buffer's tickets [first, ...): like look_back, with
+// each status word tagged by the launch generation (an entry of another
+// generation reads as not yet published).
+__device__ uint64_t look_back_gen(uint64_t* status, uint64_t idx, uint64_t first, uint64_t agg, uint32_t gen,
+                                  int lane) {
+  const uint64_t G = (uint64_t)gen << kGenShift;
+  if (idx == first) {
+    if (lane == 0) atomic_st64(status + idx, kFlagPrefix | G | agg);
+    return 0;
+  }
+  if (lane == 0) atomic_st64(status + idx, kFlagAgg | G | agg);
+  const uint64_t excl = look_back_window<8>(status, (int64_t)idx - 1, (int64_t)first, kFlagPrefix | G, lane,
+                                            [gen](uint64_t v, uint32_t& f, uint64_t& val) {
+                                              f = ((v >> kGenShift) & kBatchGenMask) == gen ? (uint32_t)(v >> 62) : 0u;
+                                              val = v & kValue48;
+                                            });
+  if (lane == 0) atomic_st64(status + idx, kFlagPrefix | G | (excl + agg));
+  return excl;
+}
+
+// ---- histogram item -------------------------------------------------------
+// Counts bytes [c0, c1) of buffer B (plane = byte offset % P) into this
+// wave's LDS counters [P][256][REP] (REP = 8 / P replicas, lane l adds into
+// replica l % REP), then adds them to the buffer's accumulators.
+__device__ void hist_chunk(const BatchBuf& B, uint32_t b, uint64_t c0, uint64_t c1, uint32_t* cnt,
+                           uint32_t* ghist) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t P = B.eb;
+  const uint32_t rep = 8u / P;
+  const uint32_t r = lane % rep;
+  const uint4* p4 = (const uint4*)(B.in + c0);
+  const uint64_t n16 = (c1 - c0) / 16u;
+  auto count16 = [&](const uint4& v) {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const uint32_t plane = (uint32_t)(4 * k + m) % P;
+        const uint32_t byte = __builtin_amdgcn_ubfe(w[k], 8u * (uint32_t)m, 8u);
+        atomicAdd(cnt + (plane * 256u + byte) * rep + r, 1u);
+      }
+  };
+  uint64_t i = lane;
+  for (; i + 192 < n16; i += 256) {
+    const uint4 a0 = p4[i], a1 = p4[i + 64], a2 = p4[i + 128], a3 = p4[i + 192];
+    count16(a0);
+    count16(a1);
+    count16(a2);
+    count16(a3);
+  }
+  for (; i < n16; i += 64) count16(p4[i]);
+  const uint64_t tail = c0 + 16u * n16;
+  if (tail + lane < c1) {
+    const uint32_t plane = (uint32_t)((tail - c0 + lane) % P);  // c0 is a multiple of 16
+    atomicAdd(cnt + (plane * 256u + B.in[tail + lane]) * rep + r, 1u);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  for (uint32_t idx = lane; idx < P * 256u; idx += 64) {
+    uint32_t s = 0;
+    for (uint32_t q = 0; q < rep; ++q) {
+      s += cnt[idx * rep + q];
+      cnt[idx * rep + q] = 0;
+    }
+    if (s) atomicAdd(ghist + (uint64_t)b * 1024u + idx, s);
+  }
+}
+
+// ---- finisher: one wavefront normalises every plane of buffer b -----------
+template <int PB>
+__device__ void finish_buffer(const BatchBuf& B, uint32_t b, const BatchEncodeArgs& a, uint32_t gen, uint32_t* keys) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t P = B.eb;
+  bool bad = false;
+  for (uint32_t k = 0; k < P; ++k) {
+    uint32_t* h = a.hist + (uint64_t)b * 1024u + k * 256u;
+    uint32_t c[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      c[j] = atomic_ld32(h + 4 * lane + j);
+      atomic_st32(h + 4 * lane + j, 0u);
+    }
+    uint4 e[4];
+    normalize_wave(c, (uint32_t)B.n, PB, keys, e, bad);
+    uint64_t* t2 = (uint64_t*)(a.tables + ((uint64_t)b * 4u + k) * 256u);
+    uint16_t pr[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      // the encoder's form: {pdf << (31 - p), magic, shift | cdf << 8, 2^p - pdf}
+      const uint4 f = make_uint4(e[j].x << (31 - PB), e[j].z, e[j].w | (e[j].y << 8), (1u << PB) - e[j].x);
+      atomic_st64(t2 + 2 * (4 * lane + j), ((uint64_t)f.y << 32) | f.x);
+      atomic_st64(t2 + 2 * (4 * lane + j) + 1, ((uint64_t)f.w << 32) | f.z);
+      pr[j] = (uint16_t)e[j].x;
+    }
+    uint8_t* sec = B.out + B.section[k];
+    ((uint2*)(sec + kAnsHeaderBytes))[lane] =
+        make_uint2((uint32_t)pr[0] | ((uint32_t)pr[1] << 16), (uint32_t)pr[2] | ((uint32_t)pr[3] << 16));
+    if (lane < 8) {
+      uint32_t f = 0;
+      if (lane == 0) f = (kAnsMagic << 16) | 2u;  // native format version
+      if (lane == 1) f = (uint32_t)B.nb;
+      if (lane == 2) f = (uint32_t)B.n;
+      if (lane == 4) f = (uint32_t)PB;
+      // totalCompressedWords (field 3) is written by the buffer's last
+      // ticket to finish; no other store may touch its bytes before that
+      if (lane != 3 || B.nticks == 0) ((uint32_t*)sec)[lane] = f;
+    }
+    if ((B.nb & 1) && lane < 2) ((uint32_t*)(sec + kSectionFixed + 4ull * kL * B.nb + 8ull * B.nb))[lane] = 0;
+  }
+  // frame header; data_words / total_size (bytes 48..63) belong to the
+  // buffer's last ticket unless there is none
+  if (lane < 32 && (lane < 12 || lane >= 16 || B.nticks == 0)) ((uint32_t*)B.out)[lane] = B.frame_hdr[lane];
+  if (B.nticks == 0 && lane == 0) a.sizes[b] = B.data_off;
+  if (bad && lane == 0) atomicOr(a.err, kErrTable);
+  publish();
+  if (lane == 0) {
+    atomic_st32(a.done + b, 0u);
+    atomic_st32(a.ready + b, gen);
+  }
+}
+
+// ---- encode ticket --------------------------------------------------------
+// Two chains (c = 0, 1) over units u0, u0 + 1 of buffer B.  SAME: both units
+// are planes of the same element block (P >= 2), fed by one element stream.
+template <int PB, bool SAME, int KB>
+__device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
+                              uint32_t gen, uint32_t* wl, uint16_t* spill) {
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  static_assert(kBlock / kL % KB == 0, "KB steps per prefetch batch");
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t P = B.eb;
+  const uint64_t U = (uint64_t)P * B.nb;
+  const uint64_t u0 = 2ull * lt;
+  uint32_t x[2], cnt[2], n[2], fl[2], kp[2], soff[2];
+  uint64_t blk[2];
+  bool act[2];
+  const uint4* tab[2];
+  uint4* s_tab = (uint4*)wl;
+  uint16_t* wstage = (uint16_t*)(wl + kTabWords);
+  const uint32_t doff = 2u * kR + 2u * lane;  // dummy slot, from wstage
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const uint64_t u = u0 + c;
+    act[c] = u < U;
+    kp[c] = act[c] ? (uint32_t)(u % P) : 0u;
+    blk[c] = act[c] ? u / P : 0u;
+    const uint64_t start = blk[c] * kBlock;
+    n[c] = act[c] ? (uint32_t)((B.n - start) < kBlock ? (B.n - start) : kBlock) : 0u;
+    x[c] = kMinState;
+    cnt[c] = 0;
+    fl[c] = 0;
+    soff[c] = (uint32_t)c * kR;
+    tab[c] = s_tab + c * 256;
+    // tables of this chain's plane, published by the finisher
+    const uint64_t* g = (const uint64_t*)(a.tables + ((uint64_t)b * 4u + kp[c]) * 256u);
+    uint64_t* d = (uint64_t*)(s_tab + c * 256);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) d[lane + 64 * j] = atomic_ld64(g + lane + 64 * j);
+  }
+  const uint32_t n_min = wave_u32(n[0] < n[1] || !act[1] ? n[0] : n[1]);
+  const uint32_t n_max = wave_u32(n[0] > n[1] ? n[0] : n[1]);
+  const uint32_t all = (uint32_t)div_up(n_max, kL);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+
+  auto spill_full = [&]() {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (cnt[c] - fl[c] >= kH) {
+        const uint4 q = ((const uint4*)(wstage + soff[c] + (fl[c] & kMask)))[lane];
+        ((uint4*)(spill + c * CSTAGE + fl[c]))[lane] = q;
+        fl[c] += kH;
+      }
+    }
+  };
+
+  uint32_t s = 0;
+  {
+    // element streams: chain c reads byte kp[c] of element lane + 64 s of
+    // its block through the aligned dword holding it
+    const uint8_t* e0 = B.in + blk[0] * kBlock * P;
+    const uint8_t* e1 = B.in + blk[1] * kBlock * P;
+    const gptr<uint32_t> dp0 = (gptr<uint32_t>)e0 + ((lane * P) >> 2);
+    const gptr<uint32_t> dp1 = (gptr<uint32_t>)e1 + ((lane * P) >> 2);
+    const uint32_t sh0 = 8u * ((lane * P) & 3u) + 8u * kp[0];
+    const uint32_t sh1 = 8u * ((lane * P) & 3u) + 8u * kp[1];
+    const uint32_t dstep = kL * P / 4;  // dwords per step
+    const bool full = n_min == kBlock && act[1];
+    if (full) {
+      const uint32_t nbatch = kBlock / kL / KB;
+      const uint32_t last = (nbatch - 1) * KB;
+      uint32_t bA0[KB], bB0[KB], bA1[SAME ? 1 : KB], bB1[SAME ? 1 : KB];
+      auto load = [&](uint32_t (&b0)[KB], uint32_t (&b1)[SAME ? 1 : KB], uint32_t s0) {
+#pragma unroll
+        for (int j = 0; j < KB; ++j) b0[j] = dp0[(uint64_t)(s0 + j) * dstep];
+        if constexpr (!SAME) {
+#pragma unroll
+          for (int j = 0; j < KB; ++j) b1[j] = dp1[(uint64_t)(s0 + j) * dstep];
+        }
+      };
+      auto code = [&](const uint32_t (&b0)[KB], const uint32_t (&b1)[SAME ? 1 : KB]) {
+        auto sym = [&](int j, int c) -> uint32_t {
+          if constexpr (SAME) return __builtin_amdgcn_ubfe(b0[j], c ? sh1 : sh0, 8u);
+          else return c ? __builtin_amdgcn_ubfe(b1[j], sh1, 8u) : __builtin_amdgcn_ubfe(b0[j], sh0, 8u);
+        };
+        uint4 e[2];
+        e[0] = tab[0][sym(0, 0)];
+        e[1] = tab[1][sym(0, 1)];
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+          uint4 en[2];
+          if (j + 1 < KB) {
+            en[0] = tab[0][sym(j + 1, 0)];
+            en[1] = tab[1][sym(j + 1, 1)];
+          }
+#pragma unroll
+          for (int c = 0; c < 2; ++c) enc_step<kL, kMask>(x[c], cnt[c], wstage, soff[c], doff, e[c], true, 0u);
+          spill_full();
+          if (j + 1 < KB) {
+            e[0] = en[0];
+            e[1] = en[1];
+          }
+        }
+      };
+      load(bA0, bA1, 0);
+      for (;;) {
+        load(bB0, bB1, s + KB <= last ? s + KB : last);
+        code(bA0, bA1);
+        s += KB;
+        if (s > last) break;
+        load(bA0, bA1, s + KB <= last ? s + KB : last);
+        code(bB0, bB1);
+        s += KB;
+        if (s > last) break;
+      }
+    }
+    for (; s < all; ++s) {
+      const uint32_t i = s * kL + lane;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const bool v = i < n[c];
+        const uint8_t* eb = c ? e1 : e0;
+        const uint32_t sy = v ? (uint32_t)eb[(uint64_t)i * P + kp[c]] : 0u;
+        const uint4 e = tab[c][sy];
+        enc_step<kL, kMask>(x[c], cnt[c], wstage, soff[c], doff, e, v, 0u);
+      }
+      spill_full();
+    }
+  }
+
+  // pad, states, aggregate
+  uint32_t padded[2];
+  uint64_t agg = 0;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    padded[c] = (uint32_t)round_up(cnt[c], 8);
+    if (cnt[c] > CSTAGE && lane == 0) atomicOr(a.err, kErrStageOverflow);
+    if (lane < padded[c] - cnt[c]) wstage[soff[c] + ((cnt[c] + lane) & kMask)] = 0;
+    if (act[c]) ((uint32_t*)(B.out + B.section[kp[c]] + kSectionFixed))[blk[c] * kL + lane] = x[c];
+    agg += wave_u32(padded[c]);
+  }
+  const uint64_t excl = look_back_gen(a.status, tix, tix - lt, agg, gen, (int)lane);
+  drain();  // spilled words are read back by the lanes that wrote them
+  uint64_t y = excl;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (act[c] && lane == 0) {
+      uint8_t* sec = B.out + B.section[kp[c]];
+      uint32_t* bw = (uint32_t*)(sec + kSectionFixed + 4ull * kL * B.nb + 8ull * blk[c]);
+      bw[0] = (n[c] << 16) | cnt[c];
+      bw[1] = (uint32_t)y;
+      atomicAdd(a.ptot + 4ull * b + kp[c], padded[c]);
+    }
+    uint4* dst = (uint4*)(B.out + B.data_off + 2ull * y);
+    const uint4* ring4 = (const uint4*)(wstage + soff[c]);
+    const uint4* spill4 = (const uint4*)(spill + c * CSTAGE);
+    for (uint32_t j = lane; j < padded[c] / 8; j += kL) dst[j] = 8 * j < fl[c] ? spill4[j] : ring4[((8 * j) & kMask) / 8];
+    y += padded[c];
+  }
+  if (lt + 1 == B.nticks && lane == 0) {
+    const uint64_t total_words = excl + agg;
+    const uint64_t size = B.data_off + 2ull * total_words;
+    a.sizes[b] = size;
+    ((uint64_t*)B.out)[6] = total_words;  // data_words at byte 48
+    ((uint64_t*)B.out)[7] = size;         // total_size at byte 56
+  }
+  // the last ticket of the buffer to finish writes each plane's
+  // totalCompressedWords (section header field 3)
+  publish();
+  uint32_t f = 0;
+  if (lane == 0) f = atomicAdd(a.fin + b, 1u);
+  if (wave_u32(f) + 1 == B.nticks && lane < B.eb) {
+    const uint32_t v = atomic_ld32(a.ptot + 4ull * b + lane);
+    atomic_st32(a.ptot + 4ull * b + lane, 0u);
+    ((uint32_t*)(B.out + B.section[lane]))[3] = v;
+    if (lane == 0) atomic_st32(a.fin + b, 0u);
+  }
+}
+
+template <int PB, int NW, int KB>
+__global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a) {
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  extern __shared__ uint32_t lds[];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = wave_u32(threadIdx.x >> 6);
+  uint32_t* wl = lds + wave * kWaveWords;
+  uint16_t* const spill = a.spill + ((uint64_t)blockIdx.x * NW + wave) * (2ull * CSTAGE);
+  for (uint32_t i = lane; i < kTabWords; i += 64) wl[i] = 0;  // histogram counters start at zero
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  // this launch's generation (every wavefront reads it before the last one
+  // out advances it)
+  const uint32_t gen = wave_u32(lane == 0 ? atomic_ld32(a.queue + 2) : 0u);
+
+  uint32_t it = 0;
+  if (lane == 0) it = atomicAdd(a.queue, 1u);
+  it = wave_u32(it);
+  while (it < a.nitems) {
+    const uint32_t b = wave_u32(a.item_buf[it]);
+    const BatchBuf& B = a.bufs[b];
+    if (it < a.nhist_items) {
+      const uint64_t bytes = B.n * B.eb;
+      const uint64_t c0 = (uint64_t)(it - B.hist_first) * kBatchHistChunk;
+      const uint64_t c1 = c0 + kBatchHistChunk < bytes ? c0 + kBatchHistChunk : bytes;
+      if (c0 < c1) hist_chunk(B, b, c0, c1, wl, a.hist);
+      publish();
+      uint32_t d = 0;
+      if (lane == 0) d = atomicAdd(a.done + b, 1u);
+      if (wave_u32(d) + 1 == B.nhist) finish_buffer<PB>(B, b, a, gen, wl + kTabWords);
+    } else {
+      const uint32_t tix = it - a.nhist_items;
+      const uint32_t lt = tix - B.tick_first;
+      while (wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u) != gen) __builtin_amdgcn_s_sleep(2);
+      if (B.eb == 1) encode_ticket<PB, false, KB>(B, b, lt, tix, a, gen, wl, spill);
+      else encode_ticket<PB, true, KB>(B, b, lt, tix, a, gen, wl, spill);
+      // the histogram counters share this LDS: zero them again
+      for (uint32_t i = lane; i < kTabWords; i += 64) wl[i] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    // the next item only once this one is done (items start in order)
+    uint32_t nxt = 0;
+    if (lane == 0) nxt = atomicAdd(a.queue, 1u);
+    it = wave_u32(nxt);
+  }
+  // the last wavefront out resets the queue and advances the generation for
+  // the next launch (a replayed graph passes the same arguments every time)
+  if (lane == 0) {
+    const uint32_t total = gridDim.x * NW;
+    if (atomicAdd(a.queue + 1, 1u) + 1 == total) {
+      atomic_st32(a.queue, 0u);
+      atomic_st32(a.queue + 1, 0u);
+      atomic_st32(a.queue + 2, gen % kBatchGenMask + 1u);
+    }
+  }
+}
+
+template <int PB, int NW, int KB>
+hipError_t launch_enc_shape(const BatchEncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  constexpr size_t lds = (size_t)NW * kWaveWords * 4u;
+  static_assert(lds == batch_encode_lds(BatchShape{NW, KB}), "encode LDS layout");
+  if constexpr (lds > kLdsBytes) {
+    return hipErrorInvalidConfiguration;
+  } else {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_encode_kernel<PB, NW, KB>, 64 * NW, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    uint64_t grid = (uint64_t)cu * (uint64_t)per_cu;
+    const uint64_t want = div_up(a.nitems, NW);
+    if (want < grid) grid = want;
+    if (grid < 1) grid = 1;
+    const uint64_t spill_bytes = grid * NW * 2ull * CSTAGE * 2ull;
+    if (need_spill) {
+      *need_spill = spill_bytes;
+      return hipSuccess;
+    }
+    if (spill_bytes > a.spill_bytes) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+    return hipGetLastError();
+  }
+}
+
+template <int PB, int NW>
+hipError_t launch_enc_nw(const BatchEncodeArgs& a, int kb, int cu, hipStream_t s, uint64_t* need_spill) {
+  switch (kb) {
+    case 4: return launch_enc_shape<PB, NW, 4>(a, cu, s, need_spill);
+    case 8: return launch_enc_shape<PB, NW, 8>(a, cu, s, need_spill);
+    case 16: return launch_enc_shape<PB, NW, 16>(a, cu, s, need_spill);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int PB>
+hipError_t launch_enc(const BatchEncodeArgs& a, BatchShape sh, int cu, hipStream_t s, uint64_t* need_spill) {
+  switch (sh.nw) {
+    case 4: return launch_enc_nw<PB, 4>(a, sh.kb, cu, s, need_spill);
+    case 8: return launch_enc_nw<PB, 8>(a, sh.kb, cu, s, need_spill);
+    case 16: return launch_enc_nw<PB, 16>(a, sh.kb, cu, s, need_spill);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ---- batched decode --------------------------------------------------------
+// Decode ring of a chain for KB steps per ring upkeep: two halves of H
+// words plus a mirror of its first 64 words.
+//  * KB = 1: the half below the ring is fetched into registers once the
+//    upper half is being read (ptr <= lo + H) and written when the reads
+//    come within one step of the ring's bottom (ptr < lo + 64).
+//  * KB > 1: it is fetched as soon as the previous one is written, and
+//    written once the half it replaces is consumed (ptr <= lo + H); until
+//    then ptr > lo + H >= lo + 64 KB, so the KB steps to the next upkeep
+//    (at most 64 words each) read inside the ring.
+template <int KB>
+struct DecRing {
+  static constexpr uint32_t H = 64u * KB > 512u ? 64u * KB : 512u;
+  static constexpr uint32_t R = 2 * H, MASK = R - 1, WORDS = R + 64;
+  static_assert(64u * KB <= H, "KB steps must fit in a half");
+};
+
+// LUT item: check buffer b's frame against its plan and build the packed LUT
+// (slot - cdf) << 20 | pdf << 8 | sym of every plane into global memory.
+template <int PB>
+__device__ void build_luts(const BatchBuf& B, uint32_t b, const BatchDecodeArgs& a, uint32_t gen, uint32_t* wl) {
+  constexpr uint32_t W = 1u << PB;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t P = B.eb;
+  bool bad = B.in_bytes < B.data_off;
+  if (!bad) {
+    const uint32_t* fh = (const uint32_t*)B.in;
+    const uint8_t* fb = B.in;
+    // magic, version, element width, mode 0, lanes, precision, planes, order,
+    // block bytes, n, data offset and section offsets must match the plan
+    bad = fh[0] != kFrameMagic || fb[4] != kFrameVersion || fb[5] != 0 || fb[6] != P || fb[7] != 0 ||
+          fb[8] != kL || fb[9] != PB || fb[10] != P || fb[11] != 0 || fh[3] != kBlock ||
+          ((const uint64_t*)B.in)[2] != B.n || ((const uint64_t*)B.in)[5] != B.data_off;
+    for (uint32_t k = 0; k < P && !bad; ++k) bad = ((const uint64_t*)B.in)[8 + k] != B.section[k];
+    const uint64_t data_words = ((const uint64_t*)B.in)[6];
+    bad = bad || B.data_off + 2ull * data_words > B.in_bytes;
+  }
+  uint32_t* cdf = wl;  // 257 words
+  for (uint32_t k = 0; k < P && !bad; ++k) {
+    const uint8_t* sec = B.in + B.section[k];
+    const uint32_t* h = (const uint32_t*)sec;
+    if ((h[0] >> 16) != kAnsMagic || (h[0] & 0xffffu) != 2u || (h[4] & 0xfu) != (uint32_t)PB || h[1] != B.nb ||
+        h[2] != (uint32_t)B.n) {
+      bad = true;
+      break;
+    }
+    const uint16_t* probs = (const uint16_t*)(sec + kAnsHeaderBytes);
+    uint32_t v[4], ls = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[j] = probs[4 * lane + j];
+      ls += v[j];
+    }
+    uint32_t inc = ls;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = __shfl_up(inc, o, 64);
+      if (lane >= (uint32_t)o) inc += u;
+    }
+    uint32_t run = inc - ls;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      cdf[4 * lane + j] = run;
+      run += v[j];
+    }
+    if (lane == 63) cdf[256] = run;
+    drain();
+    if (B.n > 0 && cdf[256] != W) {
+      bad = true;
+      break;
+    }
+    uint32_t* g = a.luts + ((uint64_t)b * 4u + k) * W;
+    for (uint32_t slot = 2 * lane; slot < W; slot += 128) {
+      uint32_t ent[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const uint32_t sl = slot + q;
+        uint32_t lo = 0, hi = kNumSymbols;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (cdf[mid] <= sl) lo = mid; else hi = mid;
+        }
+        const uint32_t c = cdf[lo], pdf = cdf[lo + 1] - c;
+        ent[q] = ((sl - c) << 20) | (pdf << 8) | lo;
+      }
+      atomic_st64((uint64_t*)(g + slot), ((uint64_t)ent[1] << 32) | ent[0]);
+    }
+    drain();
+  }
+  bad = __ballot(bad) != 0;
+  if (bad && lane == 0) atomicOr(a.err, kErrMagic);
+  publish();
+  if (lane == 0) atomic_st32(a.ready + b, gen | (bad ? kBatchBad : 0u));
+}
+
+// Decode ticket: units u0, u0 + 1 of buffer B with the 64-lane ring decoder.
+// SAME: two planes of the same elements (stored together as 16 bits at byte
+// kp[0] of each element); otherwise two blocks of a uint8 buffer.
+template <int PB, bool SAME, int KB>
+__device__ void decode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, const BatchDecodeArgs& a, uint32_t* wl) {
+  constexpr uint32_t W = 1u << PB;
+  constexpr uint32_t kH = DecRing<KB>::H, kR = DecRing<KB>::R, kMask = DecRing<KB>::MASK;
+  constexpr uint32_t kDecRing = DecRing<KB>::WORDS;
+  constexpr uint32_t H4 = kH / 8;    // uint4 per half
+  constexpr int QH = (int)(H4 / 64);  // uint4 per lane per half
+  constexpr uint32_t UNIT_CAP = kL * lane_max_words(kBlock / kL, PB);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t P = B.eb;
+  const uint64_t U = (uint64_t)P * B.nb;
+  const uint64_t u0 = 2ull * lt;
+  const uint64_t data_words = ((const uint64_t*)B.in)[6];
+  uint32_t* s_lut = wl;                                   // [2][W]
+  uint16_t* wstage = (uint16_t*)(wl + 2 * W);             // [2][R + 64]
+  uint32_t x[2], ptr[2], n[2], wds[2], taken[2], lo[2], kp[2];
+  bool act[2], pend[2];
+  uint64_t blk[2];
+  uint4 rv[2][QH];
+  const uint4* src4[2];
+  const uint32_t* lut[2];
+  const uint16_t* stage[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const uint64_t u = u0 + c;
+    act[c] = u < U;
+    kp[c] = act[c] ? (uint32_t)(u % P) : 0u;
+    blk[c] = act[c] ? u / P : 0u;
+    const uint8_t* sec = B.in + B.section[kp[c]];
+    uint32_t nn = 0, words = 0, y = 0;
+    if (act[c]) {
+      const uint2 bw = *(const uint2*)(sec + kSectionFixed + 4ull * kL * B.nb + 8ull * blk[c]);
+      nn = bw.x >> 16;
+      words = bw.x & 0xffffu;
+      y = bw.y;
+      const uint64_t start = blk[c] * kBlock;
+      const uint64_t expect = (B.n - start) < kBlock ? (B.n - start) : kBlock;
+      if (nn != expect || words > UNIT_CAP || (uint64_t)y + round_up(words, 8) > data_words ||
+          B.data_off + 2ull * ((uint64_t)y + round_up(words, 8)) > B.in_bytes) {
+        if (lane == 0) atomicOr(a.err, kErrBlockWords);
+        act[c] = false;
+        nn = 0;
+        words = 0;
+      }
+    }
+    n[c] = nn;
+    words = wave_u32(words);
+    y = wave_u32(y);
+    // this plane's LUT
+    const uint64_t* g = (const uint64_t*)(a.luts + ((uint64_t)b * 4u + kp[c]) * W);
+    uint64_t* d = (uint64_t*)(s_lut + c * W);
+    for (uint32_t j = lane; j < W / 2; j += 64) d[j] = atomic_ld64(g + j);
+    lut[c] = s_lut + c * W;
+    src4[c] = (const uint4*)(B.in + B.data_off + 2ull * y);
+    uint16_t* reg = wstage + c * kDecRing;
+    stage[c] = reg;
+    lo[c] = 0;
+    pend[c] = false;
+#pragma unroll
+    for (int q = 0; q < QH; ++q) rv[c][q] = make_uint4(0, 0, 0, 0);
+    if (words > 0) {
+      const uint32_t jt = (words - 1) / kH;
+      const uint32_t j0 = jt > 0 ? jt - 1 : 0;
+      lo[c] = j0 * kH;
+      for (uint32_t j = j0; j <= jt; ++j) {
+        const uint32_t left = words - j * kH;
+        const uint32_t n4 = (uint32_t)div_up(left < kH ? left : kH, 8);
+        for (uint32_t i = lane; i < n4; i += 64) {
+          const uint4 q = src4[c][j * H4 + i];
+          ((uint4*)(reg + (j & 1) * kH))[i] = q;
+          if ((j & 1) == 0 && i < 8) ((uint4*)(reg + kR))[i] = q;
+        }
+      }
+    }
+    x[c] = act[c] ? ((const uint32_t*)(sec + kSectionFixed))[blk[c] * kL + lane] : kMinState;
+    ptr[c] = words;
+    wds[c] = words;
+    taken[c] = 0;
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) __asm__ volatile("" : "+v"(x[c]));
+  drain();
+
+  auto fetch = [&](int c) {
+    const uint32_t j = lo[c] / kH - 1;
+#pragma unroll
+    for (int q = 0; q < QH; ++q) rv[c][q] = src4[c][j * H4 + 64 * q + lane];
+    pend[c] = true;
+  };
+  auto commit = [&](int c) {
+    const uint32_t j = lo[c] / kH - 1;
+    uint16_t* reg = wstage + c * kDecRing;
+#pragma unroll
+    for (int q = 0; q < QH; ++q) ((uint4*)(reg + (j & 1) * kH))[64 * q + lane] = rv[c][q];
+    if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + kR))[lane] = rv[c][0];
+    lo[c] -= kH;
+    pend[c] = false;
+  };
+  auto refill = [&]() {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if constexpr (KB == 1) {
+        if (!pend[c] && lo[c] > 0 && ptr[c] <= lo[c] + kH) fetch(c);
+        if (pend[c] && ptr[c] < lo[c] + 64u) commit(c);
+      } else {
+        if (!pend[c] && lo[c] > 0) fetch(c);
+        if (pend[c] && ptr[c] <= lo[c] + kH) {
+          commit(c);
+          if (lo[c] > 0) fetch(c);
+        }
+      }
+    }
+  };
+
+  const uint32_t n_min = wave_u32(n[0] < n[1] || !act[1] ? n[0] : n[1]);
+  const uint32_t n_max = wave_u32(n[0] > n[1] ? n[0] : n[1]);
+  const uint32_t full = act[1] ? n_min / kL : 0u;
+  const uint32_t all = (uint32_t)div_up(n_max, kL);
+  // element outputs: SAME stores 16 bits at byte kp[0] of each element
+  uint8_t* o0 = B.out + blk[0] * kBlock * P + kp[0];
+  uint8_t* o1 = B.out + blk[1] * kBlock * P + kp[1];
+  auto store = [&](uint32_t s, uint32_t el, bool v0, bool v1) {
+    const uint64_t i = (uint64_t)s * kL + lane;
+    if constexpr (SAME) {
+      if (v0) *(__attribute__((address_space(1))) uint16_t*)(o0 + i * P) = (uint16_t)el;
+    } else {
+      if (v0) *(__attribute__((address_space(1))) uint8_t*)(o0 + i) = (uint8_t)el;
+      if (v1) *(__attribute__((address_space(1))) uint8_t*)(o1 + i) = (uint8_t)(el >> 8);
+    }
+  };
+  const bool vall[2] = {true, true};
+  if constexpr (KB == 1) {
+    for (uint32_t s = all; s > full;) {
+      --s;
+      const uint32_t i = s * kL + lane;
+      bool v[2] = {i < n[0], i < n[1]};
+      const uint32_t el = dec_steps<kL, PB, 2, kMask, uint32_t>(x, ptr, taken, lut, stage, v, 0u);
+      store(s, el, v[0], v[1]);
+      refill();
+    }
+    for (uint32_t s = full; s > 0;) {
+      --s;
+      const uint32_t el = dec_steps<kL, PB, 2, kMask, uint32_t>(x, ptr, taken, lut, stage, vall, 0u);
+      store(s, el, true, true);
+      refill();
+    }
+  } else {
+    // steps above `full` and full steps down to a multiple of KB one at a
+    // time (upkeep after each step s with s % KB == 0), then KB full steps
+    // per upkeep
+    uint32_t s = all;
+    while (s > full || s % KB != 0) {
+      --s;
+      const uint32_t i = s * kL + lane;
+      bool v[2] = {i < n[0], i < n[1]};
+      const uint32_t el = dec_steps<kL, PB, 2, kMask, uint32_t>(x, ptr, taken, lut, stage, v, 0u);
+      store(s, el, v[0], v[1]);
+      if (s % KB == 0) refill();
+    }
+    while (s > 0) {
+#pragma unroll
+      for (int j = 0; j < KB; ++j) {
+        --s;
+        const uint32_t el = dec_steps<kL, PB, 2, kMask, uint32_t>(x, ptr, taken, lut, stage, vall, 0u);
+        store(s, el, true, true);
+      }
+      refill();
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const bool bad = act[c] && (taken[c] != wds[c] || x[c] != kMinState);
+    if (__ballot(bad) && lane == 0) atomicOr(a.err, kErrFinalState);
+  }
+}
+
+template <int PB, int NW, int KB>
+__global__ __launch_bounds__(64 * NW) void batch_decode_kernel(BatchDecodeArgs a) {
+  constexpr uint32_t W = 1u << PB;
+  constexpr uint32_t WAVE_WORDS = 2 * W + DecRing<KB>::WORDS;  // two LUTs | two rings (u16)
+  extern __shared__ uint32_t lds[];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = wave_u32(threadIdx.x >> 6);
+  uint32_t* wl = lds + wave * WAVE_WORDS;
+  const uint32_t gen = wave_u32(lane == 0 ? atomic_ld32(a.queue + 2) : 0u);
+  uint32_t it = 0;
+  if (lane == 0) it = atomicAdd(a.queue, 1u);
+  it = wave_u32(it);
+  while (it < a.nitems) {
+    const uint32_t b = wave_u32(a.item_buf[it]);
+    const BatchBuf& B = a.bufs[b];
+    if (it < a.nbufs) {
+      build_luts<PB>(B, b, a, gen, wl);
+    } else {
+      uint32_t r = 0;
+      while (((r = wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u)) & ~kBatchBad) != gen)
+        __builtin_amdgcn_s_sleep(2);
+      if (!(r & kBatchBad)) {
+        const uint32_t lt = it - a.nbufs - B.tick_first;
+        if (B.eb == 1) decode_ticket<PB, false, KB>(B, b, lt, a, wl);
+        else decode_ticket<PB, true, KB>(B, b, lt, a, wl);
+      }
+    }
+    drain();
+    uint32_t nxt = 0;
+    if (lane == 0) nxt = atomicAdd(a.queue, 1u);
+    it = wave_u32(nxt);
+  }
+  if (lane == 0) {
+    const uint32_t total = gridDim.x * NW;
+    if (atomicAdd(a.queue + 1, 1u) + 1 == total) {
+      atomic_st32(a.queue, 0u);
+      atomic_st32(a.queue + 1, 0u);
+      atomic_st32(a.queue + 2, gen % kBatchGenMask + 1u);
+    }
+  }
+}
+
+template <int PB, int NW, int KB>
+hipError_t launch_dec_shape(const BatchDecodeArgs& a, int cu, hipStream_t s) {
+  constexpr size_t lds = (size_t)NW * (2 * (1u << PB) + DecRing<KB>::WORDS) * 4u;
+  static_assert(lds == batch_decode_lds(BatchShape{NW, KB}, PB), "decode LDS layout");
+  if constexpr (lds > kLdsBytes) {
+    return hipErrorInvalidConfiguration;
+  } else {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_decode_kernel<PB, NW, KB>, 64 * NW, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    uint64_t grid = (uint64_t)cu * (uint64_t)per_cu;
+    const uint64_t want = div_up(a.nitems, NW);
+    if (want < grid) grid = want;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL((batch_decode_kernel<PB, NW, KB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+    return hipGetLastError();
+  }
+}
+
+template <int PB, int NW>
+hipError_t launch_dec_nw(const BatchDecodeArgs& a, int kb, int cu, hipStream_t s) {
+  switch (kb) {
+    case 1: return launch_dec_shape<PB, NW, 1>(a, cu, s);
+    case 4: return launch_dec_shape<PB, NW, 4>(a, cu, s);
+    case 8: return launch_dec_shape<PB, NW, 8>(a, cu, s);
+    case 16: return launch_dec_shape<PB, NW, 16>(a, cu, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int PB>
+hipError_t launch_dec(const BatchDecodeArgs& a, BatchShape sh, int cu, hipStream_t s) {
+  switch (sh.nw) {
+    case 4: return launch_dec_nw<PB, 4>(a, sh.kb, cu, s);
+    case 8: return launch_dec_nw<PB, 8>(a, sh.kb, cu, s);
+    case 16: return launch_dec_nw<PB, 16>(a, sh.kb, cu, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+}  // namespace mans

@@ -4,9 +4,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/mans.h"
@@ -785,6 +790,10 @@ struct mans_batch {
   BatchBuf* h_bufs = nullptr; // pinned staging for asynchronous plans (or null)
   uint32_t* h_items = nullptr;
   uint64_t spill = 0;
+  // launch shape, and what the tuning cache keys it by
+  BatchShape shape = kBatchEncodeDefault;
+  uint64_t total_bytes = 0;
+  uint32_t width_mask = 0;
 };
 
 namespace {
@@ -925,6 +934,12 @@ mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h
     if (nitems > 0) HIP_TRY(hipMemcpy(b->item_buf, dst, nitems * 4, hipMemcpyHostToDevice));
   }
   b->pb = p->prob_bits;
+  b->total_bytes = 0;
+  b->width_mask = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    b->total_bytes += h_bufs[i].n_elems * (uint64_t)h_bufs[i].elem_bytes;
+    b->width_mask |= (uint32_t)h_bufs[i].elem_bytes;
+  }
   b->nbufs = (uint32_t)count;
   b->nhist = b->decode ? 0u : (uint32_t)nhist;
   b->nticks = (uint32_t)nticks;
@@ -955,6 +970,76 @@ BatchEncodeArgs batch_args(const mans_batch* b, uint64_t* d_sizes) {
   return a;
 }
 
+// ---- launch-shape tuning cache ---------------------------------------------
+// Process-wide map from a batch's key to its tuned shape, seeded once from
+// the file named by MANS_TUNE_CACHE (lines "<key> <nw> <kb>"; later lines
+// win) and appended to by mans_batch_tune.
+std::mutex g_tune_mu;
+std::map<std::string, BatchShape>* g_tune = nullptr;
+
+uint32_t log2_bucket(uint64_t v) {
+  uint32_t k = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++k;
+  }
+  return k;
+}
+
+std::string tune_key(const mans_batch* b) {
+  char buf[96];
+  snprintf(buf, sizeof(buf), "%s-p%d-n%u-b%u-w%u-cu%d", b->decode ? "dec" : "enc", b->pb, log2_bucket(b->nbufs),
+           log2_bucket(b->total_bytes), b->width_mask, b->ws->cu);
+  return buf;
+}
+
+std::map<std::string, BatchShape>& tune_cache() {  // g_tune_mu held
+  if (!g_tune) {
+    g_tune = new std::map<std::string, BatchShape>();
+    const char* path = getenv("MANS_TUNE_CACHE");
+    FILE* f = path && *path ? fopen(path, "r") : nullptr;
+    if (f) {
+      char key[96];
+      int nw = 0, kb = 0;
+      while (fscanf(f, "%95s %d %d", key, &nw, &kb) == 3) (*g_tune)[key] = BatchShape{nw, kb};
+      fclose(f);
+    }
+  }
+  return *g_tune;
+}
+
+BatchShape lookup_shape(const mans_batch* b) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto& m = tune_cache();
+  auto it = m.find(tune_key(b));
+  if (it != m.end() && batch_shape_ok(it->second, b->pb, b->decode)) return it->second;
+  return b->decode ? kBatchDecodeDefault : kBatchEncodeDefault;
+}
+
+void store_shape(const mans_batch* b, BatchShape sh) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  const std::string key = tune_key(b);
+  tune_cache()[key] = sh;
+  const char* path = getenv("MANS_TUNE_CACHE");
+  FILE* f = path && *path ? fopen(path, "a") : nullptr;
+  if (f) {
+    fprintf(f, "%s %d %d\n", key.c_str(), sh.nw, sh.kb);
+    fclose(f);
+  }
+}
+
+// Spill bytes an encode batch needs with shape sh, made available in the
+// workspace.
+mans_status batch_reserve_spill(mans_batch* b, BatchShape sh) {
+  if (b->decode || b->nitems == 0) return MANS_OK;
+  uint64_t need = 0;
+  if (launch_batch_encode(batch_args(b, nullptr), b->pb, sh, b->ws->cu, nullptr, &need) != hipSuccess)
+    return MANS_ERR_HIP;
+  mans_status st = ensure_spill(b->ws, need);
+  if (st == MANS_OK) b->spill = std::max(b->spill, need);
+  return st;
+}
+
 mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs, int32_t count,
                          mans_workspace* ws, bool decode) {
   if (!out) return MANS_ERR_INVALID_ARG;
@@ -973,9 +1058,9 @@ mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buff
   b->decode = decode;
   st = batch_alloc(b, (uint32_t)count, (decode ? (uint64_t)count : nhist) + nticks, nticks, false);
   if (st == MANS_OK) st = batch_plan(b, p, h_bufs, count, nullptr);
-  if (st == MANS_OK && !decode && b->nitems > 0) {
-    if (launch_batch_encode(batch_args(b, nullptr), b->pb, ws->cu, nullptr, &b->spill) != hipSuccess) st = MANS_ERR_HIP;
-    if (st == MANS_OK) st = ensure_spill(ws, b->spill);
+  if (st == MANS_OK) {
+    b->shape = lookup_shape(b);
+    st = batch_reserve_spill(b, b->shape);
   }
   if (st != MANS_OK) {
     batch_free(b);
@@ -1000,6 +1085,7 @@ mans_status batch_slot_create(mans_batch** out, const mans_params* p, bool decod
   b->ws = ws;
   b->pb = p->prob_bits;
   b->decode = decode;
+  b->shape = decode ? kBatchDecodeDefault : kBatchEncodeDefault;
   // units of cap_bytes in 8 KiB blocks, plus a partial block per plane and
   // buffer; tickets hold two units
   const uint64_t units = cap_bytes / 8192 + 4ull * cap_bufs;
@@ -1009,7 +1095,7 @@ mans_status batch_slot_create(mans_batch** out, const mans_params* p, bool decod
   if (st == MANS_OK && !decode) {
     BatchEncodeArgs a = batch_args(b, nullptr);
     a.nitems = 0x7fffffffu;
-    if (launch_batch_encode(a, b->pb, ws->cu, nullptr, &b->spill) != hipSuccess) st = MANS_ERR_HIP;
+    if (launch_batch_encode(a, b->pb, b->shape, ws->cu, nullptr, &b->spill) != hipSuccess) st = MANS_ERR_HIP;
     if (st == MANS_OK) st = ensure_spill(ws, b->spill);
   }
   if (st != MANS_OK) {
@@ -1042,7 +1128,7 @@ mans_status mans_batch_encode_run(mans_batch* b, uint64_t* d_sizes, void* stream
   hipStream_t s = (hipStream_t)stream;
   if (b->spill > ws->spill_cap) return MANS_ERR_INVALID_ARG;  // the workspace was replaced under the batch
   prof_mark(ws, s, "batch_encode");
-  HIP_TRY(launch_batch_encode(batch_args(b, d_sizes), b->pb, ws->cu, s, nullptr));
+  HIP_TRY(launch_batch_encode(batch_args(b, d_sizes), b->pb, b->shape, ws->cu, s, nullptr));
   prof_mark(ws, s, nullptr);
   return MANS_OK;
 }
@@ -1069,9 +1155,81 @@ mans_status mans_batch_decode_run(mans_batch* b, void* stream) {
   a.queue = b->queue;
   a.err = ws->counters + 3;
   prof_mark(ws, s, "batch_decode");
-  HIP_TRY(launch_batch_decode(a, b->pb, ws->cu, s));
+  HIP_TRY(launch_batch_decode(a, b->pb, b->shape, ws->cu, s));
   prof_mark(ws, s, nullptr);
   return MANS_OK;
+}
+
+mans_status mans_batch_get_shape(const mans_batch* b, mans_batch_shape* out) {
+  if (!b || !out) return MANS_ERR_INVALID_ARG;
+  *out = mans_batch_shape{64 * b->shape.nw, b->shape.kb, 64, 0};
+  return MANS_OK;
+}
+
+mans_status mans_batch_set_shape(mans_batch* b, const mans_batch_shape* sh) {
+  if (!b || !sh || sh->lanes != 64 || sh->reserved != 0 || sh->wg_threads % 64 != 0) return MANS_ERR_INVALID_ARG;
+  const BatchShape c{sh->wg_threads / 64, sh->syms_per_lane};
+  if (!batch_shape_ok(c, b->pb, b->decode)) return MANS_ERR_INVALID_ARG;
+  DeviceGuard g(b->ws->device);
+  const mans_status st = batch_reserve_spill(b, c);
+  if (st == MANS_OK) b->shape = c;
+  return st;
+}
+
+mans_status mans_batch_tune(mans_batch* b, void* stream, int32_t reps, mans_tune_entry* h_entries,
+                            int32_t max_entries, int32_t* n_entries) {
+  if (!b || reps < 1 || max_entries < 0 || (max_entries > 0 && !h_entries)) return MANS_ERR_INVALID_ARG;
+  // candidates: workgroup size x symbols per lane (decode adds 1: the ring
+  // looked after every step)
+  static const int kNws[3] = {4, 8, 16}, kEncKbs[3] = {4, 8, 16}, kDecKbs[4] = {1, 4, 8, 16};
+  const int nkb = b->decode ? 4 : 3;
+  const int* kbs = b->decode ? kDecKbs : kEncKbs;
+  const int ncand = 3 * nkb;
+  if (n_entries) *n_entries = ncand;
+  mans_workspace* ws = b->ws;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  const BatchShape keep = b->shape;
+  mans_status st = MANS_OK;
+  uint64_t* d_sizes = nullptr;
+  if (!b->decode) HIP_TRY(hipMalloc(&d_sizes, std::max<uint32_t>(1, b->nbufs) * 8ull));
+  // every candidate's spill first, so no run sees the workspace change
+  for (int i = 0; i < ncand && st == MANS_OK; ++i) {
+    const BatchShape c{kNws[i / nkb], kbs[i % nkb]};
+    if (batch_shape_ok(c, b->pb, b->decode)) st = batch_reserve_spill(b, c);
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == MANS_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = MANS_ERR_HIP;
+  double best = -1.0;
+  BatchShape pick = keep;
+  for (int i = 0; i < ncand && st == MANS_OK; ++i) {
+    const BatchShape c{kNws[i / nkb], kbs[i % nkb]};
+    const bool fits = batch_shape_ok(c, b->pb, b->decode);
+    double ms = fits && b->nitems == 0 ? 0.0 : -1.0;
+    if (fits && b->nitems > 0) {
+      b->shape = c;
+      auto run = [&]() { return b->decode ? mans_batch_decode_run(b, s) : mans_batch_encode_run(b, d_sizes, s); };
+      st = run();
+      if (st == MANS_OK && hipEventRecord(e0, s) != hipSuccess) st = MANS_ERR_HIP;
+      for (int32_t r = 0; r < reps && st == MANS_OK; ++r) st = run();
+      if (st == MANS_OK && hipEventRecord(e1, s) != hipSuccess) st = MANS_ERR_HIP;
+      float t = 0;
+      if (st == MANS_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+        st = MANS_ERR_HIP;
+      ms = (double)t / reps;
+      if (st == MANS_OK && (best < 0 || ms < best)) {
+        best = ms;
+        pick = c;
+      }
+    }
+    if (i < max_entries) h_entries[i] = mans_tune_entry{mans_batch_shape{64 * c.nw, c.kb, 64, 0}, ms};
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (d_sizes) (void)hipFree(d_sizes);
+  b->shape = st == MANS_OK ? pick : keep;
+  if (st == MANS_OK && best >= 0) store_shape(b, pick);
+  return st;
 }
 
 mans_status mans_batch_destroy(mans_batch* b) {

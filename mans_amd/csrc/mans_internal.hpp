@@ -257,9 +257,40 @@ constexpr uint32_t kBatchGenMask = 0x3fffu;     // 14-bit generations
 constexpr uint32_t kBatchHistChunk = 1u << 18;  // bytes per histogram item
 
 hipError_t launch_hist(const HistArgs& a, int cu, hipStream_t s);
+
+// Launch shape of the batched kernels; no choice changes an output bit.
+//   nw: wavefronts per workgroup (4, 8, 16: 256 / 512 / 1024 threads)
+//   kb: symbols per lane per batch.  Encode (4, 8, 16): the elements each
+//       lane loads ahead in registers; decode (1, 4, 8, 16): the steps
+//       between two upkeeps of the word ring (which holds
+//       2 x max(512, 64 kb) words per chain)
+struct BatchShape {
+  int nw;
+  int kb;
+};
+constexpr BatchShape kBatchEncodeDefault = {4, 16};
+constexpr BatchShape kBatchDecodeDefault = {4, 1};
+// LDS bytes per workgroup of a shape (encode: table | ring | dummies per
+// wave; decode: two LUTs | two rings per wave)
+constexpr uint64_t batch_encode_lds(BatchShape sh) { return (uint64_t)sh.nw * (2u * 256u * 4u + 1024u + 64u) * 4u; }
+constexpr uint64_t batch_decode_lds(BatchShape sh, int pb) {
+  return (uint64_t)sh.nw * (2ull * (1u << pb) * 4u +
+                            2ull * 2u * ((2u * (64u * sh.kb > 512u ? 64u * sh.kb : 512u)) + 64u));
+}
+constexpr uint64_t kLdsBytes = 160u << 10;
+bool batch_shape_ok(BatchShape sh, int pb, bool decode);
+
 // batched encode; with need_spill set, only reports the spill bytes needed
-hipError_t launch_batch_encode(const BatchEncodeArgs& a, int pb, int cu, hipStream_t s, uint64_t* need_spill);
-hipError_t launch_batch_decode(const BatchDecodeArgs& a, int pb, int cu, hipStream_t s);
+hipError_t launch_batch_encode(const BatchEncodeArgs& a, int pb, BatchShape sh, int cu, hipStream_t s,
+                               uint64_t* need_spill);
+hipError_t launch_batch_decode(const BatchDecodeArgs& a, int pb, BatchShape sh, int cu, hipStream_t s);
+// one translation unit per precision
+hipError_t launch_batch_encode_p9(const BatchEncodeArgs&, BatchShape, int, hipStream_t, uint64_t*);
+hipError_t launch_batch_encode_p10(const BatchEncodeArgs&, BatchShape, int, hipStream_t, uint64_t*);
+hipError_t launch_batch_encode_p11(const BatchEncodeArgs&, BatchShape, int, hipStream_t, uint64_t*);
+hipError_t launch_batch_decode_p9(const BatchDecodeArgs&, BatchShape, int, hipStream_t);
+hipError_t launch_batch_decode_p10(const BatchDecodeArgs&, BatchShape, int, hipStream_t);
+hipError_t launch_batch_decode_p11(const BatchDecodeArgs&, BatchShape, int, hipStream_t);
 hipError_t launch_encode(const EncodeArgs& a, int cu, hipStream_t s);
 // bytes of spill area launch_encode needs for a (0 for the 32-lane format)
 hipError_t encode_spill_bytes(const EncodeArgs& a, int cu, uint64_t* bytes);

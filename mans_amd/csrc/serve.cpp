@@ -327,31 +327,52 @@ mans_status mans_server_create(mans_server** out, const mans_params* p, uint64_t
   return MANS_OK;
 }
 
-mans_status mans_server_submit(mans_server* S, const mans_request* r, uint64_t* id) {
-  if (!S || !r || !id) return MANS_ERR_INVALID_ARG;
+// Checks one request and builds its record (not yet queued).
+static mans_status make_req(mans_server* S, const mans_request* r, Req** out) {
   if (r->kind != MANS_REQ_COMPRESS && r->kind != MANS_REQ_DECOMPRESS) return MANS_ERR_INVALID_ARG;
   if (r->elem_bytes != 1 && r->elem_bytes != 2 && r->elem_bytes != 4) return MANS_ERR_INVALID_ARG;
   if ((r->n_elems > 0 || r->kind == MANS_REQ_DECOMPRESS) && !r->d_in) return MANS_ERR_INVALID_ARG;
   if (((uintptr_t)r->d_in & 15u) != 0) return MANS_ERR_ALIGNMENT;
   if (r->n_elems > 0xffffffffull) return MANS_ERR_TOO_LARGE;
+  const uint64_t cap = out_bytes_for(S->p, *r);
+  if (cap > S->arena.total() || r->n_elems * (uint64_t)r->elem_bytes > kSlotBytes) return MANS_ERR_BUDGET;
   Req* q = new Req();
   q->r = *r;
-  q->out_cap = out_bytes_for(S->p, *r);
-  if (q->out_cap > S->arena.total() || r->n_elems * (uint64_t)r->elem_bytes > kSlotBytes) {
-    delete q;
-    return MANS_ERR_BUDGET;
+  q->out_cap = cap;
+  *out = q;
+  return MANS_OK;
+}
+
+mans_status mans_server_submit(mans_server* S, const mans_request* r, uint64_t* id) {
+  if (!S || !r || !id) return MANS_ERR_INVALID_ARG;
+  return mans_server_submit_many(S, r, 1, id);
+}
+
+mans_status mans_server_submit_many(mans_server* S, const mans_request* r, int32_t count, uint64_t* ids) {
+  if (!S || count < 0 || (count > 0 && (!r || !ids))) return MANS_ERR_INVALID_ARG;
+  std::vector<Req*> qs((size_t)count, nullptr);
+  for (int32_t i = 0; i < count; ++i) {
+    const mans_status st = make_req(S, r + i, &qs[(size_t)i]);
+    if (st != MANS_OK) {
+      for (Req* q : qs) delete q;
+      return st;
+    }
   }
-  q->t_submit = Clock::now();
+  const auto now = Clock::now();
   {
     std::lock_guard<std::mutex> g(S->mu);
     if (S->stop) {
-      delete q;
+      for (Req* q : qs) delete q;
       return MANS_ERR_INVALID_ARG;
     }
-    q->id = S->next_id++;
-    S->reqs[q->id] = q;
-    S->waiting.push_back(q);
-    *id = q->id;
+    for (int32_t i = 0; i < count; ++i) {
+      Req* q = qs[(size_t)i];
+      q->t_submit = now;
+      q->id = S->next_id++;
+      S->reqs[q->id] = q;
+      S->waiting.push_back(q);
+      ids[i] = q->id;
+    }
   }
   S->cv_work.notify_one();
   return MANS_OK;

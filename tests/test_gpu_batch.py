@@ -171,3 +171,96 @@ def test_batch_decode_flags_a_corrupt_frame(ws):
         assert (outs[i].cpu().numpy().view(xs[i].dtype) == xs[i]).all(), i
     dec.close()
     enc.close()
+
+
+SHAPES = [(64 * nw, kb) for nw in (4, 8, 16) for kb in (4, 8, 16)]
+DEC_SHAPES = [(64 * nw, kb) for nw in (4, 8, 16) for kb in (1, 4, 8, 16)]
+
+
+def lds_fits(p: int, wg: int, kb: int, decode: bool) -> bool:
+    nw = wg // 64
+    if decode:
+        h = max(512, 64 * kb)
+        return nw * (2 * (1 << p) * 4 + 4 * (2 * h + 64)) <= 160 << 10
+    return nw * (2 * 256 * 4 + 1024 + 64) * 4 <= 160 << 10
+
+
+@pytest.mark.parametrize("p", [9, 10, 11])
+def test_batch_every_shape_is_bit_identical(ws, p):
+    rng = np.random.default_rng(30 + p)
+    sizes = EDGE_SIZES + [int(v) for v in np.exp(rng.uniform(np.log(1 << 10), np.log(1 << 21), 24))]
+    xs, ts = make_batch(30 + p, sizes)
+    prm = mans_amd.params(p, 64)
+    enc = mans_amd.EncodeBatch(ts, prm, ws)
+    with pytest.raises(mans_amd.MansError):
+        enc.set_shape(256, 1)  # one symbol per lane is a decode shape only
+    enc.set_shape(256, 16)
+    ref_sizes = enc.run().cpu().tolist()
+    ws.check()
+    ref = [o[:n].clone() for o, n in zip(enc.outs, ref_sizes)]
+    dec = mans_amd.DecodeBatch(ref, ref_sizes, ts, prm, ws)
+    for dirn, b, shapes in (("encode", enc, SHAPES), ("decode", dec, DEC_SHAPES)):
+        for wg, kb in shapes:
+            fits = lds_fits(p, wg, kb, dirn == "decode")
+            if not fits:
+                with pytest.raises(mans_amd.MansError):
+                    b.set_shape(wg, kb)
+                continue
+            print(f"p={p} {dirn} shape wg={wg} kb={kb}", flush=True)
+            b.set_shape(wg, kb)
+            assert b.shape == (wg, kb, 64)
+            for o in b.outs:
+                o.zero_()
+            b.run()
+            torch.cuda.synchronize()
+            ws.check()
+            if dirn == "encode":
+                assert b.sizes.cpu().tolist()[:len(xs)] == ref_sizes
+                for i in range(len(xs)):
+                    assert torch.equal(b.outs[i][:ref_sizes[i]], ref[i]), (wg, kb, i)
+            else:
+                for i, x in enumerate(xs):
+                    got = b.outs[i][:x.size].cpu().numpy().view(x.dtype)
+                    assert (got == x).all(), (wg, kb, i, x.size, x.itemsize)
+    dec.close()
+    enc.close()
+
+
+def test_batch_tune_picks_the_fastest_and_caches_it(ws):
+    rng = np.random.default_rng(40)
+    sizes = [int(v) for v in np.exp(rng.uniform(np.log(4096), np.log(1 << 21), 96))]
+    xs, ts = make_batch(40, sizes)
+    prm = mans_amd.params(10, 64)
+    enc = mans_amd.EncodeBatch(ts, prm, ws)
+    rows = enc.tune(reps=2)
+    ws.check()
+    assert [(r["wg_threads"], r["syms_per_lane"]) for r in rows] == SHAPES
+    timed = [r for r in rows if r["ms"] >= 0]
+    assert timed and all(lds_fits(10, r["wg_threads"], r["syms_per_lane"], False) for r in timed)
+    best = min(timed, key=lambda r: r["ms"])
+    assert enc.shape == (best["wg_threads"], best["syms_per_lane"], 64)
+    sizes_h = enc.run().cpu().tolist()
+    ws.check()
+    for i, (x, t) in enumerate(zip(xs, ts)):
+        q = mans_amd.params(10, 64, x.itemsize, mans_amd.MODE_PLANES)
+        one, n = mans_amd.frame_encode(t, q, ws)
+        assert sizes_h[i] == n and torch.equal(enc.outs[i][:n], one[:n]), i
+    # a new batch with the same key starts from the tuned shape
+    again = mans_amd.EncodeBatch(ts, prm, ws)
+    assert again.shape == enc.shape
+    again.close()
+    dec = mans_amd.DecodeBatch(enc.outs, sizes_h, ts, prm, ws)
+    drows = dec.tune(reps=2)
+    ws.check()
+    assert [(r["wg_threads"], r["syms_per_lane"]) for r in drows] == DEC_SHAPES
+    dbest = min((r for r in drows if r["ms"] >= 0), key=lambda r: r["ms"])
+    assert dec.shape == (dbest["wg_threads"], dbest["syms_per_lane"], 64)
+    for o in dec.outs:
+        o.zero_()
+    dec.run()
+    torch.cuda.synchronize()
+    ws.check()
+    for i, x in enumerate(xs):
+        assert (dec.outs[i][:x.size].cpu().numpy().view(x.dtype) == x).all(), i
+    dec.close()
+    enc.close()

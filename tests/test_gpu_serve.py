@@ -72,6 +72,22 @@ def test_server_mix_equals_alone(ws):
         assert res is not None, "request did not finish"
         check_result(srv, res, kind, t, frame, size)
         srv.release(rid)
+    print(f"server: the same {len(reqs)} requests in one submit_many", flush=True)
+    batch = [srv.compress_request(t) if kind == "c" else srv.decompress_request(frame, size, t.numel(), eb)
+             for kind, t, frame, size, eb in reqs]
+    ids = srv.submit_many(batch)
+    assert len(set(ids)) == len(reqs)
+    for rid, (kind, t, frame, size, eb) in zip(ids, reqs):
+        res = srv.wait(rid, 60_000)
+        assert res is not None, "request did not finish"
+        check_result(srv, res, kind, t, frame, size)
+        srv.release(rid)
+    # one bad request in the list: the call fails and queues none of them
+    bad = list(batch[:3])
+    bad[1] = mans_amd.Request(mans_amd.REQ_COMPRESS, 3, batch[1].d_in, 0, 16)
+    with pytest.raises(mans_amd.MansError):
+        srv.submit_many(bad)
+    assert srv.submit_many([]) == []
     st = srv.stats()
     assert st["arena_used"] == 0 and st["waiting"] == 0
     srv.close()
