@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--serve-budget-gib", type=int, default=8, help="server memory budget")
     ap.add_argument("--dump", default=None,
                     help="write the generated codes to this file and exit (input of mans_profile)")
+    ap.add_argument("--configs", action="store_true",
+                    help="instead of the benchmark line, print one JSON line per input of the benchmark plan "
+                         "(c0, c1 at 512 MiB, c2a, c2b) and coding form")
     return ap.parse_args()
 
 
@@ -67,6 +70,8 @@ def main() -> int:
     import mans_amd
     from mans_amd import workloads
 
+    if args.configs:
+        return configs_bench(args)
     shape = tuple(int(v) for v in args.shape.split("x"))
     torch.cuda.set_device(0)
     t0 = time.time()
@@ -481,6 +486,153 @@ def served_bench(args):
         "bit_exact_vs_plain": ok,
         "gen_s": round(gen_s, 2),
     }
+
+
+def configs_bench(args) -> int:
+    """Every input of the benchmark plan, each coded three ways on the GPU:
+    native frames of byte planes, native frames with the ADM mapping (uint16
+    inputs inside its lossless domain), and the drop-in reference API
+    (mans_encode / mans_decode of the raw bytes as one 32-lane GpuANS stream,
+    which is what the reference's `compress` does with a file).  For each:
+    GPU encode / decode GB/s on uncompressed bytes (HIP events around the
+    calls, inputs resident in HBM, mean of `steps` after `warmup`), the
+    compression ratio, a bit-exact round trip, and the CPU port (oracle/) of
+    the same form on a sample of the same data with all host cores and with
+    one thread."""
+    import torch
+
+    import mans_amd
+    from mans_amd import workloads
+
+    import oracle
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:  # pragma: no cover
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+
+    def gpu(arr):
+        return torch.from_numpy(arr.view({1: np.int8, 2: np.int16, 4: np.int32}[arr.itemsize])).cuda()
+
+    cases = [
+        ("c0", "Laplace(0, 4) around 32768, uint16, 2,097,152 values, seed 0",
+         lambda: gpu(workloads.laplace_u16())),
+        ("c1", "Lorenzo codes of a 512x512x1024 sinusoid field, uint16, seed 1",
+         lambda: workloads.lorenzo_codes((512, 512, 1024))),
+        ("c2a", "Zipf(1.1) ranks over 2^20 words (high byte < 16), uint32, 2^27 values, seed 2",
+         lambda: gpu(workloads.zipf_u32(1 << 27))),
+        ("c2b", "detector frames 64 x 2048^2, round(N(1000, 8)) + 40 Poisson(0.05), uint16, seed 3",
+         lambda: gpu(workloads.detector_u16())),
+    ]
+    ws = mans_amd.Workspace(0)
+    stream = torch.cuda.current_stream()
+    for name, desc, make in cases:
+        t0 = time.time()
+        x = make()
+        torch.cuda.synchronize()
+        gen_s = time.time() - t0
+        n, eb = x.numel(), x.element_size()
+        nbytes = n * eb
+        x_np = x.cpu().numpy().view({1: np.uint8, 2: np.uint16, 4: np.uint32}[eb])
+        forms = [("frame planes, 64 lanes", "planes")]
+        if eb == 2:
+            forms.append(("frame ADM, 64 lanes", "adm"))
+        forms.append(("reference API stream (bytes), 32 lanes", "stream"))
+        for label, kind in forms:
+            row = {"config": name, "input": desc, "bytes": nbytes, "form": label}
+            if kind == "stream":
+                p = mans_amd.params(args.prob_bits, 32, 1, mans_amd.MODE_PLANES)
+                xb = x.view(torch.uint8)
+                ws.reserve(nbytes, p)
+                out = torch.empty(mans_amd.max_compressed_size(nbytes, p), dtype=torch.uint8, device="cuda")
+                back = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+                _, csize = mans_amd.encode(xb, p, ws, out=out)
+
+                def enc():
+                    mans_amd.encode(xb, p, ws, out=out, sync_size=False)
+
+                def dec():
+                    mans_amd.decode(out, nbytes, p, ws, out=back, c=csize)
+
+                def check():
+                    return bool(torch.equal(back, xb))
+            else:
+                mode = mans_amd.MODE_ADM if kind == "adm" else mans_amd.MODE_PLANES
+                p = mans_amd.params(args.prob_bits, 64, eb, mode)
+                ws.reserve(nbytes, p)
+                out = torch.empty(mans_amd.frame_max_size(n, p), dtype=torch.uint8, device="cuda")
+                back = torch.empty_like(x)
+                try:
+                    _, csize = mans_amd.frame_encode(x, p, ws, out=out)
+                except mans_amd.MansError as e:
+                    row["error"] = str(e)
+                    print(json.dumps(row), flush=True)
+                    continue
+                header = bytes(out[:128].cpu().numpy().tobytes())
+
+                def enc():
+                    mans_amd.frame_encode(x, p, ws, out=out, sync_size=False)
+
+                def dec():
+                    mans_amd.frame_decode(out, csize, x.dtype, n, ws, out=back, header=header)
+
+                def check():
+                    return bool(torch.equal(back, x))
+            times = {}
+            for fn_name, fn in (("encode", enc), ("decode", dec)):
+                for _ in range(args.warmup):
+                    fn()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                e0.record(stream)
+                for _ in range(args.steps):
+                    fn()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                times[fn_name] = e0.elapsed_time(e1) / args.steps
+            ok = check()
+            ws.check()
+            row.update({
+                "compressed_bytes": int(csize),
+                "ratio": round(nbytes / csize, 4),
+                "gpu_encode_GBps": round(nbytes / (times["encode"] * 1e-3) / 1e9, 1),
+                "gpu_decode_GBps": round(nbytes / (times["decode"] * 1e-3) / 1e9, 1),
+                "gpu_encode_ms": round(times["encode"], 4),
+                "gpu_decode_ms": round(times["decode"], 4),
+                "bit_exact": ok,
+            })
+            # CPU port of the same form on a sample of the same data
+            for threads, mib in ((cores, 32), (1, 8)):
+                m = min(n, (mib << 20) // eb)
+                sample = np.ascontiguousarray(x_np[:m])
+                t = time.perf_counter()
+                if kind == "stream":
+                    c = oracle.stream_encode(sample.view(np.uint8), args.prob_bits, oracle.FORMAT_REF, threads=threads)
+                    te = time.perf_counter() - t
+                    t = time.perf_counter()
+                    d = oracle.stream_decode(c, args.prob_bits, threads=threads)
+                    td = time.perf_counter() - t
+                    cok = bool((d == sample.view(np.uint8)).all())
+                else:
+                    mode = oracle.MODE_ADM if kind == "adm" else oracle.MODE_PLANES
+                    c = oracle.frame_encode(sample, mode, args.prob_bits, oracle.FORMAT_NATIVE, threads=threads)
+                    te = time.perf_counter() - t
+                    t = time.perf_counter()
+                    d = oracle.frame_decode(c, threads=threads)
+                    td = time.perf_counter() - t
+                    cok = bool((d.view(sample.dtype)[:m] == sample).all())
+                key = "cpu" if threads > 1 else "cpu_1t"
+                row[key] = {"threads": threads, "sample_bytes": int(sample.nbytes),
+                            "encode_GBps": round(sample.nbytes / te / 1e9, 4),
+                            "decode_GBps": round(sample.nbytes / td / 1e9, 4), "bit_exact": cok}
+            row["gen_s"] = round(gen_s, 2)
+            print(json.dumps(row), flush=True)
+            del out, back
+        del x
+        torch.cuda.empty_cache()
+    ws.close()
+    return 0
 
 
 def measured_traffic(kernel: str):

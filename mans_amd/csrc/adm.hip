@@ -10,21 +10,21 @@
 // 0xFF >> (L % 8)), lanes without elements stay zero (mappingcpuV1.cpp:146);
 // lane l's bytes go to 32 O_g + l M_g with O = exclusive prefix of M.
 //
-// Structure: one wavefront maps two groups (one per half-wave).  The lane
-// bit buffers live in LDS (ds_or per element).  O_g is resolved by decoupled
-// look-back over tickets (any number of groups; the reference's fused path is
-// limited to 1024 groups, mappingV15_fused_uint16.cu:851).  Unmap parses
-// each lane's bytes with clz over 32-bit words.
-#include "mans_internal.hpp"
+// Structure: a wavefront takes 16 groups per ticket, two at a time (one
+// per half-wave).  Pass A writes codes and centers and finds each group's
+// byte length; after the look-back, pass B rebuilds the bits (lane bit
+// buffers in LDS, ds_or per element) and writes them at their offsets.
+// O_g is resolved by decoupled look-back over tickets (any number of groups;
+// the reference's fused path is limited to 1024 groups,
+// mappingV15_fused_uint16.cu:851), the whole wave reading 512 predecessors
+// per round trip.  Unmap parses each lane's bytes with clz over 32-bit words.
+#include "codec_device.hpp"
 
 namespace mans {
 namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-constexpr uint64_t kFlagAgg = 1ull << 62;
-constexpr uint64_t kFlagPrefix = 2ull << 62;
-constexpr uint64_t kValueMask = (1ull << 62) - 1;
 
 __device__ __forceinline__ uint32_t half_sum(uint32_t v) {
 #pragma unroll
@@ -40,128 +40,161 @@ __device__ __forceinline__ uint32_t half_max(uint32_t v) {
   return v;
 }
 
+// Group pairs per ticket.  A ticket of 16 groups (8192 elements) keeps the
+// look-back chain short: the inclusive prefix moves back at most 512
+// tickets per round trip, so 2-group tickets (256K per 512 MiB) made the
+// scan, not the data, the bound.
+constexpr uint32_t kPairs = kAdmTicketGroups / 2;
+// Lane bit buffers are padded to 9 words so the 64 lanes' words of one
+// index fall in distinct LDS banks.
+constexpr uint32_t kLaneWords = kAdmLaneBytes / 4 + 1;
+
+// The 16 elements of one lane (half-wave = one group): loads them, and
+// returns the group's center (all lanes of the half-wave) and the count.
+__device__ __forceinline__ uint32_t load_lane(const AdmMapArgs& a, uint64_t g, uint32_t lin, uint32_t (&v)[kAdmChunk],
+                                              uint32_t& cnt, uint64_t& base) {
+  const bool gactive = g < a.gsize;
+  base = g * kAdmGroup + (uint64_t)lin * kAdmChunk;
+  cnt = gactive && base < a.n ? (uint32_t)((a.n - base) < kAdmChunk ? (a.n - base) : kAdmChunk) : 0u;
+  if (cnt == kAdmChunk) {
+    const uint4* p = (const uint4*)(a.in + base);
+    const uint4 q0 = p[0], q1 = p[1];
+    const uint32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      v[2 * i] = w[i] & 0xffffu;
+      v[2 * i + 1] = w[i] >> 16;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < (int)kAdmChunk; ++i) v[i] = (uint32_t)i < cnt ? (uint32_t)a.in[base + i] : 0u;
+  }
+  uint32_t lsum = 0;
+#pragma unroll
+  for (int i = 0; i < (int)kAdmChunk; ++i) lsum += (uint32_t)i < cnt ? v[i] : 0u;
+  const uint32_t gsum = half_sum(lsum), gcnt = half_sum(cnt);
+  return gcnt ? gsum / gcnt : 0u;  // mappingcpuV1.cpp:60-66
+}
+
+// Per element (mappingcpuV1.cpp:68-146): d = |x - c|, len = 1 if x == c else
+// ceil(d / 126), code = 1 if x == c else 2 (d - 126 (len - 1)) (+1 if x < c);
+// a one at bit `bo` (MSB first), then bo += len.  With `bits`, sets those
+// bits in the lane's LDS buffer; returns bo (the lane's bit length).
+template <bool PACK>
+__device__ __forceinline__ uint32_t map_lane(const uint32_t (&v)[kAdmChunk], uint32_t cnt, uint32_t c,
+                                             uint32_t (&codes)[4], uint32_t* bits) {
+  uint32_t bo = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) codes[i] = 0;
+#pragma unroll
+  for (int i = 0; i < (int)kAdmChunk; ++i) {
+    if ((uint32_t)i < cnt) {
+      const uint32_t x = v[i];
+      const uint32_t d = x > c ? x - c : c - x;
+      const uint32_t len = x == c ? 1u : (d + kAdmMod - 1) / kAdmMod;
+      const uint32_t code = x == c ? 1u : ((d - kAdmMod * (len - 1)) * 2u + (x > c ? 0u : 1u));
+      codes[i >> 2] |= (code & 0xffu) << (8 * (i & 3));
+      if (PACK && bo < 8 * kAdmLaneBytes) atomicOr(bits + (bo >> 5), 1u << (((bo >> 3) & 3u) * 8u + 7u - (bo & 7u)));
+      bo += len;
+    }
+  }
+  return bo;
+}
+
 __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
-  __shared__ uint32_t s_bits[kWaves][64][kAdmLaneBytes / 4];
+  __shared__ uint32_t s_bits[kWaves][64][kLaneWords];
+  __shared__ uint32_t s_m[kWaves][2 * kPairs];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t half = (uint32_t)lane >> 5, lin = (uint32_t)lane & 31u;
   uint32_t* mybits = s_bits[wave][lane];
-  uint8_t* wavebytes = (uint8_t*)s_bits[wave];
+  const uint8_t* wavebytes = (const uint8_t*)s_bits[wave];
+  uint32_t* wm = s_m[wave];
+  constexpr uint32_t kTicketGroups = 2 * kPairs;
 
   for (;;) {
     uint32_t tk = 0;
-    if (lane == 0) tk = atomicAdd(a.ticket, 2u);
+    if (lane == 0) tk = atomicAdd(a.ticket, kTicketGroups);
     tk = __builtin_amdgcn_readfirstlane(tk);
     if ((uint64_t)tk >= a.gsize) break;
-    const uint64_t g = (uint64_t)tk + half;
-    const bool gactive = g < a.gsize;
-    const uint64_t base = g * kAdmGroup + (uint64_t)lin * kAdmChunk;
-    const uint32_t cnt = gactive && base < a.n ? (uint32_t)((a.n - base) < kAdmChunk ? (a.n - base) : kAdmChunk) : 0u;
 
-    uint32_t v[kAdmChunk];
-    if (cnt == kAdmChunk) {
-      const uint4* p = (const uint4*)(a.in + base);
-      uint4 q0 = p[0], q1 = p[1];
-      uint32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-#pragma unroll
-      for (int i = 0; i < 8; ++i) { v[2 * i] = w[i] & 0xffffu; v[2 * i + 1] = w[i] >> 16; }
-    } else {
-#pragma unroll
-      for (int i = 0; i < (int)kAdmChunk; ++i) v[i] = (uint32_t)i < cnt ? (uint32_t)a.in[base + i] : 0u;
-    }
-    uint32_t lsum = 0;
-#pragma unroll
-    for (int i = 0; i < (int)kAdmChunk; ++i) lsum += (uint32_t)i < cnt ? v[i] : 0u;
-    const uint32_t gsum = half_sum(lsum), gcnt = half_sum(cnt);
-    const uint32_t c = gcnt ? gsum / gcnt : 0u;  // mappingcpuV1.cpp:60-66
-
-#pragma unroll
-    for (int i = 0; i < (int)(kAdmLaneBytes / 4); ++i) mybits[i] = 0;
-    __threadfence_block();
-    uint32_t codes[4] = {0, 0, 0, 0};
-    uint32_t bo = 0;
+    // ---- phase A: codes, centers and each group's byte length M_g --------
     bool over = false;
-#pragma unroll
-    for (int i = 0; i < (int)kAdmChunk; ++i) {
-      if ((uint32_t)i < cnt) {
-        const uint32_t x = v[i];
-        const uint32_t d = x > c ? x - c : c - x;
-        const uint32_t len = x == c ? 1u : (d + kAdmMod - 1) / kAdmMod;
-        const uint32_t code = x == c ? 1u : ((d - kAdmMod * (len - 1)) * 2u + (x > c ? 0u : 1u));
-        codes[i >> 2] |= (code & 0xffu) << (8 * (i & 3));
-        if (bo < 8 * kAdmLaneBytes) atomicOr(mybits + (bo >> 5), 1u << (((bo >> 3) & 3u) * 8u + 7u - (bo & 7u)));
-        else over = true;
-        bo += len;
-      }
-    }
-    if (bo > 8 * kAdmLaneBytes) over = true;
-    const uint32_t lbytes = (bo + 7u) / 8u;
-    uint32_t m = half_max(cnt ? lbytes : 0u);
-    if (m > kAdmLaneBytes) m = kAdmLaneBytes;
-    if (cnt && bo < 8u * m) {
-      const uint32_t pad = (bo & 7u) == 0 ? 0xffu : (0xffu >> (bo & 7u));
-      atomicOr(mybits + (bo >> 5), pad << (((bo >> 3) & 3u) * 8u));
-    }
-    if (__ballot(over) && lin == 0) atomicOr(a.err, kErrAdmDomain);
-    // codes: 16 bytes per lane
-    if (cnt == kAdmChunk) {
-      *(uint4*)(a.codes + base) = make_uint4(codes[0], codes[1], codes[2], codes[3]);
-    } else {
-      for (uint32_t i = 0; i < cnt; ++i) a.codes[base + i] = (uint8_t)(codes[i >> 2] >> (8 * (i & 3)));
-    }
-
-    // ---- look-back for O_g (units of bytes per lane) ----------------------
-    const uint32_t m_lo = __builtin_amdgcn_readlane(m, 0), m_hi = __builtin_amdgcn_readlane(m, 32);
-    const uint64_t agg = (uint64_t)m_lo + m_hi;
-    const uint64_t chain = tk / 2u;
-    uint64_t excl = 0;
-    if (lane == 0) {
-      uint64_t* st = a.status + chain;
-      if (chain == 0) {
-        __hip_atomic_store(st, kFlagPrefix | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t pr = 0; pr < kPairs; ++pr) {
+      const uint64_t g = (uint64_t)tk + 2 * pr + half;
+      uint32_t v[kAdmChunk], cnt, codes[4];
+      uint64_t base;
+      const uint32_t c = load_lane(a, g, lin, v, cnt, base);
+      const uint32_t bo = map_lane<false>(v, cnt, c, codes, nullptr);
+      if (bo > 8 * kAdmLaneBytes) over = true;
+      uint32_t m = half_max(cnt ? (bo + 7u) / 8u : 0u);
+      if (m > kAdmLaneBytes) m = kAdmLaneBytes;
+      if (cnt == kAdmChunk) {
+        *(uint4*)(a.codes + base) = make_uint4(codes[0], codes[1], codes[2], codes[3]);
       } else {
-        __hip_atomic_store(st, kFlagAgg | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint64_t j = chain - 1;
-        for (;;) {
-          const uint64_t sv = __hip_atomic_load(a.status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const uint64_t f = sv >> 62;
-          if (f == 0) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-          }
-          excl += sv & kValueMask;
-          if (f == 2) break;
-          --j;
-        }
-        __hip_atomic_store(st, kFlagPrefix | (excl + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (uint32_t i = 0; i < cnt; ++i) a.codes[base + i] = (uint8_t)(codes[i >> 2] >> (8 * (i & 3)));
       }
+      if (g < a.gsize && lin == 0) a.centers[g] = (uint16_t)c;
+      if (lin == 0) wm[2 * pr + half] = m;
     }
+    if (__ballot(over) && lane == 0) atomicOr(a.err, kErrAdmDomain);
+    __threadfence_block();
+    // the ticket's groups: exclusive byte offsets within it, and their sum
+    uint32_t mine = (uint32_t)lane < kTicketGroups ? wm[lane] : 0u;
+    uint32_t incl = mine;
+#pragma unroll
+    for (int o = 1; o < (int)kTicketGroups; o <<= 1) {
+      const uint32_t u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    const uint64_t agg = __builtin_amdgcn_readlane(incl, kTicketGroups - 1);
+
+    // ---- look-back for the ticket's first offset (bytes per lane) ---------
+    const uint64_t chain = tk / kTicketGroups;
+    uint64_t excl = look_back(a.status, chain, agg, lane);
     excl = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(excl >> 32)) << 32) |
            (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)excl);
-    const uint64_t og = excl + (half ? m_lo : 0u);
-    if (gactive && lin == 0) {
-      a.centers[g] = (uint16_t)c;
-      a.offsets[g] = (int32_t)og;
+    {
+      const uint64_t g = (uint64_t)tk + lane;
+      if ((uint32_t)lane < kTicketGroups && g < a.gsize) a.offsets[g] = (int32_t)(excl + incl - mine);
     }
-    if ((uint64_t)tk + 2u >= a.gsize && lane == 0) a.offsets[a.gsize] = (int32_t)(excl + agg);
+    if ((uint64_t)tk + kTicketGroups >= a.gsize && lane == 0) a.offsets[a.gsize] = (int32_t)(excl + agg);
+    const uint32_t ex_lo = incl - mine;  // lane g's exclusive offset within the ticket
 
-    // ---- bit plane: lane l's m bytes at 32 og + l m (contiguous 32 m) -----
-    __threadfence_block();
-    if (gactive && m > 0) {
-      const uint64_t dst0 = 32ull * og;
-      const uint32_t total = 32u * m;
-      const uint32_t inv = (65536u + m - 1u) / m;
-      const uint64_t valid_lanes = a.n > g * kAdmGroup ? div_up(a.n - g * kAdmGroup, kAdmChunk) : 0;
-      if (dst0 + total <= a.bits_capacity) {
-        for (uint32_t q = lin; q < total; q += 32u) {
-          const uint32_t l = (q * inv) >> 16, j = q - l * m;
-          const uint8_t byte = l < valid_lanes ? wavebytes[(half * 32u + l) * kAdmLaneBytes + j] : (uint8_t)0;
-          a.bits[dst0 + q] = byte;
-        }
-      } else if (lin == 0) {
-        atomicOr(a.err, kErrCapacity);
+    // ---- phase B: each pair's bit plane, lane l's m bytes at 32 og + l m ---
+    for (uint32_t pr = 0; pr < kPairs; ++pr) {
+      const uint64_t g = (uint64_t)tk + 2 * pr + half;
+      uint32_t v[kAdmChunk], cnt, codes[4];
+      uint64_t base;
+      const uint32_t c = load_lane(a, g, lin, v, cnt, base);
+#pragma unroll
+      for (int i = 0; i < (int)kLaneWords; ++i) mybits[i] = 0;
+      __threadfence_block();
+      const uint32_t bo = map_lane<true>(v, cnt, c, codes, mybits);
+      const uint32_t m = wm[2 * pr + half];
+      if (cnt && bo < 8u * m) {
+        const uint32_t pad = (bo & 7u) == 0 ? 0xffu : (0xffu >> (bo & 7u));
+        atomicOr(mybits + (bo >> 5), pad << (((bo >> 3) & 3u) * 8u));
       }
+      __threadfence_block();
+      const uint32_t og_lo = __shfl(ex_lo, (int)(2 * pr), 64), og_hi = __shfl(ex_lo, (int)(2 * pr + 1), 64);
+      const uint64_t og = excl + (half ? og_hi : og_lo);
+      if (g < a.gsize && m > 0) {
+        const uint64_t dst0 = 32ull * og;
+        const uint32_t total = 32u * m;
+        const uint32_t inv = (65536u + m - 1u) / m;
+        const uint64_t valid_lanes = a.n > g * kAdmGroup ? div_up(a.n - g * kAdmGroup, kAdmChunk) : 0;
+        if (dst0 + total <= a.bits_capacity) {
+          for (uint32_t q = lin; q < total; q += 32u) {
+            const uint32_t l = (q * inv) >> 16, j = q - l * m;
+            const uint8_t byte = l < valid_lanes ? wavebytes[(half * 32u + l) * (4u * kLaneWords) + j] : (uint8_t)0;
+            a.bits[dst0 + q] = byte;
+          }
+        } else if (lin == 0) {
+          atomicOr(a.err, kErrCapacity);
+        }
+      }
+      __threadfence_block();
     }
-    __threadfence_block();
   }
 }
 
@@ -249,7 +282,7 @@ hipError_t launch_adm_map(const AdmMapArgs& a, int cu, hipStream_t s) {
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adm_map_kernel, kThreads, 0);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
-  uint64_t grid = div_up(div_up(a.gsize, 2), kWaves);
+  uint64_t grid = div_up(div_up(a.gsize, kAdmTicketGroups), kWaves);
   const uint64_t cap = (uint64_t)cu * per_cu;
   if (grid > cap) grid = cap;
   hipLaunchKernelGGL(adm_map_kernel, dim3((uint32_t)grid), dim3(kThreads), 0, s, a);

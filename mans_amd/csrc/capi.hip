@@ -482,7 +482,7 @@ static mans_status adm_map_impl(const uint16_t* d_in, uint64_t n, int32_t* d_off
                                 uint8_t* d_codes, uint8_t* d_bits, uint64_t bits_capacity, uint64_t* h_bits_len,
                                 int32_t* h_in_domain, mans_workspace* ws, hipStream_t s) {
   const uint64_t gsize = div_up(n, kAdmGroup);
-  const uint64_t chain = div_up(gsize, 2);
+  const uint64_t chain = div_up(gsize, kAdmTicketGroups);
   mans_status st = ensure_status(ws, std::max<uint64_t>(chain, 1));
   if (st != MANS_OK) return st;
   HIP_TRY(launch_fill_u64(ws->status, chain, 0, s));
