@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--configs", action="store_true",
                     help="instead of the benchmark line, print one JSON line per input of the benchmark plan "
                          "(c0, c1 at 512 MiB, c2a, c2b) and coding form")
+    ap.add_argument("--only", default="c0,c1,c2a,c2b", help="--configs: inputs to run (comma-separated)")
+    ap.add_argument("--forms", default="planes,adm,stream", help="--configs: coding forms to run (comma-separated)")
     return ap.parse_args()
 
 
@@ -525,9 +527,12 @@ def configs_bench(args) -> int:
         ("c2b", "detector frames 64 x 2048^2, round(N(1000, 8)) + 40 Poisson(0.05), uint16, seed 3",
          lambda: gpu(workloads.detector_u16())),
     ]
+    only, want = set(args.only.split(",")), set(args.forms.split(","))
     ws = mans_amd.Workspace(0)
     stream = torch.cuda.current_stream()
     for name, desc, make in cases:
+        if name not in only:
+            continue
         t0 = time.time()
         x = make()
         torch.cuda.synchronize()
@@ -539,7 +544,7 @@ def configs_bench(args) -> int:
         if eb == 2:
             forms.append(("frame ADM, 64 lanes", "adm"))
         forms.append(("reference API stream (bytes), 32 lanes", "stream"))
-        for label, kind in forms:
+        for label, kind in [f for f in forms if f[1] in want]:
             row = {"config": name, "input": desc, "bytes": nbytes, "form": label}
             if kind == "stream":
                 p = mans_amd.params(args.prob_bits, 32, 1, mans_amd.MODE_PLANES)
