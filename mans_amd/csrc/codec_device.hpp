@@ -260,6 +260,32 @@ __device__ __forceinline__ uint32_t lut_next(uint32_t e, uint32_t x) {
 }
 __device__ __forceinline__ uint32_t lut_sym(uint2 e) { return e.x >> 24; }
 __device__ __forceinline__ uint32_t lut_sym(uint32_t e) { return e & 0xffu; }
+// The CH decoded symbols packed into an element (chain c's byte at bits
+// 8c..8c+7) with byte permutes: a uint2 entry's symbol is byte 3 of e.x, a
+// uint32_t entry's byte 0.  v_perm selector bytes 0-3 pick the second
+// operand's bytes, 4-7 the first's, 0x0c gives zero.
+__device__ __forceinline__ uint32_t sym_word(uint2 e) { return e.x; }
+__device__ __forceinline__ uint32_t sym_word(uint32_t e) { return e; }
+template <typename LE>
+constexpr uint32_t sym_byte() { return sizeof(LE) == 8 ? 3u : 0u; }
+template <int CH, typename LE>
+__device__ __forceinline__ uint32_t pack_syms(const LE (&e)[CH]) {
+  constexpr uint32_t b = sym_byte<LE>();
+  if constexpr (CH == 1) {
+    return lut_sym(e[0]);
+  } else if constexpr (CH == 2) {
+    return __builtin_amdgcn_perm(sym_word(e[1]), sym_word(e[0]), 0x0c0c0000u | ((4u + b) << 8) | b);
+  } else if constexpr (CH == 4) {
+    const uint32_t lo = __builtin_amdgcn_perm(sym_word(e[1]), sym_word(e[0]), 0x0c0c0000u | ((4u + b) << 8) | b);
+    const uint32_t hi = __builtin_amdgcn_perm(sym_word(e[3]), sym_word(e[2]), ((4u + b) << 24) | (b << 16) | 0x0c0cu);
+    return lo | hi;
+  } else {
+    uint32_t el = 0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) el |= lut_sym(e[c]) << (8 * c);
+    return el;
+  }
+}
 
 // CLAMP: ptr stops at 0 (whole-unit stages, where a read past the unit's
 // words must stay inside the stage); without it (rings, every read masked
@@ -309,13 +335,9 @@ __device__ __forceinline__ uint32_t dec_steps(uint32_t (&x)[CH], uint32_t (&ptr)
   // its `r` and puts a branch between the chains
 #pragma unroll
   for (int c = 0; c < CH; ++c) __asm__ volatile("" : "+v"(word[c]));
-  uint32_t el = 0;
 #pragma unroll
-  for (int c = 0; c < CH; ++c) {
-    x[c] = r[c] ? (x[c] << 16) | word[c] : x[c];
-    el |= lut_sym(e[c]) << (8 * c);
-  }
-  return el;
+  for (int c = 0; c < CH; ++c) x[c] = r[c] ? (x[c] << 16) | word[c] : x[c];
+  return pack_syms<CH>(e);
 }
 
 // Sum of a 32-bit value over the 64 lanes (every lane gets it).

@@ -367,9 +367,17 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
     }
     while (s > 0) {
       s -= RK;
+      // one address per block for the RK steps (the steps' stores take
+      // immediate offsets from it)
+      __attribute__((address_space(1))) T* gp[BPT];
 #pragma unroll
-      for (int k = (int)RK - 1; k >= 0; --k)
-        put(s + k, dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half), vall);
+      for (int d = 0; d < BPT; ++d) gp[d] = ep[d] + (uint64_t)s * L;
+#pragma unroll
+      for (int k = (int)RK - 1; k >= 0; --k) {
+        const uint32_t el = dec_steps<L, PB, CH, MASK, uint2, !RING>(x, ptr, taken, lut, stage, vall, half);
+#pragma unroll
+        for (int d = 0; d < BPT; ++d) gp[d][k * L] = (T)(el >> (8 * S * d));
+      }
       refill();
     }
     // a valid block uses every word and returns each lane to 2^15
