@@ -365,14 +365,16 @@ def served_bench(args):
     decompress of that buffer's frame) sent to the request server
     (mans_server_*), which admits them against its device arena and codes
     them in batched launches while the previous batch runs.  Measured on the
-    same requests three ways: plain (each request alone: one
-    mans_frame_encode / mans_frame_decode call, then a synchronise; its
-    latency is that call's), the server under a burst (all requests submitted
-    in one mans_server_submit_many call: aggregate throughput = request
-    bytes / makespan), and the server
-    under open-loop Poisson arrivals at 50 % and 80 % of the burst rate
-    (latency = submit to completion, queueing included, as the server reports
-    it).  Every served result is compared byte for byte with the plain one."""
+    same requests three ways:
+      * plain: each request alone (one mans_frame_encode / mans_frame_decode
+        call, then a synchronise; its latency is that call's);
+      * the server under a burst: all requests submitted in one
+        mans_server_submit_many call (aggregate throughput = request bytes /
+        makespan), and again submitted one request at a time;
+      * the server under open-loop Poisson arrivals at 50 % and 80 % of the
+        one-at-a-time burst rate, submitted one at a time (latency = submit
+        to completion, queueing included, as the server reports it).
+    Every served result is compared byte for byte with the plain one."""
     import torch
 
     import mans_amd
@@ -446,9 +448,19 @@ def served_bench(args):
         srv.wait(rid, 120_000)
     burst_s = time.perf_counter() - t0
     ok, lat_burst = collect(ids, True)
+    # the same burst submitted one request at a time (the open-loop phases'
+    # submission path); its rate is what they take fractions of
+    t0 = time.perf_counter()
+    ids = [submit(i) for i in range(len(xs))]
+    for rid in ids:
+        srv.wait(rid, 120_000)
+    burst_each_s = time.perf_counter() - t0
+    ok_each, _ = collect(ids, False)
+    ok = ok and ok_each
 
-    # open loop: Poisson arrivals (fixed seed) at a fraction of the burst rate
-    rate = len(xs) / burst_s
+    # open loop: Poisson arrivals (fixed seed) at a fraction of the
+    # one-at-a-time burst rate
+    rate = len(xs) / burst_each_s
     loads = {}
     for frac in (0.5, 0.8):
         gaps = np.random.default_rng(7).exponential(1.0 / (frac * rate), len(xs))
@@ -483,6 +495,8 @@ def served_bench(args):
         "speedup_vs_plain": round(plain_s / burst_s, 2),
         "served_burst_p50_ms": round(float(np.percentile(lat_burst, 50)), 3),
         "served_burst_p99_ms": round(float(np.percentile(lat_burst, 99)), 3),
+        "served_burst_each_s": round(burst_each_s, 4),
+        "served_each_GBps": round(total / burst_each_s / 1e9, 2),
         **loads,
         "server_batches": st["batches"],
         "bit_exact_vs_plain": ok,
