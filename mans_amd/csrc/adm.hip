@@ -18,6 +18,7 @@
 // the reference's fused path is limited to 1024 groups,
 // mappingV15_fused_uint16.cu:851), the whole wave reading 512 predecessors
 // per round trip.  Unmap parses each lane's bytes with clz over 32-bit words.
+#include "adm_device.hpp"
 #include "codec_device.hpp"
 
 namespace mans {
@@ -200,54 +201,12 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
 
 __global__ __launch_bounds__(kThreads) void adm_unmap_kernel(AdmUnmapArgs a) {
   __shared__ uint8_t s_sig[kThreads][kAdmChunk];
-  const int t = threadIdx.x, lane = t & 63;
-  const uint32_t lin = (uint32_t)lane & 31u;
-  uint8_t* sig = s_sig[t];
+  const int t = threadIdx.x;
   const uint64_t nlanes = (uint64_t)gridDim.x * kThreads;
   for (uint64_t li = (uint64_t)blockIdx.x * kThreads + t; li < a.gsize * kAdmLanes; li += nlanes) {
-    const uint64_t g = li / kAdmLanes;
     const uint64_t base = li * kAdmChunk;
     if (base >= a.n) continue;
     const uint32_t cnt = (uint32_t)((a.n - base) < kAdmChunk ? (a.n - base) : kAdmChunk);
-    const int32_t o0 = a.offsets[g], o1 = a.offsets[g + 1];
-    if (o0 < 0 || o1 < o0 || o1 - o0 > (int32_t)kAdmLaneBytes || 32ull * (uint64_t)o1 > a.bits_len) {
-      if (lin == 0) atomicOr(a.err, kErrBlockWords);
-      continue;
-    }
-    const uint32_t m = (uint32_t)(o1 - o0);
-    const uint8_t* src = a.bits + 32ull * (uint64_t)o0 + (uint64_t)lin * m;
-    uint32_t w[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      uint32_t v = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const uint32_t i = 4u * r + b;
-        v = (v << 8) | (i < m ? (uint32_t)src[i] : 0u);
-      }
-      w[r] = v;  // bit position 32r + j is bit (31 - j) of w[r]
-    }
-#pragma unroll
-    for (int i = 0; i < (int)kAdmChunk; ++i) sig[i] = 0;
-    // mappingcpuV1.cpp:233-243: a one advances the element index (stop at
-    // 16), zeros after the i-th one count into signal[i] (uint8)
-    int idx = -1;
-    uint32_t prev = 0;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      uint32_t v = w[r];
-      while (v != 0 && idx < (int)kAdmChunk) {
-        const uint32_t z = (uint32_t)__clz((int)v);
-        const uint32_t pos = 32u * r + z;
-        if (idx >= 0) sig[idx] = (uint8_t)(pos - prev - 1u);
-        ++idx;
-        prev = pos;
-        v &= ~(0x80000000u >> z);
-      }
-    }
-    if (idx >= 0 && idx < (int)kAdmChunk) sig[idx] = (uint8_t)(8u * m - prev - 1u);
-    const uint32_t c = a.centers[g];
-    uint32_t out[8];
     const uint8_t* cp = a.codes + base;
     uint32_t cw[4];
     if (cnt == kAdmChunk) {
@@ -257,20 +216,7 @@ __global__ __launch_bounds__(kThreads) void adm_unmap_kernel(AdmUnmapArgs a) {
       for (int i = 0; i < 4; ++i) cw[i] = 0;
       for (uint32_t i = 0; i < cnt; ++i) cw[i >> 2] |= (uint32_t)cp[i] << (8 * (i & 3));
     }
-#pragma unroll
-    for (int i = 0; i < (int)kAdmChunk; ++i) {
-      const uint32_t code = (cw[i >> 2] >> (8 * (i & 3))) & 0xffu;
-      const uint32_t d = (code >> 1) + kAdmMod * (uint32_t)sig[i];
-      const uint32_t x = (code & 1u) ? (c - d) & 0xffffu : (c + d) & 0xffffu;
-      if (i & 1) out[i >> 1] |= x << 16; else out[i >> 1] = x;
-    }
-    if (cnt == kAdmChunk) {
-      uint4* op = (uint4*)(a.out + base);
-      op[0] = make_uint4(out[0], out[1], out[2], out[3]);
-      op[1] = make_uint4(out[4], out[5], out[6], out[7]);
-    } else {
-      for (uint32_t i = 0; i < cnt; ++i) a.out[base + i] = (uint16_t)(out[i >> 1] >> (16 * (i & 1)));
-    }
+    adm_restore_lane(a, li, cnt, cw, s_sig[t]);
   }
 }
 

@@ -293,6 +293,39 @@ def test_frame_adm_matches_oracle(ws, lanes):
         assert (back == x).all()
 
 
+@pytest.mark.parametrize("p", [9, 11])
+def test_frame_adm_long_signals(ws, p):
+    # wide spread: |x - c| beyond 126 gives unary signals of 2-3 bits, and
+    # several 8192-element blocks end in a ragged group
+    rng = np.random.default_rng(60 + p)
+    for n in [5 * 8192 + 777, 16 * 8192]:
+        x = np.clip(32768 + np.round(rng.laplace(0, 90, n)), 0, 65535).astype(np.uint16)
+        print(f"frame adm long signals n={n} p={p}", flush=True)
+        out, size = mans_amd.frame_encode(u16_dev(x), mans_amd.params(p, 64, 2, mans_amd.MODE_ADM), ws)
+        ref = oracle.frame_encode(x, oracle.MODE_ADM, p, FMTS[64])
+        got = host(out[:size])
+        assert size == ref.size and (got == ref).all()
+        back = host(mans_amd.frame_decode(out, size, torch.int16, n, ws)).view(np.uint16)
+        ws.check()
+        assert (back == x).all()
+
+
+def test_frame_adm_rejects_bad_offsets(ws):
+    rng = np.random.default_rng(62)
+    n = 3 * 8192
+    x = np.clip(32768 + np.round(rng.laplace(0, 4, n)), 0, 65535).astype(np.uint16)
+    out, size = mans_amd.frame_encode(u16_dev(x), mans_amd.params(10, 64, 2, mans_amd.MODE_ADM), ws)
+    frame = host(out[:size]).copy()
+    meta = int(frame[32:40].view(np.uint64)[0])
+    offs = frame[meta:meta + 8].view(np.int32)
+    offs[1] = offs[0] + 1000  # a group longer than 32 bytes per lane
+    print("frame adm with a corrupt offset table", flush=True)
+    bad = torch.from_numpy(frame).cuda()
+    mans_amd.frame_decode(bad, size, torch.int16, n, ws)
+    with pytest.raises(mans_amd.MansError):
+        ws.check()
+
+
 def test_frame_auto_falls_back_outside_domain(ws):
     rng = np.random.default_rng(51)
     x = np.clip(32768 + np.round(rng.laplace(0, 4, 50_000)), 0, 65535).astype(np.uint16)
