@@ -198,9 +198,12 @@ __device__ void finish_buffer(const BatchBuf& B, uint32_t b, const BatchEncodeAr
 // ---- encode ticket --------------------------------------------------------
 // Two chains (c = 0, 1) over units u0, u0 + 1 of buffer B.  SAME: both units
 // are planes of the same element block (P >= 2), fed by one element stream.
+// tkey: which tables the wave's LDS holds ((buffer << 4) | (plane of chain 0
+// << 2) | plane of chain 1, ~0 for none); a ticket of the same buffer and
+// planes reuses them.
 template <int PB, bool SAME, int KB>
 __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
-                              uint32_t gen, uint32_t* wl, uint16_t* spill) {
+                              uint32_t gen, uint32_t* wl, uint16_t* spill, uint64_t& tkey) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
   static_assert(kBlock / kL % KB == 0, "KB steps per prefetch batch");
   const uint32_t lane = threadIdx.x & 63u;
@@ -227,11 +230,18 @@ __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64
     fl[c] = 0;
     soff[c] = (uint32_t)c * kR;
     tab[c] = s_tab + c * 256;
-    // tables of this chain's plane, published by the finisher
-    const uint64_t* g = (const uint64_t*)(a.tables + ((uint64_t)b * 4u + kp[c]) * 256u);
-    uint64_t* d = (uint64_t*)(s_tab + c * 256);
+  }
+  // tables of the chains' planes, published by the finisher
+  const uint64_t key = ((uint64_t)b << 4) | (kp[0] << 2) | kp[1];
+  if (key != tkey) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) d[lane + 64 * j] = atomic_ld64(g + lane + 64 * j);
+    for (int c = 0; c < 2; ++c) {
+      const uint64_t* g = (const uint64_t*)(a.tables + ((uint64_t)b * 4u + kp[c]) * 256u);
+      uint64_t* d = (uint64_t*)(s_tab + c * 256);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) d[lane + 64 * j] = atomic_ld64(g + lane + 64 * j);
+    }
+    tkey = key;
   }
   const uint32_t n_min = wave_u32(n[0] < n[1] || !act[1] ? n[0] : n[1]);
   const uint32_t n_max = wave_u32(n[0] > n[1] ? n[0] : n[1]);
@@ -385,6 +395,7 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
   // this launch's generation (every wavefront reads it before the last one
   // out advances it)
   const uint32_t gen = wave_u32(lane == 0 ? atomic_ld32(a.queue + 2) : 0u);
+  uint64_t tkey = ~0ull;  // tables held in this wave's LDS
 
   uint32_t it = 0;
   if (lane == 0) it = atomicAdd(a.queue, 1u);
@@ -393,6 +404,7 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
     const uint32_t b = wave_u32(a.item_buf[it]);
     const BatchBuf& B = a.bufs[b];
     if (it < a.nhist_items) {
+      tkey = ~0ull;
       const uint64_t bytes = B.n * B.eb;
       const uint64_t c0 = (uint64_t)(it - B.hist_first) * kBatchHistChunk;
       const uint64_t c1 = c0 + kBatchHistChunk < bytes ? c0 + kBatchHistChunk : bytes;
@@ -405,10 +417,10 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
       const uint32_t tix = it - a.nhist_items;
       const uint32_t lt = tix - B.tick_first;
       while (wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u) != gen) __builtin_amdgcn_s_sleep(2);
-      if (B.eb == 1) encode_ticket<PB, false, KB>(B, b, lt, tix, a, gen, wl, spill);
-      else encode_ticket<PB, true, KB>(B, b, lt, tix, a, gen, wl, spill);
-      // the histogram counters share this LDS: zero them again
-      for (uint32_t i = lane; i < kTabWords; i += 64) wl[i] = 0;
+      // (the tables share LDS with the histogram counters; items start in
+      // queue order, so no histogram item follows a ticket in a wave)
+      if (B.eb == 1) encode_ticket<PB, false, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
+      else encode_ticket<PB, true, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     // the next item only once this one is done (items start in order)
@@ -573,8 +585,10 @@ __device__ void build_luts(const BatchBuf& B, uint32_t b, const BatchDecodeArgs&
 // Decode ticket: units u0, u0 + 1 of buffer B with the 64-lane ring decoder.
 // SAME: two planes of the same elements (stored together as 16 bits at byte
 // kp[0] of each element); otherwise two blocks of a uint8 buffer.
+// tkey: which LUTs the wave's LDS holds, as for encode_ticket.
 template <int PB, bool SAME, int KB>
-__device__ void decode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, const BatchDecodeArgs& a, uint32_t* wl) {
+__device__ void decode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, const BatchDecodeArgs& a, uint32_t* wl,
+                              uint64_t& tkey) {
   constexpr uint32_t W = 1u << PB;
   constexpr uint32_t kH = DecRing<KB>::H, kR = DecRing<KB>::R, kMask = DecRing<KB>::MASK;
   constexpr uint32_t kDecRing = DecRing<KB>::WORDS;
@@ -621,10 +635,6 @@ __device__ void decode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, const 
     n[c] = nn;
     words = wave_u32(words);
     y = wave_u32(y);
-    // this plane's LUT
-    const uint64_t* g = (const uint64_t*)(a.luts + ((uint64_t)b * 4u + kp[c]) * W);
-    uint64_t* d = (uint64_t*)(s_lut + c * W);
-    for (uint32_t j = lane; j < W / 2; j += 64) d[j] = atomic_ld64(g + j);
     lut[c] = s_lut + c * W;
     src4[c] = (const uint4*)(B.in + B.data_off + 2ull * y);
     uint16_t* reg = wstage + c * kDecRing;
@@ -651,6 +661,17 @@ __device__ void decode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, const 
     ptr[c] = words;
     wds[c] = words;
     taken[c] = 0;
+  }
+  // the chains' LUTs, built by the buffer's LUT item
+  const uint64_t key = ((uint64_t)b << 4) | (kp[0] << 2) | kp[1];
+  if (key != tkey) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const uint64_t* g = (const uint64_t*)(a.luts + ((uint64_t)b * 4u + kp[c]) * W);
+      uint64_t* d = (uint64_t*)(s_lut + c * W);
+      for (uint32_t j = lane; j < W / 2; j += 64) d[j] = atomic_ld64(g + j);
+    }
+    tkey = key;
   }
 #pragma unroll
   for (int c = 0; c < 2; ++c) __asm__ volatile("" : "+v"(x[c]));
@@ -758,6 +779,7 @@ __global__ __launch_bounds__(64 * NW) void batch_decode_kernel(BatchDecodeArgs a
   const uint32_t wave = wave_u32(threadIdx.x >> 6);
   uint32_t* wl = lds + wave * WAVE_WORDS;
   const uint32_t gen = wave_u32(lane == 0 ? atomic_ld32(a.queue + 2) : 0u);
+  uint64_t tkey = ~0ull;  // LUTs held in this wave's LDS
   uint32_t it = 0;
   if (lane == 0) it = atomicAdd(a.queue, 1u);
   it = wave_u32(it);
@@ -765,15 +787,16 @@ __global__ __launch_bounds__(64 * NW) void batch_decode_kernel(BatchDecodeArgs a
     const uint32_t b = wave_u32(a.item_buf[it]);
     const BatchBuf& B = a.bufs[b];
     if (it < a.nbufs) {
-      build_luts<PB>(B, b, a, gen, wl);
+      build_luts<PB>(B, b, a, gen, wl);  // uses the LUT area as scratch
+      tkey = ~0ull;
     } else {
       uint32_t r = 0;
       while (((r = wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u)) & ~kBatchBad) != gen)
         __builtin_amdgcn_s_sleep(2);
       if (!(r & kBatchBad)) {
         const uint32_t lt = it - a.nbufs - B.tick_first;
-        if (B.eb == 1) decode_ticket<PB, false, KB>(B, b, lt, a, wl);
-        else decode_ticket<PB, true, KB>(B, b, lt, a, wl);
+        if (B.eb == 1) decode_ticket<PB, false, KB>(B, b, lt, a, wl, tkey);
+        else decode_ticket<PB, true, KB>(B, b, lt, a, wl, tkey);
       }
     }
     drain();
