@@ -270,7 +270,7 @@ struct BatchShape {
   int kb;
 };
 constexpr BatchShape kBatchEncodeDefault = {4, 16};
-constexpr BatchShape kBatchDecodeDefault = {4, 1};
+constexpr BatchShape kBatchDecodeDefault = {4, 8};
 // LDS bytes per workgroup of a shape (encode: table | ring | dummies per
 // wave; decode: two LUTs | two rings per wave)
 constexpr uint64_t batch_encode_lds(BatchShape sh) { return (uint64_t)sh.nw * (2u * 256u * 4u + 1024u + 64u) * 4u; }
