@@ -264,3 +264,47 @@ def test_batch_tune_picks_the_fastest_and_caches_it(ws):
         assert (dec.outs[i][:x.size].cpu().numpy().view(x.dtype) == x).all(), i
     dec.close()
     enc.close()
+
+
+PROBE = r'''
+import sys
+sys.path.insert(0, sys.argv[2])
+import torch
+import mans_amd
+g = torch.Generator(device="cuda")
+g.manual_seed(5)
+ts = [torch.randint(0, 60, (n,), generator=g, device="cuda", dtype=torch.int16) for n in (30000, 70000, 250000)]
+ws = mans_amd.Workspace(0)
+enc = mans_amd.EncodeBatch(ts, mans_amd.params(10, 64), ws)
+if sys.argv[1] == "tune":
+    enc.tune(reps=1)
+enc.run()
+torch.cuda.synchronize()
+ws.check()
+print("shape", enc.shape, flush=True)
+'''
+
+
+def test_tune_cache_file(ws, tmp_path):
+    # a tuned shape goes to the MANS_TUNE_CACHE file; a new process creating
+    # a batch with the same key starts from the shape the file holds
+    import os
+    import subprocess
+    import sys
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cache = tmp_path / "tune.txt"
+    script = tmp_path / "probe.py"
+    script.write_text(PROBE)
+    env = dict(os.environ, MANS_TUNE_CACHE=str(cache))
+    print("tune cache: tuning process", flush=True)
+    r = subprocess.run([sys.executable, str(script), "tune", repo], env=env, capture_output=True, text=True,
+                       timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    fields = cache.read_text().split()
+    assert len(fields) == 3 and fields[0].startswith("enc-p10-"), fields
+    cache.write_text(f"{fields[0]} 8 4\n")  # 512 threads, 4 symbols per lane: not the default
+    print("tune cache: reading process", flush=True)
+    r = subprocess.run([sys.executable, str(script), "read", repo], env=env, capture_output=True, text=True,
+                       timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip().splitlines()[-1] == "shape (512, 4, 64)", r.stdout
