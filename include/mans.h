@@ -64,7 +64,11 @@ int32_t mans_abi_version(void);
 
 /* Creates a workspace on `device` (-1 = current device).  A workspace holds
  * the histogram / table / look-back scratch of one call at a time: calls
- * that overlap on the same workspace race; use one workspace per stream. */
+ * that overlap on the same workspace race; use one workspace per stream.
+ * Once a call has run on a stream under HIP graph capture, the workspace
+ * never frees or replaces its buffers (a graph may hold them): a later call
+ * or mans_workspace_reserve that would need larger ones returns
+ * MANS_ERR_CAPACITY, so reserve before capturing. */
 mans_status mans_workspace_create(mans_workspace** ws, int32_t device);
 mans_status mans_workspace_destroy(mans_workspace* ws);
 /* Pre-sizes the workspace for inputs of up to max_bytes (no allocation in
@@ -175,6 +179,11 @@ mans_status mans_batch_encode_run(mans_batch* b, uint64_t* d_sizes, void* stream
 mans_status mans_batch_decode_create(mans_batch** b, const mans_params* p, const mans_buffer* h_bufs,
                                      int32_t count, mans_workspace* ws);
 mans_status mans_batch_decode_run(mans_batch* b, void* stream);
+/* Error flags of each buffer of the batch's last decode run (0: decoded;
+ * else the buffer's frame was malformed and its output is undefined), for
+ * up to max buffers.  Synchronises the stream.  The reference reports no
+ * per-input errors (its decode asserts compile out, DeviceUtils.h:23-30). */
+mans_status mans_batch_decode_flags(const mans_batch* b, uint32_t* h_flags, int32_t max, void* stream);
 mans_status mans_batch_destroy(mans_batch* b);
 /* Launch shape of a batch's kernels.  No shape changes an output bit; the
  * reference's counterparts are compile-time constants (warp 32,

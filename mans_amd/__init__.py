@@ -85,6 +85,7 @@ SIGNATURES = {
     "mans_batch_encode_run": (C.c_int, [_vp, _vp, _vp]),
     "mans_batch_decode_create": (C.c_int, [C.POINTER(_vp), _P, _B, _i32, _vp]),
     "mans_batch_decode_run": (C.c_int, [_vp, _vp]),
+    "mans_batch_decode_flags": (C.c_int, [_vp, C.POINTER(_u32), _i32, _vp]),
     "mans_batch_destroy": (C.c_int, [_vp]),
     "mans_batch_get_shape": (C.c_int, [_vp, C.POINTER(BatchShape)]),
     "mans_batch_set_shape": (C.c_int, [_vp, C.POINTER(BatchShape)]),
@@ -403,6 +404,13 @@ class DecodeBatch(_Tunable):
     def run(self, stream=None):
         _check(lib().mans_batch_decode_run(self.handle, _stream(stream)), "mans_batch_decode_run")
         return [o[:n] for o, n in zip(self.outs, self.like_n)]
+
+    def flags(self, stream=None) -> list[int]:
+        """Per-buffer error flags of the last run (0: decoded)."""
+        n = len(self.frames)
+        arr = (C.c_uint32 * max(1, n))()
+        _check(lib().mans_batch_decode_flags(self.handle, arr, n, _stream(stream)), "mans_batch_decode_flags")
+        return [arr[i] for i in range(n)]
 
     def close(self) -> None:
         if getattr(self, "handle", None):

@@ -577,7 +577,10 @@ __device__ void build_luts(const BatchBuf& B, uint32_t b, const BatchDecodeArgs&
     drain();
   }
   bad = __ballot(bad) != 0;
-  if (bad && lane == 0) atomicOr(a.err, kErrMagic);
+  if (bad && lane == 0) {
+    atomicOr(a.err, kErrMagic);
+    atomicOr(a.bad + b, kErrMagic);
+  }
   publish();
   if (lane == 0) atomic_st32(a.ready + b, gen | (bad ? kBatchBad : 0u));
 }
@@ -626,7 +629,10 @@ __device__ void decode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, const 
       const uint64_t expect = (B.n - start) < kBlock ? (B.n - start) : kBlock;
       if (nn != expect || words > UNIT_CAP || (uint64_t)y + round_up(words, 8) > data_words ||
           B.data_off + 2ull * ((uint64_t)y + round_up(words, 8)) > B.in_bytes) {
-        if (lane == 0) atomicOr(a.err, kErrBlockWords);
+        if (lane == 0) {
+          atomicOr(a.err, kErrBlockWords);
+          atomicOr(a.bad + b, kErrBlockWords);
+        }
         act[c] = false;
         nn = 0;
         words = 0;
@@ -766,7 +772,10 @@ __device__ void decode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, const 
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const bool bad = act[c] && (taken[c] != wds[c] || x[c] != kMinState);
-    if (__ballot(bad) && lane == 0) atomicOr(a.err, kErrFinalState);
+    if (__ballot(bad) && lane == 0) {
+      atomicOr(a.err, kErrFinalState);
+      atomicOr(a.bad + b, kErrFinalState);
+    }
   }
 }
 

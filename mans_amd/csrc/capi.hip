@@ -34,6 +34,9 @@ struct mans_workspace {
   uint64_t spill_cap = 0;
   uint64_t* d_u64 = nullptr;     // [4]
   uint64_t* h_pinned = nullptr;  // [32] host pinned
+  // set once a call ran on a capturing stream: the buffers above may then
+  // be referenced by a graph, so they are never freed or replaced again
+  bool frozen = false;
   bool profiling = false;
   std::vector<hipEvent_t> ev;
   std::vector<const char*> ev_names;
@@ -68,8 +71,17 @@ mans_status check_params(const mans_params* p) {
   return MANS_OK;
 }
 
+// A call on a stream under capture freezes the workspace (see
+// mans_workspace::frozen): later calls that would need larger buffers fail
+// with MANS_ERR_CAPACITY instead of freeing memory a captured graph uses.
+void note_capture(mans_workspace* ws, hipStream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) ws->frozen = true;
+}
+
 mans_status ensure_status(mans_workspace* ws, uint64_t n) {
   if (n <= ws->status_cap) return MANS_OK;
+  if (ws->frozen) return MANS_ERR_CAPACITY;
   uint64_t cap = std::max<uint64_t>(n, ws->status_cap * 2);
   if (ws->status) (void)hipFree(ws->status);
   ws->status = nullptr;
@@ -82,6 +94,7 @@ mans_status ensure_status(mans_workspace* ws, uint64_t n) {
 
 mans_status ensure_scratch(mans_workspace* ws, uint64_t n) {
   if (n <= ws->scratch_cap) return MANS_OK;
+  if (ws->frozen) return MANS_ERR_CAPACITY;
   if (ws->scratch) (void)hipFree(ws->scratch);
   ws->scratch = nullptr;
   ws->scratch_cap = 0;
@@ -92,6 +105,7 @@ mans_status ensure_scratch(mans_workspace* ws, uint64_t n) {
 
 mans_status ensure_spill(mans_workspace* ws, uint64_t bytes) {
   if (bytes <= ws->spill_cap) return MANS_OK;
+  if (ws->frozen) return MANS_ERR_CAPACITY;
   if (ws->spill) (void)hipFree(ws->spill);
   ws->spill = nullptr;
   ws->spill_cap = 0;
@@ -377,6 +391,7 @@ mans_status mans_encode(const mans_params* p, const void* d_in, uint64_t n, void
   if (out_capacity < mans_max_compressed_size(n, p)) return MANS_ERR_CAPACITY;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
+  note_capture(ws, s);
   const Format f = format_for_lanes(p->lanes);
   CodecDesc d;
   memset(&d, 0, sizeof(d));
@@ -408,6 +423,7 @@ mans_status mans_decode(const mans_params* p, const void* d_in, uint64_t c, void
   if (c && c < kSectionFixed) return MANS_ERR_CORRUPT;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
+  note_capture(ws, s);
   DecodeArgs a;
   memset(&a, 0, sizeof(a));
   a.d.num_planes = 1;
@@ -435,6 +451,7 @@ mans_status mans_histogram(const void* d_in, uint64_t n, uint32_t stride, uint32
   if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
+  note_capture(ws, s);
   HistArgs h;
   memset(&h, 0, sizeof(h));
   h.src[0] = HistSource{(const uint8_t*)d_in, n * stride, stride, 0u};
@@ -460,6 +477,7 @@ mans_status mans_normalize(const uint32_t* d_hist, uint32_t total, int32_t prob_
   if (prob_bits < 9 || prob_bits > 11) return MANS_ERR_PRECISION;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
+  note_capture(ws, s);
   HIP_TRY(hipMemcpyAsync(ws->hist, d_hist, 256 * 4, hipMemcpyDeviceToDevice, s));
   HistArgs h;
   memset(&h, 0, sizeof(h));
@@ -525,6 +543,7 @@ mans_status mans_adm_map(const uint16_t* d_in, uint64_t n, int32_t* d_offsets, u
   if (!aligned(d_in, 16) || !aligned(d_codes, 16)) return MANS_ERR_ALIGNMENT;
   if (n > 0xffffffffull) return MANS_ERR_TOO_LARGE;
   DeviceGuard g(ws->device);
+  note_capture(ws, (hipStream_t)stream);
   mans_status st = adm_map_impl(d_in, n, d_offsets, d_centers, d_codes, d_bits, bits_capacity, h_bits_len,
                                 h_in_domain, ws, (hipStream_t)stream);
   prof_mark(ws, (hipStream_t)stream, nullptr);
@@ -537,6 +556,7 @@ mans_status mans_adm_unmap(const int32_t* d_offsets, const uint16_t* d_centers, 
   if (!aligned(d_out, 16) || !aligned(d_codes, 16)) return MANS_ERR_ALIGNMENT;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
+  note_capture(ws, s);
   AdmUnmapArgs a;
   memset(&a, 0, sizeof(a));
   a.offsets = d_offsets;
@@ -578,6 +598,7 @@ mans_status mans_frame_encode(const mans_params* p, const void* d_in, uint64_t n
   if (out_capacity < mans_frame_max_size(n, p)) return MANS_ERR_CAPACITY;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
+  note_capture(ws, s);
   const Format f = format_for_lanes(p->lanes);
   uint8_t* out = (uint8_t*)d_out;
   int mode = p->mode == MANS_MODE_AUTO ? (eb == 2 && aligned(d_in, 16) ? MANS_MODE_ADM : MANS_MODE_PLANES) : p->mode;
@@ -672,6 +693,7 @@ mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header
   if (!aligned(d_in, 16)) return MANS_ERR_ALIGNMENT;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
+  note_capture(ws, s);
   uint8_t hcopy[kFrameHeaderBytes];
   const uint8_t* hb = hcopy;
   if (h_header) {
@@ -789,7 +811,12 @@ struct mans_batch {
   uint32_t* luts = nullptr;   // decode: [cap_bufs][4][2^p]
   BatchBuf* h_bufs = nullptr; // pinned staging for asynchronous plans (or null)
   uint32_t* h_items = nullptr;
-  uint64_t spill = 0;
+  // the batch's own encoder spill areas, sized at create for every launch
+  // shape, so neither a shape change nor the workspace can move them under
+  // a captured run
+  uint16_t* spill_buf = nullptr;
+  uint64_t spill_cap = 0;
+  uint32_t* bad = nullptr;    // decode: [cap_bufs] error flags of the last run
   // launch shape, and what the tuning cache keys it by
   BatchShape shape = kBatchEncodeDefault;
   uint64_t total_bytes = 0;
@@ -800,7 +827,7 @@ namespace {
 
 void batch_free(mans_batch* b) {
   void* dev[] = {b->bufs, b->item_buf, b->hist, b->done, b->ready, b->tables,
-                 b->status, b->ptot, b->fin, b->queue, b->luts};
+                 b->status, b->ptot, b->fin, b->queue, b->luts, b->spill_buf, b->bad};
   for (void* q : dev)
     if (q) (void)hipFree(q);
   if (b->h_bufs) (void)hipHostFree(b->h_bufs);
@@ -882,6 +909,8 @@ mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, ui
   HIP_TRY(hipMemcpy(b->queue, q0, 12, hipMemcpyHostToDevice));
   if (b->decode) {
     HIP_TRY(hipMalloc(&b->luts, nb1 * 4 * (4ull << b->pb)));
+    HIP_TRY(hipMalloc(&b->bad, nb1 * 4));
+    HIP_TRY(hipMemset(b->bad, 0, nb1 * 4));
   } else {
     HIP_TRY(hipMalloc(&b->hist, nb1 * 4096));
     HIP_TRY(hipMalloc(&b->done, nb1 * 4));
@@ -965,8 +994,8 @@ BatchEncodeArgs batch_args(const mans_batch* b, uint64_t* d_sizes) {
   a.queue = b->queue;
   a.err = b->ws->counters + 3;
   a.sizes = d_sizes;
-  a.spill = b->ws->spill;
-  a.spill_bytes = b->ws->spill_cap;
+  a.spill = b->spill_buf;
+  a.spill_bytes = b->spill_cap;
   return a;
 }
 
@@ -1028,16 +1057,37 @@ void store_shape(const mans_batch* b, BatchShape sh) {
   }
 }
 
-// Spill bytes an encode batch needs with shape sh, made available in the
-// workspace.
+const int kTuneNws[3] = {4, 8, 16}, kTuneEncKbs[3] = {4, 8, 16}, kTuneDecKbs[4] = {1, 4, 8, 16};
+
+// Allocates an encode batch's spill areas once: the most any launch shape
+// needs for its plan (nitems_cap work items), or only `only` when given.
+mans_status batch_alloc_spill(mans_batch* b, uint32_t nitems_cap, const BatchShape* only) {
+  if (b->decode) return MANS_OK;
+  BatchEncodeArgs a = batch_args(b, nullptr);
+  a.nitems = nitems_cap;
+  uint64_t need = 0;
+  for (int i = 0; i < 9; ++i) {
+    const BatchShape c = only ? *only : BatchShape{kTuneNws[i / 3], kTuneEncKbs[i % 3]};
+    if (!batch_shape_ok(c, b->pb, false)) continue;
+    uint64_t v = 0;
+    if (launch_batch_encode(a, b->pb, c, b->ws->cu, nullptr, &v) != hipSuccess) return MANS_ERR_HIP;
+    need = std::max(need, v);
+    if (only) break;
+  }
+  if (need == 0) return MANS_OK;
+  HIP_TRY(hipMalloc(&b->spill_buf, need));
+  b->spill_cap = need;
+  return MANS_OK;
+}
+
+// An encode batch's spill areas cover shape sh (always true for a shape
+// that fits: they were sized for every shape at create).
 mans_status batch_reserve_spill(mans_batch* b, BatchShape sh) {
   if (b->decode || b->nitems == 0) return MANS_OK;
   uint64_t need = 0;
   if (launch_batch_encode(batch_args(b, nullptr), b->pb, sh, b->ws->cu, nullptr, &need) != hipSuccess)
     return MANS_ERR_HIP;
-  mans_status st = ensure_spill(b->ws, need);
-  if (st == MANS_OK) b->spill = std::max(b->spill, need);
-  return st;
+  return need <= b->spill_cap ? MANS_OK : MANS_ERR_INVALID_ARG;
 }
 
 mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs, int32_t count,
@@ -1058,6 +1108,7 @@ mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buff
   b->decode = decode;
   st = batch_alloc(b, (uint32_t)count, (decode ? (uint64_t)count : nhist) + nticks, nticks, false);
   if (st == MANS_OK) st = batch_plan(b, p, h_bufs, count, nullptr);
+  if (st == MANS_OK && b->nitems > 0) st = batch_alloc_spill(b, b->nitems, nullptr);
   if (st == MANS_OK) {
     b->shape = lookup_shape(b);
     st = batch_reserve_spill(b, b->shape);
@@ -1092,12 +1143,7 @@ mans_status batch_slot_create(mans_batch** out, const mans_params* p, bool decod
   const uint64_t ticks = units / 2 + cap_bufs;
   const uint64_t hist = cap_bytes / kBatchHistChunk + cap_bufs;
   st = batch_alloc(b, cap_bufs, (decode ? cap_bufs : hist) + ticks, ticks, true);
-  if (st == MANS_OK && !decode) {
-    BatchEncodeArgs a = batch_args(b, nullptr);
-    a.nitems = 0x7fffffffu;
-    if (launch_batch_encode(a, b->pb, b->shape, ws->cu, nullptr, &b->spill) != hipSuccess) st = MANS_ERR_HIP;
-    if (st == MANS_OK) st = ensure_spill(ws, b->spill);
-  }
+  if (st == MANS_OK && !decode) st = batch_alloc_spill(b, 0x7fffffffu, &b->shape);
   if (st != MANS_OK) {
     batch_free(b);
     return st;
@@ -1113,6 +1159,11 @@ mans_status batch_slot_plan(mans_batch* b, const mans_params* p, const mans_buff
   return batch_plan(b, p, bufs, count, s);
 }
 
+// Device array of a decode batch's per-buffer error flags, and the bits of
+// the workspace's sticky error word that only decode launches raise.
+const uint32_t* batch_slot_bad(const mans_batch* b) { return b ? b->bad : nullptr; }
+uint32_t batch_decode_error_bits() { return kErrMagic | kErrBlockWords | kErrFinalState; }
+
 extern "C" {
 
 mans_status mans_batch_encode_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs,
@@ -1126,7 +1177,6 @@ mans_status mans_batch_encode_run(mans_batch* b, uint64_t* d_sizes, void* stream
   mans_workspace* ws = b->ws;
   DeviceGuard g(ws->device);
   hipStream_t s = (hipStream_t)stream;
-  if (b->spill > ws->spill_cap) return MANS_ERR_INVALID_ARG;  // the workspace was replaced under the batch
   prof_mark(ws, s, "batch_encode");
   HIP_TRY(launch_batch_encode(batch_args(b, d_sizes), b->pb, b->shape, ws->cu, s, nullptr));
   prof_mark(ws, s, nullptr);
@@ -1154,9 +1204,23 @@ mans_status mans_batch_decode_run(mans_batch* b, void* stream) {
   a.luts = b->luts;
   a.queue = b->queue;
   a.err = ws->counters + 3;
+  a.bad = b->bad;
+  // per-buffer flags start at zero in every run (a kernel, so the run can
+  // be captured)
+  HIP_TRY(launch_fill_u32(b->bad, b->nbufs, 0, s));
   prof_mark(ws, s, "batch_decode");
   HIP_TRY(launch_batch_decode(a, b->pb, b->shape, ws->cu, s));
   prof_mark(ws, s, nullptr);
+  return MANS_OK;
+}
+
+mans_status mans_batch_decode_flags(const mans_batch* b, uint32_t* h_flags, int32_t max, void* stream) {
+  if (!b || !b->decode || max < 0 || (max > 0 && !h_flags)) return MANS_ERR_INVALID_ARG;
+  DeviceGuard g(b->ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t n = std::min<uint32_t>(b->nbufs, (uint32_t)max);
+  HIP_TRY(hipStreamSynchronize(s));
+  if (n > 0) HIP_TRY(hipMemcpy(h_flags, b->bad, n * 4ull, hipMemcpyDeviceToHost));
   return MANS_OK;
 }
 
@@ -1181,9 +1245,9 @@ mans_status mans_batch_tune(mans_batch* b, void* stream, int32_t reps, mans_tune
   if (!b || reps < 1 || max_entries < 0 || (max_entries > 0 && !h_entries)) return MANS_ERR_INVALID_ARG;
   // candidates: workgroup size x symbols per lane (decode adds 1: the ring
   // looked after every step)
-  static const int kNws[3] = {4, 8, 16}, kEncKbs[3] = {4, 8, 16}, kDecKbs[4] = {1, 4, 8, 16};
+  const int* kNws = kTuneNws;
   const int nkb = b->decode ? 4 : 3;
-  const int* kbs = b->decode ? kDecKbs : kEncKbs;
+  const int* kbs = b->decode ? kTuneDecKbs : kTuneEncKbs;
   const int ncand = 3 * nkb;
   if (n_entries) *n_entries = ncand;
   mans_workspace* ws = b->ws;

@@ -252,6 +252,7 @@ struct BatchDecodeArgs {
   uint32_t* luts;                // [nbufs][4][2^p] packed LUT entries
   uint32_t* queue;               // [3] as BatchEncodeArgs::queue
   uint32_t* err;
+  uint32_t* bad;                 // [nbufs] the error flags of each buffer (zeroed before the launch)
 };
 constexpr uint32_t kBatchBad = 1u << 31;
 constexpr uint32_t kBatchGenMask = 0x3fffu;     // 14-bit generations

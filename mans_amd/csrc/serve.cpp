@@ -36,6 +36,8 @@ mans_status batch_slot_create(mans_batch** out, const mans_params* p, bool decod
                               uint64_t cap_bytes, mans_workspace* ws);
 mans_status batch_slot_plan(mans_batch* b, const mans_params* p, const mans_buffer* bufs, int32_t count,
                             hipStream_t s);
+const uint32_t* batch_slot_bad(const mans_batch* b);
+uint32_t batch_decode_error_bits();
 
 namespace {
 
@@ -109,6 +111,7 @@ struct Slot {
   mans_batch* dec = nullptr;
   uint64_t* d_sizes = nullptr;
   uint64_t* h_sizes = nullptr;  // pinned
+  uint32_t* h_bad = nullptr;    // pinned: per-request error flags of the decode batch
   hipEvent_t done = nullptr;
   std::vector<Req*> enc_reqs, dec_reqs;
   bool busy = false;
@@ -185,6 +188,9 @@ void launch_slot(mans_server* S, Slot& sl) {
   if (!db.empty()) {
     st = batch_slot_plan(sl.dec, &S->p, db.data(), (int32_t)db.size(), sl.stream);
     if (st == MANS_OK) st = mans_batch_decode_run(sl.dec, sl.stream);
+    if (st == MANS_OK && hipMemcpyAsync(sl.h_bad, batch_slot_bad(sl.dec), db.size() * 4, hipMemcpyDeviceToHost,
+                                        sl.stream) != hipSuccess)
+      st = MANS_ERR_HIP;
     if (st != MANS_OK)
       for (Req* q : sl.dec_reqs) q->status = st;
   }
@@ -192,20 +198,25 @@ void launch_slot(mans_server* S, Slot& sl) {
 }
 
 // Marks the requests of a finished slot done (called with the lock held).
+// A malformed frame fails only its own decompress request (the decode
+// batch's per-buffer flags); the slot's sticky error word fails the
+// compress requests only for bits an encode launch raises.
 void finish_slot(mans_server* S, Slot& sl) {
   uint32_t flags = 0;
   const mans_status chk = mans_workspace_check(sl.ws, sl.stream, &flags);
+  const bool enc_bad = chk != MANS_OK && (flags & ~batch_decode_error_bits()) != 0;
   const auto now = Clock::now();
   for (size_t i = 0; i < sl.enc_reqs.size(); ++i) {
     Req* q = sl.enc_reqs[i];
     q->out_size = sl.h_sizes[i];
-    if (q->status == MANS_OK && chk != MANS_OK) q->status = chk;
+    if (q->status == MANS_OK && enc_bad) q->status = MANS_ERR_CORRUPT;
     q->t_done = now;
     q->state = 2;
   }
-  for (Req* q : sl.dec_reqs) {
+  for (size_t i = 0; i < sl.dec_reqs.size(); ++i) {
+    Req* q = sl.dec_reqs[i];
     q->out_size = q->r.n_elems * (uint64_t)q->r.elem_bytes;
-    if (q->status == MANS_OK && chk != MANS_OK) q->status = chk;
+    if (q->status == MANS_OK && sl.h_bad[i] != 0) q->status = MANS_ERR_CORRUPT;
     q->t_done = now;
     q->state = 2;
   }
@@ -302,7 +313,8 @@ mans_status mans_server_create(mans_server** out, const mans_params* p, uint64_t
     if (st == MANS_OK) st = batch_slot_create(&sl.enc, p, false, kSlotBufs, kSlotBytes, sl.ws);
     if (st == MANS_OK) st = batch_slot_create(&sl.dec, p, true, kSlotBufs, kSlotBytes, sl.ws);
     if (st == MANS_OK && (hipMalloc(&sl.d_sizes, kSlotBufs * 8) != hipSuccess ||
-                          hipHostMalloc((void**)&sl.h_sizes, kSlotBufs * 8, hipHostMallocDefault) != hipSuccess))
+                          hipHostMalloc((void**)&sl.h_sizes, kSlotBufs * 8, hipHostMallocDefault) != hipSuccess ||
+                          hipHostMalloc((void**)&sl.h_bad, kSlotBufs * 4, hipHostMallocDefault) != hipSuccess))
       st = MANS_ERR_HIP;
   }
   if (st == MANS_OK) {
@@ -438,6 +450,7 @@ mans_status mans_server_destroy(mans_server* S) {
     if (sl.ws) mans_workspace_destroy(sl.ws);
     if (sl.d_sizes) (void)hipFree(sl.d_sizes);
     if (sl.h_sizes) (void)hipHostFree(sl.h_sizes);
+    if (sl.h_bad) (void)hipHostFree(sl.h_bad);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }

@@ -10,14 +10,16 @@
                       else 0.  Generated on the GPU in z-slabs.  The 1 GiB
                       case is the 1024 x 512 x 1024 field (2^29 codes).
 * zipf_u32         -- Zipf(s) ranks over 2^20 values mapped through a seeded
-                      permutation to 32-bit words with the high byte in [0, 15].
+                      injective map to distinct 32-bit words with the high
+                      byte in [0, 15].  Generated on the GPU.
 * detector_u16     -- frames of round(N(1000, 8)) + 40 * Poisson(0.05).
+                      Generated on the GPU.
 * batch_buffers    -- independent buffers: sizes log-uniform in [4 KiB, 16 MiB],
                       element width uint8 / uint16 / uint32 at 20 / 60 / 20 %,
                       values 50 / 50 from a Laplace source (b log-uniform in
                       [1, 64], centred at half the range, clipped) or the Zipf
-                      source (s = 1.1 ranks over 2^20 values through a seeded
-                      permutation to words with the high byte in [0, 15], low
+                      source (s = 1.1 ranks over 2^20 values through the seeded
+                      injective map to words with the high byte in [0, 15], low
                       bytes kept for narrower widths); buffer i from seed
                       1000 + i.  Generated on the GPU.
 """
@@ -75,20 +77,57 @@ def lorenzo_codes(shape=(1024, 512, 1024), seed: int = 1, error_bound: float = 1
     return out
 
 
-def zipf_u32(n: int, s: float = 1.1, seed: int = 2) -> np.ndarray:
+def zipf_words(seed: int = 2) -> np.ndarray:
+    """The 2^20 distinct 32-bit words (high byte in [0, 15]) that Zipf ranks
+    map to: a seeded injective map of the ranks, drawn without replacement."""
     rng = np.random.default_rng(seed)
-    values = rng.integers(0, 1 << 28, 1 << 20, dtype=np.uint32)  # high byte in [0, 15]
-    ranks = rng.zipf(s, n)
-    ranks = np.minimum(ranks, 1 << 20) - 1
-    return values[ranks].astype(np.uint32)
+    return rng.choice(1 << 28, size=1 << 20, replace=False).astype(np.int64)
 
 
-def detector_u16(frames: int = 64, side: int = 2048, seed: int = 3) -> np.ndarray:
-    rng = np.random.default_rng(seed)
-    n = frames * side * side
-    ped = np.round(rng.normal(1000.0, 8.0, n))
-    photons = rng.poisson(0.05, n) * 40
-    return np.clip(ped + photons, 0, 65535).astype(np.uint16)
+def _zipf_cdf(s: float, device: str):
+    import torch
+
+    w = torch.arange(1, (1 << 20) + 1, device=device, dtype=torch.float64).pow(-s)
+    cdf = torch.cumsum(w, 0)
+    return cdf / cdf[-1].clone()
+
+
+def zipf_u32(n: int = 1 << 27, s: float = 1.1, seed: int = 2, device: str = "cuda", chunk: int = 1 << 24):
+    """int32-typed CUDA tensor of uint32 values: Zipf(s) ranks over 2^20
+    values (inverse CDF of the truncated distribution) mapped to
+    zipf_words(seed).  Generated on the GPU in chunks."""
+    import torch
+
+    words = torch.from_numpy(zipf_words(seed)).to(device)
+    cdf = _zipf_cdf(s, device)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    out = torch.empty(n, dtype=torch.int32, device=device)
+    for i in range(0, n, chunk):
+        m = min(chunk, n - i)
+        u = torch.rand(m, generator=g, device=device, dtype=torch.float64)
+        r = torch.clamp(torch.searchsorted(cdf, u), max=(1 << 20) - 1)
+        out[i:i + m] = words[r].to(torch.int32)  # words < 2^28: no sign wrap
+    return out
+
+
+def detector_u16(frames: int = 64, side: int = 2048, seed: int = 3, device: str = "cuda"):
+    """int16-typed CUDA tensor of uint16 detector counts: round(N(1000, 8))
+    + 40 * Poisson(0.05) per pixel, clipped to [0, 65535].  Generated on the
+    GPU one frame at a time."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    m = side * side
+    out = torch.empty(frames * m, dtype=torch.int16, device=device)
+    rate = torch.full((m,), 0.05, device=device, dtype=torch.float32)
+    for f in range(frames):
+        ped = torch.round(torch.normal(1000.0, 8.0, (m,), generator=g, device=device, dtype=torch.float32))
+        ph = torch.poisson(rate, generator=g) * 40.0
+        v = torch.clamp(ped + ph, 0, 65535).to(torch.int32)
+        out[f * m:(f + 1) * m] = torch.where(v >= 32768, v - 65536, v).to(torch.int16)
+    return out
 
 
 def batch_buffers(count: int = 4096, seed_base: int = 1000, lo: int = 4096, hi: int = 16 << 20,
@@ -98,11 +137,8 @@ def batch_buffers(count: int = 4096, seed_base: int = 1000, lo: int = 4096, hi: 
     import torch
 
     # Zipf source shared by all buffers: inverse-CDF table over 2^20 ranks
-    zr = np.random.default_rng(2)
-    words = torch.from_numpy(zr.integers(0, 1 << 28, 1 << 20, dtype=np.int64)).to(device)
-    w = torch.arange(1, (1 << 20) + 1, device=device, dtype=torch.float64).pow(-1.1)
-    cdf = torch.cumsum(w, 0)
-    cdf /= cdf[-1].clone()
+    words = torch.from_numpy(zipf_words(2)).to(device)
+    cdf = _zipf_cdf(1.1, device)
     out, widths = [], []
     dt = {1: torch.int8, 2: torch.int16, 4: torch.int32}
     for i in range(count):

@@ -167,10 +167,68 @@ def test_batch_decode_flags_a_corrupt_frame(ws):
     torch.cuda.synchronize()
     with pytest.raises(mans_amd.MansError):
         ws.check()
+    flags = dec.flags()
+    assert flags[0] == 0 and flags[1] != 0 and flags[2] == 0, flags
     for i in (0, 2):
         assert (outs[i].cpu().numpy().view(xs[i].dtype) == xs[i]).all(), i
+    # a stream corrupted inside its data (a block's state), not its header
+    bad2 = enc.outs[2].clone()
+    st0 = int(np.frombuffer(bytes(bad2[64:72].cpu().numpy()), dtype="<u8")[0])  # plane 0 section
+    bad2[st0 + 544: st0 + 548] = 0
+    dec2 = mans_amd.DecodeBatch([enc.outs[0], enc.outs[1], bad2], sizes_h, ts, p, ws)
+    dec2.run()
+    flags = dec2.flags()
+    assert flags[0] == 0 and flags[1] == 0 and flags[2] != 0, flags
+    with pytest.raises(mans_amd.MansError):
+        ws.check()
+    # the flags are per run: a clean run clears them
+    dec.close()
+    dec = mans_amd.DecodeBatch(enc.outs, sizes_h, ts, p, ws)
+    dec.run()
+    assert dec.flags() == [0, 0, 0]
+    ws.check()
+    dec2.close()
     dec.close()
     enc.close()
+
+
+def test_workspace_frozen_after_capture():
+    # a frame encode captured in a graph keeps the workspace's buffers: a
+    # later call that would need larger ones fails instead of freeing them,
+    # and the graph still replays correctly
+    ws = mans_amd.Workspace(0)
+    xs, ts = make_batch(13, [300_000])
+    t = ts[0]
+    q = mans_amd.params(10, 64, t.element_size(), mans_amd.MODE_PLANES)
+    ref, ref_size = mans_amd.frame_encode(t, q, ws)
+    ref = ref[:ref_size].clone()
+    out = torch.empty(mans_amd.frame_max_size(t.numel(), q), dtype=torch.uint8, device="cuda")
+    d_size = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        mans_amd.frame_encode(t, q, ws, out=out, stream=s, sync_size=False, d_size=d_size)
+    big = torch.zeros(1 << 26, dtype=torch.int16, device="cuda")
+    with pytest.raises(mans_amd.MansError) as e:
+        mans_amd.frame_encode(big, mans_amd.params(10, 64, 2, mans_amd.MODE_PLANES), ws)
+    assert e.value.status == 5  # MANS_ERR_CAPACITY
+    with pytest.raises(mans_amd.MansError):
+        ws.reserve(1 << 30, mans_amd.params(10, 64, 2, mans_amd.MODE_PLANES))
+    for rep in range(3):
+        out.zero_()
+        d_size.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        print(f"frozen workspace replay {rep}", flush=True)
+        assert int(d_size.item()) == ref_size and torch.equal(out[:ref_size], ref), rep
+    # a call that fits still runs eagerly on the frozen workspace
+    again, again_size = mans_amd.frame_encode(t, q, ws)
+    assert again_size == ref_size and torch.equal(again[:ref_size], ref)
+    ws.check()
+    del g
+    ws.close()
 
 
 SHAPES = [(64 * nw, kb) for nw in (4, 8, 16) for kb in (4, 8, 16)]

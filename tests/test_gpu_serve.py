@@ -145,3 +145,69 @@ def test_server_admission_waits_for_memory(ws):
     assert len(done) == len(reqs)
     srv.close()
     ws.check()
+
+
+def test_server_results_match_oracle(ws):
+    # compress results byte for byte against the oracle's frame of the same
+    # input; decompress requests take frames the ORACLE wrote and must give
+    # back the oracle's input
+    import oracle
+
+    rng = np.random.default_rng(23)
+    srv = mans_amd.Server(mans_amd.params(10, 64), 4 << 30)
+    cases = []
+    for i in range(24):
+        eb = [1, 2, 4][i % 3]
+        n = max(1, int(np.exp(rng.uniform(np.log(2 << 10), np.log(1 << 20)))) // eb)
+        bits = 8 * eb
+        v = np.round(rng.laplace(0, 3.0 + i, n)).astype(np.int64) + (1 << (bits - 1))
+        x = np.clip(v, 0, (1 << bits) - 1).astype({1: np.uint8, 2: np.uint16, 4: np.uint32}[eb])
+        ref = oracle.frame_encode(x, oracle.MODE_PLANES, 10, oracle.FORMAT_NATIVE)
+        t = torch.from_numpy(x.view(VIEW[eb])).cuda()
+        if i % 2 == 0:
+            rid = srv.submit_compress(t)
+        else:
+            f = torch.from_numpy(ref).cuda()
+            rid = srv.submit_decompress(f, ref.size, n, eb)
+        cases.append((rid, i % 2 == 0, x, ref, t))
+    print(f"server: {len(cases)} requests against the oracle", flush=True)
+    for rid, is_c, x, ref, t in cases:
+        res = srv.wait(rid, 60_000)
+        assert res is not None and res.status == 0
+        if is_c:
+            got = srv.fetch(res).cpu().numpy()
+            assert got.size == ref.size and (got == ref).all()
+        else:
+            got = srv.fetch(res, t.dtype).cpu().numpy().view(x.dtype)
+            assert (got == x).all()
+        srv.release(rid)
+    srv.close()
+    ws.check()
+
+
+def test_server_corrupt_frame_fails_only_its_request(ws):
+    reqs = requests(24, 12, 8 << 10, 1 << 20, ws)
+    srv = mans_amd.Server(mans_amd.params(10, 64), 4 << 30)
+    # request 3 decompresses a frame whose first block state is zeroed
+    kind, t, frame, size, eb = reqs[3]
+    assert kind == "d"
+    bad = frame.clone()
+    st0 = int(np.frombuffer(bytes(bad[64:72].cpu().numpy()), dtype="<u8")[0])
+    bad[st0 + 544: st0 + 548] = 0
+    batch = []
+    for i, (kind, t, frame, size, eb) in enumerate(reqs):
+        if kind == "c":
+            batch.append(srv.compress_request(t))
+        else:
+            batch.append(srv.decompress_request(bad if i == 3 else frame, size, t.numel(), eb))
+    print(f"server: {len(batch)} requests, one corrupt frame", flush=True)
+    ids = srv.submit_many(batch)
+    for i, (rid, (kind, t, frame, size, eb)) in enumerate(zip(ids, reqs)):
+        res = srv.wait(rid, 60_000)
+        assert res is not None
+        if i == 3:
+            assert res.status == 6, res.status  # MANS_ERR_CORRUPT
+        else:
+            check_result(srv, res, kind, t, frame, size)
+        srv.release(rid)
+    srv.close()
