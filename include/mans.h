@@ -94,6 +94,19 @@ mans_status mans_encode(const mans_params* p, const void* d_in, uint64_t n, void
                         uint64_t out_capacity, uint32_t* d_out_size, uint64_t* h_out_size,
                         mans_workspace* ws, void* stream);
 
+/* mans_encode for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Writes the same bytes as mans_encode
+ * with p->lanes = 32 (the reference format; 64 returns
+ * MANS_ERR_INVALID_ARG) and the container size to *d_out_size (device
+ * uint32).  It allocates nothing and takes no workspace: its scratch is a
+ * fixed 1.02 MiB device array inside the library, one per GPU, held until
+ * the process ends and shared by every call on that GPU, so calls that
+ * overlap on one GPU race.  n is at most 2^30 bytes (262,144 blocks,
+ * MANS_ERR_TOO_LARGE beyond).  Runs on the current device. */
+mans_status mans_encode_noalloc(const mans_params* p, const void* d_in, uint64_t n, void* d_out,
+                                uint64_t out_capacity, uint32_t* d_out_size, void* stream);
+
 /* Decode the container at d_in (c bytes; 0 = unknown, as the reference)
  * into d_out.  Replaces multibyte_ans::ansDecodeBatch (src/GpuANS.cu:33-44)
  * / ansDecode (include/ans/GpuANSDecode.h:379-451).  No host synchronisation:

@@ -72,6 +72,7 @@ SIGNATURES = {
     "mans_workspace_check": (C.c_int, [_vp, _vp, C.POINTER(_u32)]),
     "mans_max_compressed_size": (_u64, [_u64, _P]),
     "mans_encode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, C.POINTER(_u64), _vp, _vp]),
+    "mans_encode_noalloc": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, _vp]),
     "mans_decode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, _vp]),
     "mans_histogram": (C.c_int, [_vp, _u64, _u32, _vp, _vp, _vp]),
     "mans_normalize": (C.c_int, [_vp, _u32, _i32, _vp, _vp, _vp]),
@@ -220,6 +221,17 @@ def encode(data, p: Params, ws: Workspace, out=None, stream=None, sync_size: boo
                            C.byref(h_size) if sync_size else None, ws.handle, _stream(stream))
     _check(rc, "mans_encode")
     return (out, h_size.value) if sync_size else (out, d_size)
+
+
+def encode_noalloc(data, p: Params, out, d_size, stream=None):
+    """mans_encode_noalloc: the caller passes the output buffer (at least
+    max_compressed_size bytes) and a one-element int32 CUDA tensor for the
+    size; the call allocates nothing."""
+    n = data.numel() * data.element_size()
+    rc = lib().mans_encode_noalloc(C.byref(p), _ptr(data), n, out.data_ptr(), out.numel(), d_size.data_ptr(),
+                                   _stream(stream))
+    _check(rc, "mans_encode_noalloc")
+    return out, d_size
 
 
 def decode(comp, n: int, p: Params, ws: Workspace, out=None, stream=None, c: int | None = None):

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -189,15 +190,24 @@ void frame_header(uint32_t (&hdr)[32], uint64_t n, uint32_t eb, int mode, Format
   }
 }
 
-// Shared by mans_encode and mans_frame_encode: one hist+normalise launch and
-// one encode launch over the planes of `d`.
-mans_status run_encode(mans_workspace* ws, const CodecDesc& d, const HistSource* src, int nsrc,
-                       uint8_t* out, uint32_t* d_out_size32, uint64_t* d_out_size64,
-                       const uint32_t* frame_hdr, hipStream_t s) {
+// Device scratch of one encode: histogram accumulators, counters (0 done,
+// 1 ticket, 3 error flags), tables, look-back words and encoder spill areas.
+struct EncodeScratch {
+  uint32_t* hist;
+  uint32_t* counters;
+  uint4* table;
+  uint64_t* status;
+  uint16_t* spill;
+  uint64_t spill_bytes;
+};
+
+// One hist+normalise launch and one encode launch over the planes of `d`,
+// with the scratch in `sc` (large enough: checked by the callers).
+mans_status launch_encode_pair(const CodecDesc& d, const HistSource* src, int nsrc, uint8_t* out,
+                               uint32_t* d_out_size32, uint64_t* d_out_size64, const uint32_t* frame_hdr,
+                               const EncodeScratch& sc, int cu, mans_workspace* prof, hipStream_t s) {
   const Format f = format_for_lanes(d.lanes);
   const uint64_t chain = chain_len(d.total_units, d.lanes);
-  mans_status st = ensure_status(ws, std::max<uint64_t>(chain, 1));
-  if (st != MANS_OK) return st;
   HistArgs h;
   memset(&h, 0, sizeof(h));
   for (int i = 0; i < nsrc; ++i) h.src[i] = src[i];
@@ -210,14 +220,14 @@ mans_status run_encode(mans_workspace* ws, const CodecDesc& d, const HistSource*
     h.plane_nb[k] = d.plane[k].nb;
     h.plane_section[k] = d.plane[k].section;
   }
-  h.hist = ws->hist;
-  h.done = ws->counters + 0;
-  h.table = ws->table;
+  h.hist = sc.hist;
+  h.done = sc.counters + 0;
+  h.table = sc.table;
   h.out = out;
   h.version = f.version;
-  h.status = ws->status;
+  h.status = sc.status;
   h.status_count = chain;
-  h.ticket = ws->counters + 1;
+  h.ticket = sc.counters + 1;
   h.d_out_size32 = d_out_size32;
   h.d_out_size64 = d_out_size64;
   h.empty_size = d.data_off;
@@ -227,35 +237,97 @@ mans_status run_encode(mans_workspace* ws, const CodecDesc& d, const HistSource*
     memcpy(h.frame_hdr, frame_hdr, sizeof(h.frame_hdr));
     if (d.total_units == 0) put64(h.frame_hdr, 56, d.data_off);
   }
-  prof_mark(ws, s, "hist_normalize");
-  HIP_TRY(launch_hist(h, ws->cu, s));
+  if (prof) prof_mark(prof, s, "hist_normalize");
+  HIP_TRY(launch_hist(h, cu, s));
   EncodeArgs e;
   memset(&e, 0, sizeof(e));
   e.d = d;
-  e.table = ws->table;
+  e.table = sc.table;
   e.out = out;
-  e.status = ws->status;
-  e.ticket = ws->counters + 1;
-  e.err = ws->counters + 3;
+  e.status = sc.status;
+  e.ticket = sc.counters + 1;
+  e.err = sc.counters + 3;
   e.d_out_size32 = d_out_size32;
   e.d_out_size64 = d_out_size64;
   e.frame = frame_hdr ? 1 : 0;
+  e.spill = sc.spill;
+  e.spill_bytes = sc.spill_bytes;
+  if (prof) prof_mark(prof, s, "encode");
+  HIP_TRY(launch_encode(e, cu, s));
+  if (prof) prof_mark(prof, s, nullptr);
+  return MANS_OK;
+}
+
+// Shared by mans_encode and mans_frame_encode: sizes the workspace, then
+// launches with its buffers.
+mans_status run_encode(mans_workspace* ws, const CodecDesc& d, const HistSource* src, int nsrc,
+                       uint8_t* out, uint32_t* d_out_size32, uint64_t* d_out_size64,
+                       const uint32_t* frame_hdr, hipStream_t s) {
+  const uint64_t chain = chain_len(d.total_units, d.lanes);
+  mans_status st = ensure_status(ws, std::max<uint64_t>(chain, 1));
+  if (st != MANS_OK) return st;
+  EncodeArgs probe;
+  memset(&probe, 0, sizeof(probe));
+  probe.d = d;
   uint64_t spill = 0;
-  HIP_TRY(encode_spill_bytes(e, ws->cu, &spill));
+  HIP_TRY(encode_spill_bytes(probe, ws->cu, &spill));
   st = ensure_spill(ws, spill);
   if (st != MANS_OK) return st;
-  e.spill = ws->spill;
-  e.spill_bytes = ws->spill_cap;
-  prof_mark(ws, s, "encode");
-  HIP_TRY(launch_encode(e, ws->cu, s));
-  prof_mark(ws, s, nullptr);
-  return MANS_OK;
+  const EncodeScratch sc{ws->hist, ws->counters, ws->table, ws->status, ws->spill, ws->spill_cap};
+  return launch_encode_pair(d, src, nsrc, out, d_out_size32, d_out_size64, frame_hdr, sc, ws->cu, ws, s);
 }
 
 mans_status read_u64(mans_workspace* ws, const uint64_t* d, uint64_t* h, hipStream_t s) {
   HIP_TRY(hipMemcpyAsync(ws->h_pinned, d, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   *h = ws->h_pinned[0];
+  return MANS_OK;
+}
+
+// ---- allocation-free encode: device scratch held by the library -----------
+// mans_encode_noalloc keeps its scratch in the file-scope array `scratch`
+// below, one copy per GPU for the life of the process, instead of a
+// workspace.  Its largest input is 2^30 bytes: 262,144 blocks of the 32-lane
+// reference format, the reference's own limit (its block-count prefix scan
+// asserts nb <= 512 x 512, BatchPrefixSum.h).
+constexpr uint64_t kNoAllocMaxBytes = 1ull << 30;
+constexpr uint64_t kNoAllocStatus = kNoAllocMaxBytes / 4096 / 2;  // one look-back word per two blocks
+struct NoAllocLayout {
+  uint32_t hist[4 * 256];    // histogram accumulators (cleared by each call)
+  uint32_t counters[64];     // 0 done, 1 ticket, 3 error flags (cleared by each call)
+  uint4 table[4 * 256];      // normalised tables
+  uint64_t status[kNoAllocStatus];  // look-back words (cleared by the histogram launch)
+};
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+NoAllocLayout* g_scratch[kMaxDevices];  // device address of `scratch` per device
+int g_scratch_cu[kMaxDevices];
+
+}  // namespace
+
+// 1.02 MiB of device memory per GPU the library runs mans_encode_noalloc on.
+// In host code this name is only a stand-in for the symbol: kernels and
+// copies use the address hipGetSymbolAddress returns for it (g_scratch).
+static __device__ __align__(256) uint8_t scratch[sizeof(NoAllocLayout)];
+
+namespace {
+
+// The device address of `scratch` on device dev (current), looked up once
+// per device with that device current, and the device's compute units.
+mans_status noalloc_scratch(int dev, NoAllocLayout** sc, int* cu) {
+  if (dev < 0 || dev >= kMaxDevices) return MANS_ERR_NO_DEVICE;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return MANS_ERR_HIP;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+      return MANS_ERR_NO_DEVICE;
+    g_scratch_cu[dev] = n;
+    g_scratch[dev] = (NoAllocLayout*)ptr;
+  }
+  *sc = g_scratch[dev];
+  *cu = g_scratch_cu[dev];
   return MANS_OK;
 }
 
@@ -412,6 +484,42 @@ mans_status mans_encode(const mans_params* p, const void* d_in, uint64_t n, void
   if (st != MANS_OK) return st;
   if (h_out_size) return read_u64(ws, ws->d_u64, h_out_size, s);
   return MANS_OK;
+}
+
+mans_status mans_encode_noalloc(const mans_params* p, const void* d_in, uint64_t n, void* d_out,
+                                uint64_t out_capacity, uint32_t* d_out_size, void* stream) {
+  mans_status st = check_params(p);
+  if (st != MANS_OK) return st;
+  if (p->lanes != 32 || !d_out || !d_out_size || (n > 0 && !d_in)) return MANS_ERR_INVALID_ARG;
+  if (!aligned(d_out, 16) || !aligned(d_in, 4) || !aligned(d_out_size, 4)) return MANS_ERR_ALIGNMENT;
+  if (n > kNoAllocMaxBytes) return MANS_ERR_TOO_LARGE;
+  if (out_capacity < mans_max_compressed_size(n, p)) return MANS_ERR_CAPACITY;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return MANS_ERR_NO_DEVICE;
+  NoAllocLayout* sc = nullptr;
+  int cu = 0;
+  st = noalloc_scratch(dev, &sc, &cu);
+  if (st != MANS_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  // the accumulators and counters start at zero in every call, whatever an
+  // earlier call left (hist and counters are adjacent: one fill kernel)
+  static_assert(offsetof(NoAllocLayout, counters) == sizeof(uint32_t) * 4 * 256, "adjacent");
+  HIP_TRY(launch_fill_u32(sc->hist, 4 * 256 + 64, 0, s));
+  const Format f = format_for_lanes(32);
+  CodecDesc d;
+  memset(&d, 0, sizeof(d));
+  d.num_planes = 1;
+  d.order = 0;
+  d.prob_bits = p->prob_bits;
+  d.lanes = 32;
+  d.plane[0] = PlaneDesc{(const uint8_t*)d_in, nullptr, n, div_up(n, f.block_bytes), 0, 0, 1, 0};
+  d.total_units = d.plane[0].nb;
+  d.data_off = section_bytes(d.plane[0].nb, f);
+  if (chain_len(d.total_units, 32) > kNoAllocStatus) return MANS_ERR_TOO_LARGE;
+  const HistSource src = {(const uint8_t*)d_in, n, 1u, 0u};
+  // the 32-lane encoder stages whole blocks in LDS: no spill areas
+  const EncodeScratch es{sc->hist, sc->counters, sc->table, sc->status, nullptr, 0};
+  return launch_encode_pair(d, &src, n ? 1 : 0, (uint8_t*)d_out, d_out_size, nullptr, nullptr, es, cu, nullptr, s);
 }
 
 mans_status mans_decode(const mans_params* p, const void* d_in, uint64_t c, void* d_out, uint64_t out_capacity,
