@@ -67,14 +67,48 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
 
+// Encoder table entry of the 64-lane coder, for probability precision p:
+// {pdf << (31 - p) | cdf, 2^p - pdf, lo, hi of rcp}, rcp = 1 / pdf rounded
+// up to a double.  (p <= 15, so the threshold pdf << (31 - p) has no bits
+// below bit 16 and cdf fits under it.)  For 0 <= x < 2^31 and pdf <= 2^11,
+// floor(RN(x * rcp)) = floor(x / pdf): x * rcp >= x / pdf >= q, and
+// x * rcp < x / pdf + 2^-21 <= q + 1 - 2^-11 + 2^-21, so neither the
+// product's rounding nor the truncating conversion can leave [q, q + 1).
+// This replaces the reference's magic-number division
+// (GpuANSCodec.h:36-61: q = (umulhi(x, magic) + x) >> shift) by two full-rate
+// conversions and one full-rate f64 multiply on gfx950; q is the same.
+__device__ __forceinline__ uint4 enc64_entry(uint32_t pdf, uint32_t cdf, int p) {
+  double r = 0.0;
+  if (pdf > 0) {
+    r = 1.0 / (double)pdf;
+    // round up: the exact residual r * pdf - 1 (one fma) below zero means
+    // the correctly rounded quotient is below 1 / pdf
+    if (__fma_rn(r, (double)pdf, -1.0) < 0.0) r = __longlong_as_double(__double_as_longlong(r) + 1);
+  }
+  const uint64_t rb = (uint64_t)__double_as_longlong(r);
+  return make_uint4((pdf << (31 - p)) | cdf, (1u << p) - pdf, (uint32_t)rb, (uint32_t)(rb >> 32));
+}
+
+// x / pdf for the entry e (see enc64_entry).
+__device__ __forceinline__ uint32_t enc64_quot(uint32_t x, const uint4& e) {
+  const double r = __hiloint2double((int)e.w, (int)e.z);
+  return (uint32_t)((double)x * r);
+}
+
+// sbase += 2 * popcount(m) in two scalar instructions
+__device__ __forceinline__ void advance2(uint32_t& sbase, uint64_t m) {
+  uint32_t c;
+  __asm__("s_bcnt1_i32_b64 %0, %1" : "=s"(c) : "s"(m) : "scc");
+  __asm__("s_lshl1_add_u32 %0, %1, %0" : "+s"(sbase) : "s"(c) : "scc");
+}
+
 // One symbol of one chain in the 64-lane format with a linear stage, table
-// entry e = {pdf << (31 - p) | cdf, magic, shift, 2^p - pdf} (p <= 15, so
-// the threshold pdf << (31 - p) has no bits below bit 16 and cdf fits under
-// it).  An emitted word goes to LDS byte address sbase + 2 * rank, where
-// sbase (wave uniform) is where the chain's next word goes and rank the
-// number of emitting lanes below this one; a lane that emits nothing writes
-// its own dummy slot at byte address dummy.  sbase advances by two bytes per
-// emitted word.  Lanes with v false keep their state and emit nothing.
+// entry e (enc64_entry).  An emitted word goes to LDS byte address
+// sbase + 2 * rank, where sbase (wave uniform) is where the chain's next word
+// goes and rank the number of emitting lanes below this one; a lane that
+// emits nothing writes its own dummy slot at byte address dummy.  sbase
+// advances by two bytes per emitted word.  Lanes with v false keep their
+// state and emit nothing.
 __device__ __forceinline__ void enc_step64(uint32_t& x, uint32_t& sbase, uint32_t dummy, const uint4& e,
                                            bool v = true) {
   const bool w = v && (x >> 16) >= (e.x >> 16);
@@ -90,16 +124,15 @@ __device__ __forceinline__ void enc_step64(uint32_t& x, uint32_t& sbase, uint32_
   __asm__ volatile("" : "+v"(addr));
   *(lds_u16*)(uintptr_t)addr = (uint16_t)x;
   x = w ? x >> 16 : x;
-  sbase += 2u * (uint32_t)__popcll(m);
-  const uint32_t q = (__umulhi(x, e.y) + x) >> (e.z & 31u);
-  const uint32_t xn = __umul24(q, e.w) + x + (e.x & 0xffffu);
+  advance2(sbase, m);
+  const uint32_t xn = __umul24(enc64_quot(x, e), e.y) + x + (e.x & 0xffffu);
   x = v ? xn : x;
 }
 
 // enc_step64 for a step where every lane codes a symbol, written with
 // sub-dword operand selects: the threshold compare reads the high halves of
 // x and e.x, the renormalising shift is a select of x's high half, and cdf
-// is added from e.x's low half (three VALU operations fewer per symbol).
+// is added from e.x's low half.
 __device__ __forceinline__ void enc_step64f(uint32_t& x, uint32_t& sbase, uint32_t dummy, const uint4& e) {
   uint64_t m;
   uint32_t xr;
@@ -124,9 +157,8 @@ __device__ __forceinline__ void enc_step64f(uint32_t& x, uint32_t& sbase, uint32
       : "s"(m), "v"(emit), "v"(x)
       : "memory", "scc");
   (void)dummy;
-  sbase += 2u * (uint32_t)__popcll(m);
-  const uint32_t q = (__umulhi(xr, e.y) + xr) >> (e.z & 31u);
-  const uint32_t xn = __umul24(q, e.w) + xr;
+  advance2(sbase, m);
+  const uint32_t xn = __umul24(enc64_quot(xr, e), e.y) + xr;
   __asm__("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
           : "=v"(x)
           : "v"(xn), "v"(e.x));

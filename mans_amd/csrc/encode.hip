@@ -317,9 +317,9 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
 // the final place issues a batch of loads before any store.
 // (byte streams: at least 4 waves per SIMD, a few loop-invariant spills)
 template <int PB, int S, int BPT, int NW, int HB>
-__global__ __launch_bounds__(64 * NW, S == 1 ? 4 : 1) void encode64_kernel(EncodeArgs a) {
+__global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
   constexpr int CH = S * BPT;       // chains per lane
-  constexpr int KB = kBatch / BPT;  // steps per prefetch batch
+  constexpr int KB = 8 / BPT < 4 ? 4 : 8 / BPT;  // steps per prefetch batch (elements loaded ahead per block)
   constexpr uint32_t L = 64, BLOCK = 8192u;
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
   constexpr uint32_t H = 1u << HB;  // spill chunk words
@@ -336,12 +336,12 @@ __global__ __launch_bounds__(64 * NW, S == 1 ? 4 : 1) void encode64_kernel(Encod
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int P = a.d.num_planes;
 
-  // [P][256] {pdf << (31 - PB) | cdf, magic, shift, 2^PB - pdf}, at a fixed
-  // LDS address so each chain's base is an immediate offset of its reads
+  // [P][256] entries of enc64_entry, at a fixed LDS address so each chain's
+  // base is an immediate offset of its reads
   __shared__ uint4 s_tab[(S > 1 ? S : kMaxPlanes) * kNumSymbols];
   for (int i = t; i < P * (int)kNumSymbols; i += 64 * NW) {
-    uint4 e = a.table[i];
-    s_tab[i] = make_uint4((e.x << (31 - PB)) | e.y, e.z, e.w, (1u << PB) - e.x);
+    const uint4 e = a.table[i];
+    s_tab[i] = enc64_entry(e.x, e.y, PB);
   }
   if (t < P) {
     s_pl[t].src = a.d.plane[t].src;

@@ -1,6 +1,7 @@
 """The 64-lane encoder narrows exec around its emit store with an inline-asm
-s_and_saveexec_b64, which also writes SCC.  If the asm did not declare the SCC
-clobber, the compiler could keep a scalar compare (the stage-upkeep test)
+s_and_saveexec_b64, and advances its stage pointer with inline-asm
+s_bcnt1 / s_lshl1_add; all of them write SCC.  If an asm did not declare the
+SCC clobber, the compiler could keep a scalar compare (the stage-upkeep test)
 live in SCC across the block and branch on the clobbered value.  This test
 compiles the encoder for gfx950 and checks the generated assembly: no SCC
 value defined before such a block may be read after it."""
@@ -34,7 +35,7 @@ def scc_violations(asm: str) -> list[int]:
             continue
         if SCC_USE.match(line) and clobbered:
             bad.append(i + 1)
-        if inasm and "s_and_saveexec" in line:
+        if inasm and ("s_and_saveexec" in line or SCC_DEF.match(line)):
             clobbered = True
         elif not inasm and SCC_DEF.match(line):
             clobbered = False
