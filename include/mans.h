@@ -138,6 +138,31 @@ mans_status mans_adm_unmap(const int32_t* d_offsets, const uint16_t* d_centers, 
                            const uint8_t* d_bits, uint64_t n, uint16_t* d_out, mans_workspace* ws,
                            void* stream);
 
+/* ---- dual-quantisation Lorenzo front end -------------------------------
+ * No counterpart in the reference (its mapping programs take the uint16
+ * codes as input, mapping/cpu/mappingcpuV1.cpp).  The producer of those
+ * codes for a float32 field f[nz][ny][nx] and absolute error bound eb:
+ * q = rint(f * float32(1 / (2 eb))), r = the 3-D Lorenzo residual of q (q
+ * is 0 outside the field), code = 32768 + r when |r| < radius, else 0 with
+ * (index, r) appended to the outlier list, in the kernel's tile order
+ * (deterministic).  radius in [1, 32768]; |f / (2 eb)| < 2^28 (else
+ * MANS_ERR_DOMAIN); nx <= 8192 and nz * ny * nx < 2^32.
+ * *d_outlier_count (device) gets the number of outliers, which may exceed
+ * outlier_capacity (then only the first outlier_capacity are written).
+ * With h_outlier_count non-NULL the call synchronises, writes the count
+ * there and returns MANS_ERR_CAPACITY / MANS_ERR_DOMAIN when they apply. */
+mans_status mans_lorenzo_quantize(const float* d_field, uint32_t nz, uint32_t ny, uint32_t nx, double error_bound,
+                                  uint32_t radius, uint16_t* d_codes, uint32_t* d_outlier_idx, int32_t* d_outlier_val,
+                                  uint64_t outlier_capacity, uint64_t* d_outlier_count, uint64_t* h_outlier_count,
+                                  mans_workspace* ws, void* stream);
+/* The inverse: q as the 3-D prefix sum of the residuals (codes and
+ * outliers), d_field = q * float32(2 eb).  Needs 4 bytes of workspace per
+ * element. */
+mans_status mans_lorenzo_reconstruct(const uint16_t* d_codes, const uint32_t* d_outlier_idx,
+                                     const int32_t* d_outlier_val, uint64_t outlier_count, uint32_t nz, uint32_t ny,
+                                     uint32_t nx, double error_bound, float* d_field, mans_workspace* ws,
+                                     void* stream);
+
 /* ---- multi-byte frames (byte planes or ADM planes, one container) ------ */
 uint64_t mans_frame_max_size(uint64_t n_elems, const mans_params* p);
 /* Encode n_elems elements of p->elem_bytes bytes.  Frame size to *d_size
@@ -185,13 +210,39 @@ mans_status mans_batch_encode_create(mans_batch** b, const mans_params* p, const
 /* d_sizes: device array of count uint64 frame sizes, written by the launch. */
 mans_status mans_batch_encode_run(mans_batch* b, uint64_t* d_sizes, void* stream);
 /* Decodes frames written by the batched or the single-buffer frame encode
- * (byte planes, 64 lanes, p->prob_bits) whose element count and width the
- * caller knows.  A frame that does not match them, or whose streams are
+ * (byte planes, 64 lanes, p->prob_bits).  A buffer given with elem_bytes 0
+ * (and in_bytes >= 128) takes its element count and width from its frame's
+ * header (read once, at create: the call synchronises); otherwise they are
+ * the caller's.  A frame that does not match them, or whose streams are
  * inconsistent, raises the sticky error read by mans_workspace_check
- * (MANS_ERR_CORRUPT); its output is then undefined. */
+ * (MANS_ERR_CORRUPT) and its buffer's flag (mans_batch_decode_flags); its
+ * output is then undefined. */
 mans_status mans_batch_decode_create(mans_batch** b, const mans_params* p, const mans_buffer* h_bufs,
                                      int32_t count, mans_workspace* ws);
 mans_status mans_batch_decode_run(mans_batch* b, void* stream);
+/* ---- archives: a batch's frames in one container ----------------------
+ * No counterpart in the reference (its container holds one stream,
+ * GpuANSCodec.h:68-209).  Layout: a 64-byte header (u32 "MANA", u8 version
+ * 1, u32 count at 8, u64 total size at 16, index offset 64 at 24, data
+ * offset at 32), then count index entries of 32 bytes {u64 offset, u64
+ * frame bytes, u64 elements, u32 element bytes, u32 0}, then the frames,
+ * each 16-byte aligned in index order with zero padding.
+ * mans_batch_pack copies the frames of an encode batch's last run (d_sizes:
+ * that run's sizes) into the archive at d_archive (capacity at least
+ * mans_batch_archive_max_size) and writes its size to *d_archive_size
+ * (device, may be NULL).  Stream-ordered, only launches: it can be captured
+ * with the run. */
+uint64_t mans_batch_archive_max_size(const mans_batch* b);
+mans_status mans_batch_pack(const mans_batch* b, const uint64_t* d_sizes, void* d_archive, uint64_t capacity,
+                            uint64_t* d_archive_size, void* stream);
+/* Parses an archive header already on the host. */
+mans_status mans_archive_info(const void* h_head, uint64_t head_bytes, uint32_t* count, uint64_t* total_size);
+/* A decode batch over every frame of the archive at d_archive (its header
+ * and index are read once, synchronously): frame i decodes into h_outs[i]
+ * (device, h_out_caps[i] bytes, at least its elements x element bytes). */
+mans_status mans_batch_decode_create_archive(mans_batch** b, const mans_params* p, const void* d_archive,
+                                             uint64_t archive_bytes, void* const* h_outs, const uint64_t* h_out_caps,
+                                             mans_workspace* ws);
 /* Error flags of each buffer of the batch's last decode run (0: decoded;
  * else the buffer's frame was malformed and its output is undefined), for
  * up to max buffers.  Synchronises the stream.  The reference reports no

@@ -78,6 +78,9 @@ SIGNATURES = {
     "mans_normalize": (C.c_int, [_vp, _u32, _i32, _vp, _vp, _vp]),
     "mans_adm_map": (C.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _u64, C.POINTER(_u64), C.POINTER(_i32), _vp, _vp]),
     "mans_adm_unmap": (C.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp]),
+    "mans_lorenzo_quantize": (C.c_int, [_vp, _u32, _u32, _u32, C.c_double, _u32, _vp, _vp, _vp, _u64, _vp,
+                                        C.POINTER(_u64), _vp, _vp]),
+    "mans_lorenzo_reconstruct": (C.c_int, [_vp, _vp, _vp, _u64, _u32, _u32, _u32, C.c_double, _vp, _vp, _vp]),
     "mans_frame_max_size": (_u64, [_u64, _P]),
     "mans_frame_encode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, C.POINTER(_u64), _vp, _vp]),
     "mans_frame_decode": (C.c_int, [_vp, _u64, _vp, _vp, _u64, C.POINTER(_u64), _vp, _vp]),
@@ -87,6 +90,11 @@ SIGNATURES = {
     "mans_batch_decode_create": (C.c_int, [C.POINTER(_vp), _P, _B, _i32, _vp]),
     "mans_batch_decode_run": (C.c_int, [_vp, _vp]),
     "mans_batch_decode_flags": (C.c_int, [_vp, C.POINTER(_u32), _i32, _vp]),
+    "mans_batch_archive_max_size": (_u64, [_vp]),
+    "mans_batch_pack": (C.c_int, [_vp, _vp, _vp, _u64, _vp, _vp]),
+    "mans_archive_info": (C.c_int, [_vp, _u64, C.POINTER(_u32), C.POINTER(_u64)]),
+    "mans_batch_decode_create_archive": (C.c_int, [C.POINTER(_vp), _P, _vp, _u64, C.POINTER(_vp), C.POINTER(_u64),
+                                                   _vp]),
     "mans_batch_destroy": (C.c_int, [_vp]),
     "mans_batch_get_shape": (C.c_int, [_vp, C.POINTER(BatchShape)]),
     "mans_batch_set_shape": (C.c_int, [_vp, C.POINTER(BatchShape)]),
@@ -288,6 +296,41 @@ def adm_unmap(offsets, centers, codes, bits, n: int, ws: Workspace, stream=None)
     return out[:n]
 
 
+def lorenzo_quantize(field, error_bound: float, ws: Workspace, radius: int = 512, outlier_capacity: int | None = None,
+                     codes=None, stream=None):
+    """Lorenzo codes of a float32 CUDA tensor of shape (nz, ny, nx)
+    (mans_lorenzo_quantize).  Returns (codes as an int16-typed tensor of the
+    uint16 codes, outlier indices, outlier residuals), synchronising."""
+    torch = _torch()
+    nz, ny, nx = field.shape
+    n = field.numel()
+    cap = outlier_capacity if outlier_capacity is not None else max(1024, n // 64)
+    if codes is None:
+        codes = torch.empty(n, dtype=torch.int16, device=field.device)
+    idx = torch.empty(max(1, cap), dtype=torch.int32, device=field.device)
+    val = torch.empty(max(1, cap), dtype=torch.int32, device=field.device)
+    d_cnt = torch.zeros(1, dtype=torch.int64, device=field.device)
+    h_cnt = C.c_uint64(0)
+    rc = lib().mans_lorenzo_quantize(_ptr(field), nz, ny, nx, error_bound, radius, _ptr(codes), idx.data_ptr(),
+                                     val.data_ptr(), cap, d_cnt.data_ptr(), C.byref(h_cnt), ws.handle, _stream(stream))
+    _check(rc, "mans_lorenzo_quantize")
+    k = h_cnt.value
+    return codes, idx[:k], val[:k]
+
+
+def lorenzo_reconstruct(codes, idx, val, shape, error_bound: float, ws: Workspace, out=None, stream=None):
+    """float32 field of `shape` from Lorenzo codes and outliers
+    (mans_lorenzo_reconstruct)."""
+    torch = _torch()
+    nz, ny, nx = shape
+    if out is None:
+        out = torch.empty((nz, ny, nx), dtype=torch.float32, device=codes.device)
+    rc = lib().mans_lorenzo_reconstruct(_ptr(codes), _ptr(idx), _ptr(val), idx.numel(), nz, ny, nx, error_bound,
+                                        _ptr(out), ws.handle, _stream(stream))
+    _check(rc, "mans_lorenzo_reconstruct")
+    return out
+
+
 def frame_encode(x, p: Params, ws: Workspace, out=None, stream=None, sync_size: bool = True, d_size=None):
     """Encode a CUDA tensor of 1/2/4-byte elements into a MANS frame.  The
     size is returned on the host (sync_size) or left in `d_size` (a one-element
@@ -379,6 +422,21 @@ class EncodeBatch(_Tunable):
                "mans_batch_encode_run")
         return self.sizes
 
+    def archive_max_size(self) -> int:
+        return lib().mans_batch_archive_max_size(self.handle)
+
+    def pack(self, archive=None, d_size=None, stream=None):
+        """Frames of the last run in one archive (mans_batch_pack); returns
+        (archive, d_size) with the size left on the device."""
+        torch = _torch()
+        if archive is None:
+            archive = torch.empty(self.archive_max_size(), dtype=torch.uint8, device="cuda")
+        if d_size is None:
+            d_size = torch.zeros(1, dtype=torch.int64, device="cuda")
+        _check(lib().mans_batch_pack(self.handle, self.sizes.data_ptr(), archive.data_ptr(), archive.numel(),
+                                     d_size.data_ptr(), _stream(stream)), "mans_batch_pack")
+        return archive, d_size
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib().mans_batch_destroy(self.handle)
@@ -395,18 +453,20 @@ class DecodeBatch(_Tunable):
     """A planned batch decode (mans_batch_decode_create) of byte-plane frames
     (64 lanes): `frames` are uint8 CUDA tensors holding each frame (at least
     its size in bytes), `like` gives each buffer's element count and dtype
-    (tensors, only their numel / dtype are used).  Outputs are allocated in
-    `outs`.  run() only launches."""
+    (tensors, only their numel / dtype are used; with from_headers the
+    library reads them from the frames and `like` only sizes the outputs).
+    Outputs are allocated in `outs`.  run() only launches."""
 
-    def __init__(self, frames, sizes, like, p: Params, ws: Workspace):
+    def __init__(self, frames, sizes, like, p: Params, ws: Workspace, from_headers: bool = False):
         torch = _torch()
         self.frames = list(frames)
         self.ws = ws
         self.outs = [torch.empty(max(1, t.numel()), dtype=t.dtype, device="cuda") for t in like]
         arr = (Buffer * max(1, len(self.frames)))()
         for i, (f, sz, t, o) in enumerate(zip(self.frames, sizes, like, self.outs)):
-            arr[i] = Buffer(f.data_ptr(), int(sz), o.data_ptr(), o.numel() * o.element_size(), t.numel(),
-                            t.element_size(), 0)
+            # from_headers: element count and width come from each frame's header
+            arr[i] = Buffer(f.data_ptr(), int(sz), o.data_ptr(), o.numel() * o.element_size(),
+                            0 if from_headers else t.numel(), 0 if from_headers else t.element_size(), 0)
         h = C.c_void_p()
         _check(lib().mans_batch_decode_create(C.byref(h), C.byref(p), arr, len(self.frames), ws.handle),
                "mans_batch_decode_create")
@@ -434,6 +494,44 @@ class DecodeBatch(_Tunable):
             self.close()
         except Exception:
             pass
+
+
+_NP_DT = {1: "int8", 2: "int16", 4: "int32"}
+
+
+def archive_index(archive) -> list[tuple[int, int, int, int]]:
+    """(offset, frame bytes, elements, element bytes) of every frame of an
+    archive (mans_archive_info + its index, read from the device)."""
+    import numpy as np
+
+    head = archive[:64].cpu().numpy().tobytes()
+    cnt, total = C.c_uint32(0), C.c_uint64(0)
+    _check(lib().mans_archive_info(head, 64, C.byref(cnt), C.byref(total)), "mans_archive_info")
+    n = cnt.value
+    idx = archive[64:64 + 32 * n].cpu().numpy().view(np.uint64).reshape(n, 4) if n else np.zeros((0, 4), np.uint64)
+    return [(int(r[0]), int(r[1]), int(r[2]), int(r[3]) & 0xffffffff) for r in idx]
+
+
+class ArchiveDecodeBatch(DecodeBatch):
+    """A decode batch over every frame of an archive
+    (mans_batch_decode_create_archive); outputs are allocated from the
+    archive's index.  run() only launches."""
+
+    def __init__(self, archive, archive_bytes: int, p: Params, ws: Workspace):
+        torch = _torch()
+        self.ws = ws
+        entries = archive_index(archive)
+        dt = {1: torch.int8, 2: torch.int16, 4: torch.int32}
+        self.outs = [torch.empty(max(1, n), dtype=dt[eb], device="cuda") for _, _, n, eb in entries]
+        self.like_n = [n for _, _, n, _ in entries]
+        self.frames = [None] * len(entries)
+        k = max(1, len(entries))
+        ptrs = (C.c_void_p * k)(*[o.data_ptr() for o in self.outs])
+        caps = (C.c_uint64 * k)(*[o.numel() * o.element_size() for o in self.outs])
+        h = C.c_void_p()
+        _check(lib().mans_batch_decode_create_archive(C.byref(h), C.byref(p), archive.data_ptr(), archive_bytes,
+                                                      ptrs, caps, ws.handle), "mans_batch_decode_create_archive")
+        self.handle = h
 
 
 class Server:

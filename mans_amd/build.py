@@ -32,7 +32,7 @@ DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 
 HIP_SOURCES = ["hist.hip", "encode.hip", "decode.hip", "adm.hip", "batch.hip", "batch_p9.hip", "batch_p10.hip",
-               "batch_p11.hip", "capi.hip"]
+               "batch_p11.hip", "archive.hip", "lorenzo.hip", "capi.hip"]
 HOST_SOURCES = ["gpu_ans_shim.cpp", "serve.cpp"]
 
 

@@ -682,6 +682,97 @@ mans_status mans_adm_unmap(const int32_t* d_offsets, const uint16_t* d_centers, 
   return MANS_OK;
 }
 
+mans_status mans_lorenzo_quantize(const float* d_field, uint32_t nz, uint32_t ny, uint32_t nx, double error_bound,
+                                  uint32_t radius, uint16_t* d_codes, uint32_t* d_outlier_idx, int32_t* d_outlier_val,
+                                  uint64_t outlier_capacity, uint64_t* d_outlier_count, uint64_t* h_outlier_count,
+                                  mans_workspace* ws, void* stream) {
+  const uint64_t n = (uint64_t)nz * ny * nx;
+  if (!ws || !d_outlier_count || (n > 0 && (!d_field || !d_codes))) return MANS_ERR_INVALID_ARG;
+  if (outlier_capacity > 0 && (!d_outlier_idx || !d_outlier_val)) return MANS_ERR_INVALID_ARG;
+  if (!(error_bound > 0.0) || radius < 1 || radius > 32768 || nx > 8192) return MANS_ERR_INVALID_ARG;
+  if (n >= (1ull << 32)) return MANS_ERR_TOO_LARGE;
+  if (!aligned(d_field, 4) || !aligned(d_codes, 2) || !aligned(d_outlier_count, 8)) return MANS_ERR_ALIGNMENT;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  note_capture(ws, s);
+  const uint32_t ntiles = n ? lorenzo_tiles(nz, ny, nx) : 0;
+  mans_status st = ensure_status(ws, std::max<uint64_t>(ntiles, 1));
+  if (st == MANS_OK) st = ensure_scratch(ws, 12ull * std::max<uint32_t>(ntiles, 1));
+  if (st != MANS_OK) return st;
+  // counters[5]: tile ticket; counters[4]: this call's flags
+  HIP_TRY(launch_fill_u64(ws->status, ntiles, 0, s));
+  HIP_TRY(launch_fill_u32(ws->counters + 4, 2, 0, s));
+  if (ntiles == 0) HIP_TRY(launch_fill_u64(d_outlier_count, 1, 0, s));
+  LorenzoArgs a;
+  memset(&a, 0, sizeof(a));
+  a.field = d_field;
+  a.codes = d_codes;
+  a.nz = nz;
+  a.ny = ny;
+  a.nx = nx;
+  a.radius = radius;
+  a.inv2eb = (float)(1.0 / (2.0 * error_bound));
+  a.twoeb = (float)(2.0 * error_bound);
+  a.out_idx = d_outlier_idx;
+  a.out_val = d_outlier_val;
+  a.out_cap = outlier_capacity;
+  a.count = d_outlier_count;
+  a.status = ws->status;
+  a.ticket = ws->counters + 5;
+  a.tile_off = (uint64_t*)ws->scratch;
+  a.tile_flag = (uint32_t*)(ws->scratch + 8ull * std::max<uint32_t>(ntiles, 1));
+  a.ntiles = ntiles;
+  a.err = ws->counters + 4;
+  prof_mark(ws, s, "lorenzo");
+  HIP_TRY(launch_lorenzo(a, s));
+  prof_mark(ws, s, nullptr);
+  if (h_outlier_count) {
+    HIP_TRY(hipMemcpyAsync(ws->h_pinned, d_outlier_count, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ws->h_pinned + 1, ws->counters + 4, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *h_outlier_count = ws->h_pinned[0];
+    const uint32_t flags = (uint32_t)ws->h_pinned[1];
+    if (flags & kErrAdmDomain) return MANS_ERR_DOMAIN;
+    if (ws->h_pinned[0] > outlier_capacity) return MANS_ERR_CAPACITY;
+  }
+  return MANS_OK;
+}
+
+mans_status mans_lorenzo_reconstruct(const uint16_t* d_codes, const uint32_t* d_outlier_idx,
+                                     const int32_t* d_outlier_val, uint64_t outlier_count, uint32_t nz, uint32_t ny,
+                                     uint32_t nx, double error_bound, float* d_field, mans_workspace* ws,
+                                     void* stream) {
+  const uint64_t n = (uint64_t)nz * ny * nx;
+  if (!ws || (n > 0 && (!d_codes || !d_field)) || (outlier_count > 0 && (!d_outlier_idx || !d_outlier_val)))
+    return MANS_ERR_INVALID_ARG;
+  if (!(error_bound > 0.0) || nx > 8192) return MANS_ERR_INVALID_ARG;
+  if (n >= (1ull << 32)) return MANS_ERR_TOO_LARGE;
+  if (!aligned(d_field, 4) || !aligned(d_codes, 2)) return MANS_ERR_ALIGNMENT;
+  if (n == 0) return MANS_OK;
+  DeviceGuard g(ws->device);
+  hipStream_t s = (hipStream_t)stream;
+  note_capture(ws, s);
+  mans_status st = ensure_scratch(ws, 4 * n);
+  if (st != MANS_OK) return st;
+  LorenzoArgs a;
+  memset(&a, 0, sizeof(a));
+  a.codes = (uint16_t*)d_codes;
+  a.nz = nz;
+  a.ny = ny;
+  a.nx = nx;
+  a.twoeb = (float)(2.0 * error_bound);
+  a.in_idx = d_outlier_idx;
+  a.in_val = d_outlier_val;
+  a.n_out = outlier_count;
+  a.q = (int32_t*)ws->scratch;
+  a.recon = d_field;
+  a.err = ws->counters + 3;
+  prof_mark(ws, s, "lorenzo_inverse");
+  HIP_TRY(launch_lorenzo_inverse(a, ws->cu, s));
+  prof_mark(ws, s, nullptr);
+  return MANS_OK;
+}
+
 uint64_t mans_frame_max_size(uint64_t n, const mans_params* p) {
   if (!p) return 0;
   const int pb = (p->prob_bits >= 9 && p->prob_bits <= 11) ? p->prob_bits : 11;
@@ -925,6 +1016,9 @@ struct mans_batch {
   uint16_t* spill_buf = nullptr;
   uint64_t spill_cap = 0;
   uint32_t* bad = nullptr;    // decode: [cap_bufs] error flags of the last run
+  uint64_t* arch_off = nullptr;  // encode: [cap_bufs] frame offsets of the last pack
+  uint64_t max_frame = 0;     // encode: largest frame bound of the plan
+  uint64_t archive_max = 0;   // encode: archive bound of the plan
   // launch shape, and what the tuning cache keys it by
   BatchShape shape = kBatchEncodeDefault;
   uint64_t total_bytes = 0;
@@ -935,7 +1029,7 @@ namespace {
 
 void batch_free(mans_batch* b) {
   void* dev[] = {b->bufs, b->item_buf, b->hist, b->done, b->ready, b->tables,
-                 b->status, b->ptot, b->fin, b->queue, b->luts, b->spill_buf, b->bad};
+                 b->status, b->ptot, b->fin, b->queue, b->luts, b->spill_buf, b->bad, b->arch_off};
   for (void* q : dev)
     if (q) (void)hipFree(q);
   if (b->h_bufs) (void)hipHostFree(b->h_bufs);
@@ -1026,6 +1120,7 @@ mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, ui
     HIP_TRY(hipMalloc(&b->status, std::max<uint64_t>(1, cap_ticks) * 8));
     HIP_TRY(hipMalloc(&b->ptot, nb1 * 16));
     HIP_TRY(hipMalloc(&b->fin, nb1 * 4));
+    HIP_TRY(hipMalloc(&b->arch_off, nb1 * 8));
     HIP_TRY(hipMemset(b->hist, 0, nb1 * 4096));
     HIP_TRY(hipMemset(b->done, 0, nb1 * 4));
     HIP_TRY(hipMemset(b->status, 0, std::max<uint64_t>(1, cap_ticks) * 8));
@@ -1073,9 +1168,18 @@ mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h
   b->pb = p->prob_bits;
   b->total_bytes = 0;
   b->width_mask = 0;
+  b->max_frame = 0;
+  b->archive_max = kArchiveHeaderBytes + (uint64_t)kArchiveEntryBytes * (uint64_t)count;
   for (int32_t i = 0; i < count; ++i) {
     b->total_bytes += h_bufs[i].n_elems * (uint64_t)h_bufs[i].elem_bytes;
     b->width_mask |= (uint32_t)h_bufs[i].elem_bytes;
+    mans_params q = *p;
+    q.lanes = 64;
+    q.elem_bytes = h_bufs[i].elem_bytes;
+    q.mode = MANS_MODE_PLANES;
+    const uint64_t fm = mans_frame_max_size(h_bufs[i].n_elems, &q);
+    b->max_frame = std::max(b->max_frame, fm);
+    b->archive_max += round_up(fm, 16);
   }
   b->nbufs = (uint32_t)count;
   b->nhist = b->decode ? 0u : (uint32_t)nhist;
@@ -1198,13 +1302,65 @@ mans_status batch_reserve_spill(mans_batch* b, BatchShape sh) {
   return need <= b->spill_cap ? MANS_OK : MANS_ERR_INVALID_ARG;
 }
 
-mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs, int32_t count,
+// Decode buffers whose elem_bytes is 0 take their element count and width
+// from their frame's header: one gather launch and one copy for all of them
+// (the frames must be byte-plane frames of the batch's precision).
+mans_status read_frame_dims(const mans_params* p, std::vector<mans_buffer>& v, mans_workspace* ws) {
+  std::vector<uint32_t> idx;
+  std::vector<const uint8_t*> src;
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (v[i].elem_bytes != 0) continue;
+    if (!v[i].d_in || v[i].in_bytes < kFrameHeaderBytes) return MANS_ERR_INVALID_ARG;
+    if (!aligned(v[i].d_in, 16)) return MANS_ERR_ALIGNMENT;
+    idx.push_back((uint32_t)i);
+    src.push_back((const uint8_t*)v[i].d_in);
+  }
+  if (idx.empty()) return MANS_OK;
+  DeviceGuard g(ws->device);
+  const size_t n = idx.size();
+  const uint8_t** d_src = nullptr;
+  uint4* d_hdr = nullptr;
+  std::vector<uint8_t> h((size_t)n * kFrameHeaderBytes);
+  mans_status st = MANS_OK;
+  if (hipMalloc(&d_src, n * sizeof(void*)) != hipSuccess || hipMalloc(&d_hdr, n * kFrameHeaderBytes) != hipSuccess ||
+      hipMemcpy(d_src, src.data(), n * sizeof(void*), hipMemcpyHostToDevice) != hipSuccess ||
+      launch_gather_headers(d_src, (uint32_t)n, d_hdr, nullptr) != hipSuccess ||
+      hipMemcpy(h.data(), d_hdr, n * kFrameHeaderBytes, hipMemcpyDeviceToHost) != hipSuccess)
+    st = MANS_ERR_HIP;
+  if (d_src) (void)hipFree(d_src);
+  if (d_hdr) (void)hipFree(d_hdr);
+  if (st != MANS_OK) return st;
+  for (size_t k = 0; k < n; ++k) {
+    const uint8_t* hb = h.data() + k * kFrameHeaderBytes;
+    uint32_t magic;
+    uint64_t ne, total;
+    memcpy(&magic, hb, 4);
+    memcpy(&ne, hb + 16, 8);
+    memcpy(&total, hb + 56, 8);
+    const uint32_t eb = hb[6], mode = hb[7], lanes = hb[8], pb = hb[9], P = hb[10], order = hb[11];
+    mans_buffer& m = v[idx[k]];
+    if (magic != kFrameMagic || hb[4] != kFrameVersion || mode != MANS_MODE_PLANES || lanes != 64 ||
+        (int)pb != p->prob_bits || P != eb || order != 0 || (eb != 1 && eb != 2 && eb != 4) || total > m.in_bytes)
+      return MANS_ERR_CORRUPT;
+    m.n_elems = ne;
+    m.elem_bytes = (int32_t)eb;
+  }
+  return MANS_OK;
+}
+
+mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buffer* h_bufs_in, int32_t count,
                          mans_workspace* ws, bool decode) {
   if (!out) return MANS_ERR_INVALID_ARG;
   *out = nullptr;
   mans_status st = check_params(p);
   if (st != MANS_OK) return st;
-  if (!ws || count < 0 || (count > 0 && !h_bufs) || p->lanes != 64) return MANS_ERR_INVALID_ARG;
+  if (!ws || count < 0 || (count > 0 && !h_bufs_in) || p->lanes != 64) return MANS_ERR_INVALID_ARG;
+  std::vector<mans_buffer> given(h_bufs_in, h_bufs_in + count);
+  if (decode) {
+    st = read_frame_dims(p, given, ws);
+    if (st != MANS_OK) return st;
+  }
+  const mans_buffer* h_bufs = given.data();
   std::vector<BatchBuf> bufs;
   uint64_t nhist = 0, nticks = 0;
   st = plan_batch(p, h_bufs, count, decode, bufs, nhist, nticks);
@@ -1320,6 +1476,84 @@ mans_status mans_batch_decode_run(mans_batch* b, void* stream) {
   HIP_TRY(launch_batch_decode(a, b->pb, b->shape, ws->cu, s));
   prof_mark(ws, s, nullptr);
   return MANS_OK;
+}
+
+uint64_t mans_batch_archive_max_size(const mans_batch* b) { return (b && !b->decode) ? b->archive_max : 0; }
+
+mans_status mans_batch_pack(const mans_batch* b, const uint64_t* d_sizes, void* d_archive, uint64_t capacity,
+                            uint64_t* d_archive_size, void* stream) {
+  if (!b || b->decode || !d_archive || (b->nbufs > 0 && !d_sizes)) return MANS_ERR_INVALID_ARG;
+  if (!aligned(d_archive, 16)) return MANS_ERR_ALIGNMENT;
+  if (capacity < b->archive_max) return MANS_ERR_CAPACITY;
+  DeviceGuard g(b->ws->device);
+  ArchiveArgs a;
+  memset(&a, 0, sizeof(a));
+  a.bufs = b->bufs;
+  a.sizes = d_sizes;
+  a.offsets = b->arch_off;
+  a.out = (uint8_t*)d_archive;
+  a.d_size = d_archive_size;
+  a.data_off = kArchiveHeaderBytes + (uint64_t)kArchiveEntryBytes * b->nbufs;
+  a.count = b->nbufs;
+  HIP_TRY(launch_archive_pack(a, b->max_frame, (hipStream_t)stream));
+  return MANS_OK;
+}
+
+mans_status mans_archive_info(const void* h_head, uint64_t head_bytes, uint32_t* count, uint64_t* total_size) {
+  if (!h_head || head_bytes < kArchiveHeaderBytes) return MANS_ERR_INVALID_ARG;
+  const uint8_t* h = (const uint8_t*)h_head;
+  uint32_t magic, n;
+  uint64_t total, index_off, data_off;
+  memcpy(&magic, h, 4);
+  memcpy(&n, h + 8, 4);
+  memcpy(&total, h + 16, 8);
+  memcpy(&index_off, h + 24, 8);
+  memcpy(&data_off, h + 32, 8);
+  if (magic != kArchiveMagic || h[4] != 1 || index_off != kArchiveHeaderBytes ||
+      data_off != kArchiveHeaderBytes + (uint64_t)kArchiveEntryBytes * n || total < data_off)
+    return MANS_ERR_CORRUPT;
+  if (count) *count = n;
+  if (total_size) *total_size = total;
+  return MANS_OK;
+}
+
+mans_status mans_batch_decode_create_archive(mans_batch** out, const mans_params* p, const void* d_archive,
+                                             uint64_t archive_bytes, void* const* h_outs, const uint64_t* h_out_caps,
+                                             mans_workspace* ws) {
+  if (!out || !p || !d_archive || !ws) return MANS_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (!aligned(d_archive, 16)) return MANS_ERR_ALIGNMENT;
+  if (archive_bytes < kArchiveHeaderBytes) return MANS_ERR_CORRUPT;
+  DeviceGuard g(ws->device);
+  uint8_t head[kArchiveHeaderBytes];
+  HIP_TRY(hipMemcpy(head, d_archive, kArchiveHeaderBytes, hipMemcpyDeviceToHost));
+  uint32_t n = 0;
+  uint64_t total = 0;
+  mans_status st = mans_archive_info(head, kArchiveHeaderBytes, &n, &total);
+  if (st != MANS_OK) return st;
+  if (total > archive_bytes || n > 0x7fffffffu) return MANS_ERR_CORRUPT;
+  if (n > 0 && (!h_outs || !h_out_caps)) return MANS_ERR_INVALID_ARG;
+  std::vector<uint64_t> idx((size_t)n * 4);
+  if (n) HIP_TRY(hipMemcpy(idx.data(), (const uint8_t*)d_archive + kArchiveHeaderBytes,
+                           (size_t)n * kArchiveEntryBytes, hipMemcpyDeviceToHost));
+  std::vector<mans_buffer> bufs(n);
+  const uint64_t data_off = kArchiveHeaderBytes + (uint64_t)kArchiveEntryBytes * n;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint64_t off = idx[4 * i], size = idx[4 * i + 1], ne = idx[4 * i + 2];
+    const uint32_t eb = (uint32_t)idx[4 * i + 3];
+    if ((off & 15) || off < data_off || size > total || off > total - size || (idx[4 * i + 3] >> 32) != 0)
+      return MANS_ERR_CORRUPT;
+    if (eb != 1 && eb != 2 && eb != 4) return MANS_ERR_CORRUPT;
+    mans_buffer& m = bufs[i];
+    m.d_in = (const uint8_t*)d_archive + off;
+    m.in_bytes = size;
+    m.d_out = h_outs[i];
+    m.out_capacity = h_out_caps[i];
+    m.n_elems = ne;
+    m.elem_bytes = (int32_t)eb;
+    m.reserved = 0;
+  }
+  return batch_create(out, p, bufs.data(), (int32_t)n, ws, true);
 }
 
 mans_status mans_batch_decode_flags(const mans_batch* b, uint32_t* h_flags, int32_t max, void* stream) {

@@ -1,0 +1,322 @@
+// lorenzo.hip -- dual-quantisation 3-D Lorenzo front end: the producer of
+// uint16 quantisation codes for the codec, and its inverse.
+//
+// The reference has no such kernel; its mapping programs take the codes as
+// given (mapping/cpu/mappingcpuV1.cpp reads a uint16 file).  This is the
+// error-bounded step that makes them from a float32 field
+// f[nz][ny][nx] with absolute error bound eb:
+//   q = rint(f * (1 / (2 eb)))                    (prequantisation, float32)
+//   L2(z, y, x) = q(z,y,x) - q(z,y-1,x) - q(z,y,x-1) + q(z,y-1,x-1)
+//   r(z, y, x)  = L2(z, y, x) - L2(z-1, y, x)     (3-D Lorenzo residual; q = 0
+//                                                 outside the field)
+//   code = 32768 + r if |r| < radius, else 0 and (index, r) is an outlier.
+// The inverse rebuilds q as the 3-D prefix sum of r and f' = q * (2 eb), so
+// |f - f'| <= eb up to float rounding.
+//
+// Forward: a workgroup of 256 threads takes a tile of 256 x-columns x 16 rows
+// x 16 planes (tiles taken by an atomic ticket, in ticket order).  A thread
+// keeps its column's L2 of the previous plane for its 16 rows in registers,
+// so each element's f is read once (plus one halo row and the plane below
+// the tile); q(z, y, x - 1) comes from the neighbouring lane.  Outliers are
+// compacted in tile order (tile ticket, thread, plane, row) with a decoupled
+// look-back over tiles; a thread holds up to kHold of them in registers, and
+// a tile with more is rewritten by the fix-up launch.
+#include "codec_device.hpp"
+
+namespace mans {
+namespace {
+
+constexpr uint32_t kTX = 256, kTY = 16, kTZ = 16;
+constexpr int kHold = 4;
+constexpr int32_t kQLimit = 1 << 28;  // |q| bound: residuals stay inside int32
+
+__device__ __forceinline__ int32_t quant(float f, float inv, bool& bad) {
+  const float v = rintf(f * inv);
+  if (!(fabsf(v) < (float)kQLimit)) {  // also NaN / inf
+    bad = true;
+    return 0;
+  }
+  return (int32_t)v;
+}
+
+struct TileGeom {
+  uint32_t nxb, nyb, nzb;
+};
+
+__device__ __forceinline__ void tile_of(const LorenzoArgs& a, uint32_t t, uint32_t& x0, uint32_t& y0, uint32_t& z0) {
+  const uint32_t nxb = (uint32_t)div_up(a.nx, kTX), nyb = (uint32_t)div_up(a.ny, kTY);
+  x0 = (t % nxb) * kTX;
+  y0 = ((t / nxb) % nyb) * kTY;
+  z0 = (t / (nxb * nyb)) * kTZ;
+}
+
+// Codes of one tile; outliers go to `hold` (first kHold) and are counted in
+// `cnt`.  With WRITE_ALL, every outlier is written at a.out_idx / a.out_val
+// from position `pos` on instead (the fix-up pass).
+template <bool WRITE_ALL>
+__device__ void code_tile(const LorenzoArgs& a, uint32_t x0, uint32_t y0, uint32_t z0, uint32_t& cnt,
+                          uint32_t (&hidx)[kHold], int32_t (&hval)[kHold], uint64_t pos, bool& bad) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t x = x0 + threadIdx.x;
+  const bool xin = x < a.nx;
+  const uint64_t plane = (uint64_t)a.ny * a.nx;
+  const float inv = a.inv2eb;
+  // q at (z, y, x) and (z, y, x - 1); rows or planes outside the field are 0
+  auto qpair = [&](uint32_t z, int64_t y, int32_t& qc, int32_t& ql) {
+    qc = 0;
+    if (y >= 0 && xin) qc = quant(a.field[z * plane + (uint64_t)y * a.nx + x], inv, bad);
+    int32_t nb = __shfl_up(qc, 1, 64);
+    if (lane == 0) {
+      nb = 0;
+      if (y >= 0 && x > 0 && x - 1 < a.nx) nb = quant(a.field[z * plane + (uint64_t)y * a.nx + x - 1], inv, bad);
+    }
+    ql = nb;
+  };
+  const uint32_t ny1 = a.ny - y0 < kTY ? a.ny - y0 : kTY;
+  const uint32_t nz1 = a.nz - z0 < kTZ ? a.nz - z0 : kTZ;
+  int32_t prev[kTY];
+#pragma unroll
+  for (uint32_t j = 0; j < kTY; ++j) prev[j] = 0;
+  // the plane below the tile: its L2 per row
+  if (z0 > 0) {
+    int32_t qu, qul;
+    qpair(z0 - 1, (int64_t)y0 - 1, qu, qul);
+#pragma unroll
+    for (uint32_t j = 0; j < kTY; ++j) {
+      if (j < ny1) {
+        int32_t qc, ql;
+        qpair(z0 - 1, y0 + j, qc, ql);
+        prev[j] = qc - qu - ql + qul;
+        qu = qc;
+        qul = ql;
+      }
+    }
+  }
+  for (uint32_t dz = 0; dz < nz1; ++dz) {
+    const uint32_t z = z0 + dz;
+    int32_t qu, qul;
+    qpair(z, (int64_t)y0 - 1, qu, qul);
+#pragma unroll
+    for (uint32_t j = 0; j < kTY; ++j) {
+      if (j < ny1) {
+        const uint32_t y = y0 + j;
+        int32_t qc, ql;
+        qpair(z, y, qc, ql);
+        const int32_t l2 = qc - qu - ql + qul;
+        const int32_t r = l2 - prev[j];
+        prev[j] = l2;
+        qu = qc;
+        qul = ql;
+        if (xin) {
+          const uint64_t i = z * plane + (uint64_t)y * a.nx + x;
+          const bool in = r > -(int32_t)a.radius && r < (int32_t)a.radius;
+          if (!WRITE_ALL) a.codes[i] = in ? (uint16_t)(32768 + r) : (uint16_t)0;
+          if (!in) {
+            if (WRITE_ALL) {
+              if (pos + cnt < a.out_cap) {
+                a.out_idx[pos + cnt] = (uint32_t)i;
+                a.out_val[pos + cnt] = r;
+              }
+            } else {
+#pragma unroll
+              for (int k = 0; k < kHold; ++k)
+                if ((uint32_t)k == cnt) {
+                  hidx[k] = (uint32_t)i;
+                  hval[k] = r;
+                }
+            }
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Exclusive scan of v over the workgroup's 256 threads; *total gets the sum.
+__device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t* s_w, uint32_t& total) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o, 64);
+    if (lane >= (uint32_t)o) inc += u;
+  }
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  uint32_t before = 0;
+  total = 0;
+  for (uint32_t k = 0; k < kTX / 64; ++k) {
+    if (k < w) before += s_w[k];
+    total += s_w[k];
+  }
+  __syncthreads();
+  return before + inc - v;
+}
+
+__global__ __launch_bounds__(kTX) void lorenzo_kernel(LorenzoArgs a) {
+  __shared__ uint32_t s_t, s_w[kTX / 64];
+  __shared__ uint64_t s_off;
+  __shared__ int s_bad;
+  if (threadIdx.x == 0) {
+    s_t = atomicAdd(a.ticket, 1u);
+    s_bad = 0;
+  }
+  __syncthreads();
+  const uint32_t t = s_t;
+  uint32_t x0, y0, z0;
+  tile_of(a, t, x0, y0, z0);
+  uint32_t cnt = 0;
+  uint32_t hidx[kHold];
+  int32_t hval[kHold];
+  bool bad = false;
+  code_tile<false>(a, x0, y0, z0, cnt, hidx, hval, 0, bad);
+  if (bad) s_bad = 1;
+  uint32_t total = 0;
+  const uint32_t mine = block_scan256(cnt, s_w, total);
+  // wave 0: the tile's offset among the outliers of earlier tiles
+  if (threadIdx.x < 64) {
+    const uint64_t excl = look_back(a.status, t, total, (int)threadIdx.x);
+    if (threadIdx.x == 0) {
+      s_off = excl;
+      a.tile_off[t] = excl;
+      if (t + 1 == a.ntiles) *a.count = excl + total;
+    }
+  }
+  __syncthreads();
+  const uint64_t pos = s_off + mine;
+#pragma unroll
+  for (int k = 0; k < kHold; ++k)
+    if ((uint32_t)k < cnt && pos + k < a.out_cap) {
+      a.out_idx[pos + k] = hidx[k];
+      a.out_val[pos + k] = hval[k];
+    }
+  // a thread with more outliers than it holds: the fix-up launch rewrites
+  // the tile; outliers past the capacity: flagged
+  const uint32_t over = __syncthreads_or(cnt > (uint32_t)kHold);
+  if (threadIdx.x == 0) {
+    a.tile_flag[t] = over ? 1u : 0u;
+    if (s_off + total > a.out_cap) atomicOr(a.err, kErrCapacity);
+    if (s_bad) atomicOr(a.err, kErrAdmDomain);
+  }
+}
+
+// Rewrites every outlier of the tiles the main launch flagged.
+__global__ __launch_bounds__(kTX) void lorenzo_fixup_kernel(LorenzoArgs a) {
+  __shared__ uint32_t s_w[kTX / 64];
+  const uint32_t t = blockIdx.x;
+  if (a.tile_flag[t] == 0) return;
+  uint32_t x0, y0, z0;
+  tile_of(a, t, x0, y0, z0);
+  uint32_t cnt = 0, dummy_i[kHold];
+  int32_t dummy_v[kHold];
+  bool bad = false;
+  // count first (same pass, nothing written), then write from the offsets
+  code_tile<false>(a, x0, y0, z0, cnt, dummy_i, dummy_v, 0, bad);
+  uint32_t total = 0;
+  const uint64_t pos = a.tile_off[t] + block_scan256(cnt, s_w, total);
+  cnt = 0;
+  code_tile<true>(a, x0, y0, z0, cnt, dummy_i, dummy_v, pos, bad);
+}
+
+// ---- inverse ---------------------------------------------------------------
+// q of outliers first (scattered), then per plane: x-prefix of each row plus
+// the y-prefix (running per column), then per column: the z-prefix and
+// f' = q * 2 eb.
+__global__ void lorenzo_scatter_kernel(LorenzoArgs a) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n_out; k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t i = a.in_idx[k];
+    if ((uint64_t)i < (uint64_t)a.nx * a.ny * a.nz) a.q[i] = a.in_val[k];
+    else atomicOr(a.err, kErrBlockWords);
+  }
+}
+
+constexpr uint32_t kPlaneThreads = 1024;
+
+__global__ __launch_bounds__(kPlaneThreads) void lorenzo_plane_kernel(LorenzoArgs a) {
+  __shared__ int32_t s_w[kPlaneThreads / 64];
+  const uint32_t z = blockIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint64_t plane = (uint64_t)a.ny * a.nx;
+  constexpr int kMaxChunks = 8;  // nx <= 8192
+  int32_t colsum[kMaxChunks];
+#pragma unroll
+  for (int c = 0; c < kMaxChunks; ++c) colsum[c] = 0;
+  for (uint32_t y = 0; y < a.ny; ++y) {
+    int32_t carry = 0;
+#pragma unroll
+    for (int c = 0; c < kMaxChunks; ++c) {
+      const uint32_t x = (uint32_t)c * kPlaneThreads + threadIdx.x;
+      if ((uint32_t)c * kPlaneThreads >= a.nx) break;
+      const uint64_t i = z * plane + (uint64_t)y * a.nx + x;
+      int32_t r = 0;
+      if (x < a.nx) {
+        const uint16_t code = a.codes[i];
+        r = code ? (int32_t)code - 32768 : a.q[i];
+      }
+      int32_t inc = r;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int32_t u = __shfl_up(inc, o, 64);
+        if (lane >= (uint32_t)o) inc += u;
+      }
+      if (lane == 63) s_w[w] = inc;
+      __syncthreads();
+      int32_t before = carry, all = 0;
+      for (uint32_t k = 0; k < kPlaneThreads / 64; ++k) {
+        if (k < w) before += s_w[k];
+        all += s_w[k];
+      }
+      __syncthreads();
+      carry += all;
+      colsum[c] += before + inc;  // x-prefix of this row, summed over rows
+      if (x < a.nx) a.q[i] = colsum[c];
+    }
+  }
+}
+
+__global__ void lorenzo_column_kernel(LorenzoArgs a) {
+  const uint64_t plane = (uint64_t)a.ny * a.nx;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < plane; j += (uint64_t)gridDim.x * blockDim.x) {
+    int32_t s = 0;
+    for (uint32_t z = 0; z < a.nz; ++z) {
+      s += a.q[z * plane + j];
+      a.recon[z * plane + j] = (float)s * a.twoeb;
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_lorenzo(const LorenzoArgs& a, hipStream_t s) {
+  if (a.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(lorenzo_kernel, dim3(a.ntiles), dim3(kTX), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(lorenzo_fixup_kernel, dim3(a.ntiles), dim3(kTX), 0, s, a);
+  return hipGetLastError();
+}
+
+uint32_t lorenzo_tiles(uint32_t nz, uint32_t ny, uint32_t nx) {
+  return (uint32_t)(div_up(nx, kTX) * div_up(ny, kTY) * div_up(nz, kTZ));
+}
+
+hipError_t launch_lorenzo_inverse(const LorenzoArgs& a, int cu, hipStream_t s) {
+  const uint64_t n = (uint64_t)a.nx * a.ny * a.nz;
+  if (n == 0) return hipSuccess;
+  if (a.n_out) {
+    const uint32_t g = (uint32_t)std::min<uint64_t>(div_up(a.n_out, 256), 4096);
+    hipLaunchKernelGGL(lorenzo_scatter_kernel, dim3(g), dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(lorenzo_plane_kernel, dim3(a.nz), dim3(kPlaneThreads), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint64_t plane = (uint64_t)a.ny * a.nx;
+  const uint32_t g = (uint32_t)std::min<uint64_t>(div_up(plane, 256), (uint64_t)cu * 16);
+  hipLaunchKernelGGL(lorenzo_column_kernel, dim3(g), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace mans

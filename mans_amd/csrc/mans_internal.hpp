@@ -258,6 +258,55 @@ constexpr uint32_t kBatchBad = 1u << 31;
 constexpr uint32_t kBatchGenMask = 0x3fffu;     // 14-bit generations
 constexpr uint32_t kBatchHistChunk = 1u << 18;  // bytes per histogram item
 
+// ---- archive: the frames of a batch in one container (archive.hip) --------
+constexpr uint32_t kArchiveMagic = 0x414E414Du;  // "MANA"
+constexpr uint32_t kArchiveHeaderBytes = 64;
+constexpr uint32_t kArchiveEntryBytes = 32;
+struct ArchiveArgs {
+  const BatchBuf* bufs;          // frame i at bufs[i].out, its elements / widths
+  const uint64_t* sizes;         // [count] frame bytes
+  uint64_t* offsets;             // [count] scratch: frame offsets in the archive
+  uint8_t* out;                  // the archive
+  uint64_t* d_size;              // archive bytes out (optional)
+  uint64_t data_off;             // 64 + 32 * count
+  uint32_t count;
+  uint32_t pad_;
+};
+hipError_t launch_archive_pack(const ArchiveArgs& a, uint64_t max_frame, hipStream_t s);
+// 128-byte headers of the frames at src[i] (a device array) into out[8 * i ..]
+hipError_t launch_gather_headers(const uint8_t* const* src, uint32_t count, uint4* out, hipStream_t s);
+
+// ---- dual-quantisation Lorenzo front end (lorenzo.hip) ----------------------
+struct LorenzoArgs {
+  const float* field;            // forward: f[nz][ny][nx]
+  uint16_t* codes;               // forward: out; inverse: in
+  uint32_t nz, ny, nx;
+  uint32_t radius;
+  float inv2eb;                  // float32(1 / (2 eb))
+  float twoeb;                   // float32(2 eb)
+  // forward: outliers in tile order
+  uint32_t* out_idx;
+  int32_t* out_val;
+  uint64_t out_cap;
+  uint64_t* count;               // outliers found (device)
+  uint64_t* status;              // [ntiles] look-back words (zeroed before)
+  uint32_t* ticket;              // tile ticket (zeroed before)
+  uint64_t* tile_off;            // [ntiles] first outlier of each tile
+  uint32_t* tile_flag;           // [ntiles] 1: the fix-up launch rewrites the tile
+  uint32_t ntiles;
+  uint32_t pad_;
+  uint32_t* err;                 // kErrCapacity: outliers beyond out_cap; kErrAdmDomain: |q| >= 2^28
+  // inverse
+  const uint32_t* in_idx;
+  const int32_t* in_val;
+  uint64_t n_out;
+  int32_t* q;                    // [n] scratch
+  float* recon;                  // [n] out
+};
+hipError_t launch_lorenzo(const LorenzoArgs& a, hipStream_t s);
+hipError_t launch_lorenzo_inverse(const LorenzoArgs& a, int cu, hipStream_t s);
+uint32_t lorenzo_tiles(uint32_t nz, uint32_t ny, uint32_t nx);
+
 hipError_t launch_hist(const HistArgs& a, int cu, hipStream_t s);
 
 // Launch shape of the batched kernels; no choice changes an output bit.

@@ -2,13 +2,14 @@
 
 * laplace_u16      -- clip(32768 + round(Laplace(0, b)), 0, 65535), the small
                       CPU-runnable case (2,097,152 values, b = 4, seed 0).
-* lorenzo_codes    -- uint16 error-bounded quantisation codes: a float32 3-D
-                      field (sum of 64 random-phase plane sinusoids, wavelengths
-                      16-256 cells, plus N(0, 0.01) noise) quantised with
-                      absolute error bound 1e-3 as q = round(f / 2e-3), 3-D
-                      Lorenzo residual r of q, code = 32768 + r if |r| < 512
-                      else 0.  Generated on the GPU in z-slabs.  The 1 GiB
-                      case is the 1024 x 512 x 1024 field (2^29 codes).
+* sinusoid_field   -- a float32 3-D field: the sum of 64 random-phase plane
+                      sinusoids (wavelengths 16-256 cells) plus N(0, 0.01)
+                      noise, generated on the GPU.
+* lorenzo_codes    -- uint16 error-bounded quantisation codes of that field
+                      from the library's Lorenzo front end: q = rint(f / 2e-3)
+                      (absolute error bound 1e-3), 3-D Lorenzo residual r of q,
+                      code = 32768 + r if |r| < 512 else 0.  The 1 GiB case is
+                      the 1024 x 512 x 1024 field (2^29 codes).
 * zipf_u32         -- Zipf(s) ranks over 2^20 values mapped through a seeded
                       injective map to distinct 32-bit words with the high
                       byte in [0, 15].  Generated on the GPU.
@@ -33,11 +34,11 @@ def laplace_u16(n: int = 2_097_152, b: float = 4.0, seed: int = 0) -> np.ndarray
     return np.clip(32768 + np.round(rng.laplace(0, b, n)), 0, 65535).astype(np.uint16)
 
 
-def lorenzo_codes(shape=(1024, 512, 1024), seed: int = 1, error_bound: float = 1e-3, radius: int = 512,
-                  device: str = "cuda", slab: int = 32):
-    """int16-typed CUDA tensor holding the uint16 codes (z-major, then y, x)."""
+def sinusoid_field(shape=(1024, 512, 1024), seed: int = 1, device: str = "cuda", slab: int = 32):
+    """float32 CUDA tensor (nz, ny, nx): the sum of 64 random-phase plane
+    sinusoids (unit directions, wavelengths log-uniform in 16-256 cells)
+    plus N(0, 0.01) noise, generated in z-slabs."""
     import torch
-    import torch.nn.functional as F
 
     nz, ny, nx = shape
     rng = np.random.default_rng(seed)
@@ -51,11 +52,11 @@ def lorenzo_codes(shape=(1024, 512, 1024), seed: int = 1, error_bound: float = 1
     xs = torch.arange(nx, device=device, dtype=torch.float32)
     ys = torch.arange(ny, device=device, dtype=torch.float32)
     zs = torch.arange(nz, device=device, dtype=torch.float32)
-    out = torch.empty(nz * ny * nx, dtype=torch.int16, device=device)
-    q_prev = torch.zeros((1, ny, nx), dtype=torch.int32, device=device)
+    out = torch.empty((nz, ny, nx), dtype=torch.float32, device=device)
     for z0 in range(0, nz, slab):
         zc = zs[z0:z0 + slab]
-        f = torch.zeros((zc.numel(), ny, nx), dtype=torch.float32, device=device)
+        f = out[z0:z0 + zc.numel()]
+        f.zero_()
         for j in range(64):
             kx, ky, kz = (float(v) for v in kvec[j])
             px = (kx * xs)[None, None, :]
@@ -63,18 +64,25 @@ def lorenzo_codes(shape=(1024, 512, 1024), seed: int = 1, error_bound: float = 1
             pz = (kz * zc + float(phi[j]))[:, None, None]
             f += torch.sin(px + py + pz)
         f += 0.01 * torch.randn(f.shape, generator=gen, device=device, dtype=torch.float32)
-        q = torch.round(f / (2 * error_bound)).to(torch.int32)
-        del f
-        qp = F.pad(torch.cat([q_prev, q], 0), (1, 0, 1, 0))  # zero row/column in front of y and x
-        a = qp[1:, 1:, 1:]
-        res = (a - qp[:-1, 1:, 1:] - qp[1:, :-1, 1:] - qp[1:, 1:, :-1]
-               + qp[:-1, :-1, 1:] + qp[:-1, 1:, :-1] + qp[1:, :-1, :-1] - qp[:-1, :-1, :-1])
-        code = torch.where(res.abs() < radius, res + 32768, torch.zeros_like(res))
-        code = torch.where(code >= 32768, code - 65536, code).to(torch.int16)
-        out[z0 * ny * nx:(z0 + zc.numel()) * ny * nx] = code.reshape(-1)
-        q_prev = q[-1:].clone()
-        del q, qp, a, res, code
     return out
+
+
+def lorenzo_codes(shape=(1024, 512, 1024), seed: int = 1, error_bound: float = 1e-3, radius: int = 512,
+                  device: str = "cuda", ws=None):
+    """int16-typed CUDA tensor holding the uint16 codes (z-major, then y, x):
+    the library's Lorenzo front end (mans_lorenzo_quantize) on
+    sinusoid_field(shape, seed)."""
+    import mans_amd
+
+    f = sinusoid_field(shape, seed, device)
+    own = ws is None
+    if own:
+        ws = mans_amd.Workspace(0)
+    codes, _, _ = mans_amd.lorenzo_quantize(f, error_bound, ws, radius=radius)
+    del f
+    if own:
+        ws.close()
+    return codes
 
 
 def zipf_words(seed: int = 2) -> np.ndarray:

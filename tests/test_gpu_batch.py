@@ -231,6 +231,76 @@ def test_workspace_frozen_after_capture():
     ws.close()
 
 
+def test_batch_archive_round_trip(ws):
+    rng = np.random.default_rng(15)
+    sizes = EDGE_SIZES + [int(v) for v in np.exp(rng.uniform(np.log(1 << 10), np.log(1 << 21), 20))]
+    xs, ts = make_batch(15, sizes)
+    p = mans_amd.params(10, 64)
+    enc = mans_amd.EncodeBatch(ts, p, ws)
+    fsz = enc.run().cpu().tolist()
+    arch, d_size = enc.pack()
+    total = int(d_size.item())
+    ws.check()
+    print(f"archive of {len(xs)} frames, {total} bytes", flush=True)
+    entries = mans_amd.archive_index(arch)
+    assert len(entries) == len(xs)
+    end = 64 + 32 * len(xs)
+    for i, (off, sz, n, eb) in enumerate(entries):
+        assert off == (end + 15) // 16 * 16, i
+        assert sz == fsz[i] and n == xs[i].size and eb == xs[i].itemsize, i
+        assert torch.equal(arch[off:off + sz], enc.outs[i][:sz]), i
+        pad = (off + sz + 15) // 16 * 16 - (off + sz)
+        assert int(arch[off + sz: off + sz + pad].sum()) == 0, i
+        end = off + sz
+    assert total == (end + 15) // 16 * 16
+    dec = mans_amd.ArchiveDecodeBatch(arch, total, p, ws)
+    outs = dec.run()
+    torch.cuda.synchronize()
+    ws.check()
+    for i, x in enumerate(xs):
+        assert (outs[i].cpu().numpy().view(x.dtype) == x).all(), i
+    # encode + pack captured together, replayed
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    arch2 = torch.zeros_like(arch)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        enc.run(stream=s)
+        enc.pack(archive=arch2, d_size=d_size, stream=s)
+    for rep in range(2):
+        arch2.zero_()
+        d_size.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert int(d_size.item()) == total and torch.equal(arch2[:total], arch[:total]), rep
+    # a corrupt index is refused
+    bad = arch.clone()
+    bad[64:72] = 255
+    with pytest.raises(mans_amd.MansError):
+        mans_amd.ArchiveDecodeBatch(bad, total, p, ws)
+    dec.close()
+    enc.close()
+
+
+def test_batch_decode_reads_dims_from_headers(ws):
+    xs, ts = make_batch(16, [0, 5, 8192, 70_001, 300_000, 1_000_003])
+    p = mans_amd.params(10, 64)
+    enc = mans_amd.EncodeBatch(ts, p, ws)
+    fsz = enc.run().cpu().tolist()
+    dec = mans_amd.DecodeBatch(enc.outs, fsz, ts, p, ws, from_headers=True)
+    outs = dec.run()
+    torch.cuda.synchronize()
+    ws.check()
+    for i, x in enumerate(xs):
+        assert (outs[i].cpu().numpy().view(x.dtype) == x).all(), i
+    # a frame of another precision is refused at create
+    with pytest.raises(mans_amd.MansError):
+        mans_amd.DecodeBatch(enc.outs, fsz, ts, mans_amd.params(11, 64), ws, from_headers=True)
+    dec.close()
+    enc.close()
+
+
 SHAPES = [(64 * nw, kb) for nw in (4, 8, 16) for kb in (4, 8, 16)]
 DEC_SHAPES = [(64 * nw, kb) for nw in (4, 8, 16) for kb in (1, 4, 8, 16)]
 
