@@ -171,10 +171,11 @@ def main() -> int:
     dom_bytes = nbytes + C  # encode launch: reads the N input bytes once, writes the C-byte frame
     achieved = dom_bytes / (enc_kernel_ms * 1e-3) / 1e9
 
+    del dec, out, first
+    ref_api = reference_api_bench(x, args, ws)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(x, args)
-    del dec, out, first
     batched = batched_bench(args) if args.batch_count > 0 else None
     served = served_bench(args) if args.serve_seconds > 0 else None
 
@@ -216,12 +217,67 @@ def main() -> int:
             "traffic": measured_traffic("encode") if default_workload else None,
         },
         "cpu_baseline": cpu,
+        "reference_api": ref_api,
         "batched": batched,
         "served": served,
         "gen_s": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)
     return 0 if ok else 1
+
+
+def reference_api_bench(x, args, ws):
+    """The drop-in path on the same codes: their raw bytes as one 32-lane
+    reference-format stream, through mans_encode / mans_decode (what
+    multibyte_ans::ansEncodeBatch / ansDecodeBatch and the compress driver
+    call).  HIP events around each call on the codec stream; encode bytes
+    2N + C and decode bytes C + N for the HBM fractions."""
+    import torch
+
+    import mans_amd
+
+    xb = x.view(torch.uint8)
+    n = xb.numel()
+    p = mans_amd.params(args.prob_bits, 32)
+    ws.reserve(n, p)
+    out = torch.empty(mans_amd.max_compressed_size(n, p), dtype=torch.uint8, device="cuda")
+    back = torch.empty(n, dtype=torch.uint8, device="cuda")
+    _, csize = mans_amd.encode(xb, p, ws, out=out)
+    first = out[:csize].clone()
+    stream = torch.cuda.current_stream()
+    times = {}
+    for name, fn in (("encode", lambda: mans_amd.encode(xb, p, ws, out=out, sync_size=False)),
+                     ("decode", lambda: mans_amd.decode(out, n, p, ws, out=back, c=csize))):
+        for _ in range(args.warmup):
+            fn()
+        if name == "decode":
+            back.fill_(0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1) / args.steps
+    ok = bool(torch.equal(back, xb)) and bool(torch.equal(out[:csize], first))
+    ws.check()
+    C = int(csize)
+    te, td = times["encode"] * 1e-3, times["decode"] * 1e-3
+    del out, back, first
+    return {
+        "workload": "the same codes' raw bytes as one 32-lane reference-format stream (mans_encode / mans_decode, "
+                    f"the multibyte_ans::ansEncodeBatch / ansDecodeBatch path), p={args.prob_bits}",
+        "compressed_bytes": C,
+        "compression_ratio": round(n / C, 4),
+        "encode_GBps": round(n / te / 1e9, 2),
+        "decode_GBps": round(n / td / 1e9, 2),
+        "encode_hbm_frac": round((2 * n + C) / te / 1e9 / HBM_PEAK_GBPS, 4),
+        "decode_hbm_frac": round((C + n) / td / 1e9 / HBM_PEAK_GBPS, 4),
+        "encode_ms": round(times["encode"], 4),
+        "decode_ms": round(times["decode"], 4),
+        "bit_exact": ok,
+    }
 
 
 def batched_bench(args):
@@ -758,6 +814,15 @@ def cpu_baseline(x, args):
     td = time.perf_counter() - t
     ok = bool((d == sample).all())
     b = sample.nbytes
+    # the reference's stream format (32 lanes) on the same sample's raw bytes
+    raw = sample.view(np.uint8)
+    t = time.perf_counter()
+    rc = oracle.stream_encode(raw, args.prob_bits, oracle.FORMAT_REF, threads=cores)
+    tre = time.perf_counter() - t
+    t = time.perf_counter()
+    rd = oracle.stream_decode(rc, args.prob_bits, threads=cores)
+    trd = time.perf_counter() - t
+    ref_ok = bool((rd == raw).all())
     return {
         "value": round(b / (te + td) / 1e9, 4),
         "unit": "GB/s",
@@ -767,6 +832,8 @@ def cpu_baseline(x, args):
         "encode_GBps": round(b / te / 1e9, 4),
         "decode_GBps": round(b / td / 1e9, 4),
         "bit_exact": ok,
+        "reference_format_stream": {"encode_GBps": round(b / tre / 1e9, 4), "decode_GBps": round(b / trd / 1e9, 4),
+                                    "bit_exact": ref_ok},
     }
 
 

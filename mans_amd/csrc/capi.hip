@@ -415,12 +415,13 @@ mans_status mans_workspace_reserve(mans_workspace* ws, uint64_t max_bytes, const
     st = ensure_scratch(ws, round_up(n, 16) + adm_bits_cap(n) + 16);
     if (st != MANS_OK) return st;
   }
-  if (p && lanes == 64 && check_params(p) == MANS_OK) {
+  if (p && check_params(p) == MANS_OK) {
     // the encoder's spill areas for this element width (byte planes) and for
     // single byte streams (single streams, ADM planes)
     const int eb = p->elem_bytes == 4 ? 4 : (p->elem_bytes == 2 ? 2 : 1);
-    const uint64_t need = std::max(spill_for(ws, 64, p->prob_bits, eb, (uint32_t)eb, 0),
-                                   spill_for(ws, 64, p->prob_bits, 1, 1u, 1));
+    const uint64_t need = std::max({spill_for(ws, lanes, p->prob_bits, eb, (uint32_t)eb, 0),
+                                    spill_for(ws, lanes, p->prob_bits, 1, 1u, 0),
+                                    spill_for(ws, lanes, p->prob_bits, 1, 1u, 1)});
     st = ensure_spill(ws, need);
   }
   return st;

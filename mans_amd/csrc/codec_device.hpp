@@ -164,6 +164,69 @@ __device__ __forceinline__ void enc_step64f(uint32_t& x, uint32_t& sbase, uint32
           : "v"(xn), "v"(e.x));
 }
 
+// The 32-lane format on a 64-lane wavefront: lanes 0-31 code one unit and
+// lanes 32-63 another (halfv = 0 / 1 per lane), each unit with its own stage.
+// An emitted word of the lower half goes to sbA + 2 * rank, one of the upper
+// half to sbB + 2 * (rank - emitters of the lower half); the two bases are
+// scalars, the per-lane choice one 24-bit multiply-add:
+// emit = (sbA + 2 rank) + halfv * (sbB - sbA - 2 lo).
+__device__ __forceinline__ uint32_t emit32(uint64_t m, uint32_t& sbA, uint32_t& sbB, uint32_t halfv) {
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  uint32_t lo, hi;
+  __asm__("s_bcnt1_i32_b32 %0, %1" : "=s"(lo) : "s"((uint32_t)m) : "scc");
+  __asm__("s_bcnt1_i32_b32 %0, %1" : "=s"(hi) : "s"((uint32_t)(m >> 32)) : "scc");
+  __asm__ volatile("" : "+s"(sbA), "+s"(sbB));
+  uint32_t a2 = sbA;
+  __asm__("s_lshl1_add_u32 %0, %1, %0" : "+s"(a2) : "s"(lo) : "scc");
+  const int32_t delta = (int32_t)sbB - (int32_t)a2;
+  const uint32_t emit = (uint32_t)((int32_t)(sbA + 2u * rank) + __mul24((int32_t)halfv, delta));
+  sbA = a2;
+  __asm__("s_lshl1_add_u32 %0, %1, %0" : "+s"(sbB) : "s"(hi) : "scc");
+  return emit;
+}
+
+// enc_step64f for the 32-lane format (every lane codes a symbol).
+__device__ __forceinline__ void enc_step32f(uint32_t& x, uint32_t& sbA, uint32_t& sbB, uint32_t halfv,
+                                            const uint4& e) {
+  uint64_t m;
+  uint32_t xr;
+  __asm__(
+      "v_cmp_ge_u32_sdwa vcc, %2, %3 src0_sel:WORD_1 src1_sel:WORD_1\n\t"
+      "v_cndmask_b32_sdwa %1, %2, %2, vcc dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1\n\t"
+      "s_mov_b64 %0, vcc"
+      : "=s"(m), "=&v"(xr)
+      : "v"(x), "v"(e.x)
+      : "vcc");
+  const uint32_t emit = emit32(m, sbA, sbB, halfv);
+  uint64_t saved;
+  __asm__ volatile(
+      "s_and_saveexec_b64 %0, %1\n\t"
+      "ds_write_b16 %2, %3\n\t"
+      "s_mov_b64 exec, %0"
+      : "=&s"(saved)
+      : "s"(m), "v"(emit), "v"(x)
+      : "memory", "scc");
+  const uint32_t xn = __umul24(enc64_quot(xr, e), e.y) + xr;
+  __asm__("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+          : "=v"(x)
+          : "v"(xn), "v"(e.x));
+}
+
+// enc_step64 (lanes with v false keep their state) for the 32-lane format.
+__device__ __forceinline__ void enc_step32(uint32_t& x, uint32_t& sbA, uint32_t& sbB, uint32_t halfv, uint32_t dummy,
+                                           const uint4& e, bool v) {
+  const bool w = v && (x >> 16) >= (e.x >> 16);
+  const uint64_t m = __ballot(w);
+  uint32_t emit = emit32(m, sbA, sbB, halfv);
+  __asm__ volatile("" : "+v"(emit));
+  uint32_t addr = w ? emit : dummy;
+  __asm__ volatile("" : "+v"(addr));
+  *(lds_u16*)(uintptr_t)addr = (uint16_t)x;
+  x = w ? x >> 16 : x;
+  const uint32_t xn = __umul24(enc64_quot(x, e), e.y) + x + (e.x & 0xffffu);
+  x = v ? xn : x;
+}
+
 // 16 * byte B of w: a byte offset into a table of 16-byte entries.
 template <int B>
 __device__ __forceinline__ uint32_t byte_x16(uint32_t w) {

@@ -302,43 +302,53 @@ __global__ __launch_bounds__(64 * NW) void encode_kernel(EncodeArgs a) {
   }
 }
 
-// The 64-lane format: a ticket is BPT consecutive element blocks (CH = S *
-// BPT consecutive units: the S planes of each block), one chain per unit per
-// lane, so a lane runs several interleaved chains whatever the element
-// width.  A chain's words go to a linear LDS stage of
-// H + 256 words; once H are waiting (checked at most 4 steps apart) they go
-// to the wave's spill slot in HBM and the rest moves to the front.  When the
-// ticket is coded, the stage's remainder follows them, so the slot holds all
-// of the ticket's words, padded, and the wave publishes its aggregate.  The
-// offset is resolved only after the NEXT ticket has been coded (two slots per
-// wave, used in turn): by then the predecessors have had a whole ticket's
-// time to publish their own aggregates, so the look-back sums published
-// entries instead of waiting for the slowest wave still coding.  The copy to
-// the final place issues a batch of loads before any store.
-// (byte streams: at least 4 waves per SIMD, a few loop-invariant spills)
-template <int PB, int S, int BPT, int NW, int HB>
-__global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
-  constexpr int CH = S * BPT;       // chains per lane
+// The multi-chain coder, for both formats.  A ticket is BPT consecutive
+// element blocks per unit half (CH = S * BPT consecutive units: the S planes
+// of each block), one chain per unit per lane, so a lane runs several
+// interleaved chains whatever the element width.  In the 64-lane format a
+// wavefront codes CH units; in the 32-lane format it codes 2 CH, lanes 0-31
+// the first CH and lanes 32-63 the next CH (the units of one chain are
+// "streams" h * CH + c, h the half).  A stream's words go to a linear LDS
+// stage of H + 256 words; once H are waiting (checked at most 4 steps apart)
+// they go to the wave's spill slot in HBM and the rest moves to the front.
+// When the ticket is coded, the stage's remainder follows them, so the slot
+// holds all of the ticket's words, padded, and the wave publishes its
+// aggregate.  The offset is resolved only after the NEXT ticket has been
+// coded (two slots per wave, used in turn): by then the predecessors have
+// had a whole ticket's time to publish their own aggregates, so the
+// look-back sums published entries instead of waiting for the slowest wave
+// still coding.  The copy to the final place issues a batch of loads before
+// any store.  (32 lanes: unit order 0 or single streams only, so both halves
+// of a chain share its plane.)
+template <int L, int PB, int S, int BPT, int NW, int HB>
+__global__ __launch_bounds__(64 * NW, L == 32 ? 3 : 4) void encode_mc_kernel(EncodeArgs a) {
+  constexpr int CH = S * BPT;                     // chains per lane
+  constexpr int HALVES = L == 32 ? 2 : 1;
+  constexpr int NS = CH * HALVES;                 // streams (units) per ticket
   constexpr int KB = 8 / BPT < 4 ? 4 : 8 / BPT;  // steps per prefetch batch (elements loaded ahead per block)
-  constexpr uint32_t L = 64, BLOCK = 8192u;
+  constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  constexpr uint32_t SSTAGE = CSTAGE / HALVES;    // a unit's words at most, rounded
   constexpr uint32_t H = 1u << HB;  // spill chunk words
   constexpr uint32_t H4 = H / 8;    // spill chunk uint4s
-  constexpr uint32_t REG = H + 256u;  // LDS stage words per chain
+  constexpr uint32_t REG = H + 256u;  // LDS stage words per stream
   constexpr uint32_t SLOT = CH * CSTAGE;  // spill slot words
   constexpr int CB = 8;  // copy batch: uint4 loads per lane in flight
+  constexpr int TP = L == 32 ? S : (S > 1 ? S : kMaxPlanes);  // tables held
   static_assert(BPT == 1 || (S < 4 && BLOCK / L % KB == 0), "blocks per ticket");
   static_assert(H >= 256 && H4 <= 64, "256 <= spill chunk <= 512 words");
-  static_assert(CSTAGE % 8 == 0 && REG % 8 == 0, "uint4-aligned areas");
+  static_assert(CSTAGE % 16 == 0 && SSTAGE % 8 == 0 && REG % 8 == 0, "uint4-aligned areas");
   extern __shared__ uint4 smem[];
   __shared__ LanePlane s_pl[kMaxPlanes];
   __shared__ uint64_t s_ubase[kMaxPlanes + 1];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int P = a.d.num_planes;
+  const uint32_t half = L == 32 ? (uint32_t)(lane >> 5) : 0u;
+  const uint32_t lin = (uint32_t)lane & (L - 1);
 
   // [P][256] entries of enc64_entry, at a fixed LDS address so each chain's
   // base is an immediate offset of its reads
-  __shared__ uint4 s_tab[(S > 1 ? S : kMaxPlanes) * kNumSymbols];
+  __shared__ uint4 s_tab[TP * kNumSymbols];
   for (int i = t; i < P * (int)kNumSymbols; i += 64 * NW) {
     const uint4 e = a.table[i];
     s_tab[i] = enc64_entry(e.x, e.y, PB);
@@ -353,12 +363,12 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
   if (t == 0) s_ubase[P] = a.d.total_units;
   __syncthreads();
 
-  // staging: per wave, CH stages of REG words, then one dword-spaced dummy
+  // staging: per wave, NS stages of REG words, then one dword-spaced dummy
   // slot per lane
   const uint32_t wave_u = __builtin_amdgcn_readfirstlane((uint32_t)wave);
-  uint16_t* wstage = (uint16_t*)smem + wave_u * (CH * REG + 128);
-  const uint32_t dummy_addr = lds_addr(wstage + CH * REG + 2u * (uint32_t)lane);
-  // this wave's two spill slots of CH chains of CSTAGE words
+  uint16_t* wstage = (uint16_t*)smem + wave_u * (NS * REG + 128);
+  const uint32_t dummy_addr = lds_addr(wstage + NS * REG + 2u * (uint32_t)lane);
+  // this wave's two spill slots of NS streams of SSTAGE words
   uint16_t* const spill = a.spill + ((uint64_t)blockIdx.x * NW + wave_u) * (2 * SLOT);
   uint32_t ptot[kMaxPlanes];  // words coded per plane by this wave
 #pragma unroll
@@ -368,33 +378,33 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
   bool pend = false;
   uint32_t p_tk = 0, p_slot = 0;
   uint64_t p_agg = 0;
-  uint32_t p_pad[CH];
-  uint64_t p_bw[CH];  // byte offset of the unit's block-word pair, 0: inactive
+  uint32_t p_pad[NS];
+  uint64_t p_bw[NS];  // byte offset of the unit's block-word pair, 0: inactive
 #pragma unroll
-  for (int c = 0; c < CH; ++c) p_pad[c] = 0, p_bw[c] = 0;
+  for (int c = 0; c < NS; ++c) p_pad[c] = 0, p_bw[c] = 0;
 
   // resolve the pending ticket's offset, write its units' word offsets and
   // copy its words from its slot to their final place
   auto finish_pending = [&]() {
-    uint32_t qb[CH + 1];  // first uint4 of each chain in the ticket's output
+    uint32_t qb[NS + 1];  // first uint4 of each stream in the ticket's output
     qb[0] = 0;
 #pragma unroll
-    for (int c = 0; c < CH; ++c) qb[c + 1] = qb[c] + p_pad[c] / 8;
-    const uint32_t nq = qb[CH];
+    for (int c = 0; c < NS; ++c) qb[c + 1] = qb[c] + p_pad[c] / 8;
+    const uint32_t nq = qb[NS];
     // the slot was written by other lanes of this wave: their stores first
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     const u32x4* s4 = (const u32x4*)(spill + p_slot * SLOT);
     auto load = [&](u32x4 (&q)[CB], uint32_t j0) {
 #pragma unroll
       for (int u = 0; u < CB; ++u) {
-        if (j0 + u * L < nq) {  // wave-uniform
+        if (j0 + u * 64 < nq) {  // wave-uniform
           // lanes past the end load entry 0 (always in the slot)
-          const uint32_t j = j0 + u * L + (uint32_t)lane;
+          const uint32_t j = j0 + u * 64 + (uint32_t)lane;
           const uint32_t jj = j < nq ? j : 0u;
-          // chain k's words sit at k * CSTAGE / 8 in the slot, at qb[k] here
+          // stream k's words sit at k * SSTAGE / 8 in the slot, at qb[k] here
           uint32_t d = 0;
 #pragma unroll
-          for (int k = 1; k < CH; ++k) d = jj >= qb[k] ? k * (CSTAGE / 8) - qb[k] : d;
+          for (int k = 1; k < NS; ++k) d = jj >= qb[k] ? k * (SSTAGE / 8) - qb[k] : d;
           q[u] = s4[jj + d];
         }
       }
@@ -402,27 +412,27 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
     auto store = [&](const u32x4 (&q)[CB], u32x4* dst, uint32_t j0) {
 #pragma unroll
       for (int u = 0; u < CB; ++u) {
-        const uint32_t j = j0 + u * L + (uint32_t)lane;
+        const uint32_t j = j0 + u * 64 + (uint32_t)lane;
         if (j < nq) dst[j] = q[u];
       }
     };
     // the first batch is in flight during the look-back
     u32x4 q[CB];
     load(q, 0);
-    const uint64_t excl = resolve_prefix<4>(a.status, p_tk / CH, p_agg, lane);
+    const uint64_t excl = resolve_prefix<4>(a.status, p_tk / NS, p_agg, lane);
     uint64_t y = excl;
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
+    for (int c = 0; c < NS; ++c) {
       if (p_bw[c] && lane == 0) ((uint32_t*)(a.out + p_bw[c]))[1] = (uint32_t)y;
       y += p_pad[c];
     }
     u32x4* dst = (u32x4*)(a.out + a.d.data_off + 2ull * excl);
     store(q, dst, 0);
-    for (uint32_t j0 = CB * L; j0 < nq; j0 += CB * L) {
+    for (uint32_t j0 = CB * 64; j0 < nq; j0 += CB * 64) {
       load(q, j0);
       store(q, dst, j0);
     }
-    if ((uint64_t)p_tk + CH >= a.d.total_units && lane == 0) {
+    if ((uint64_t)p_tk + NS >= a.d.total_units && lane == 0) {
       const uint64_t total_words = excl + p_agg;
       const uint64_t size = a.d.data_off + 2ull * total_words;
       if (a.d_out_size32) *a.d_out_size32 = (uint32_t)size;
@@ -436,20 +446,23 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
 
   uint32_t slot = 0;
   uint32_t tk = 0;
-  if (lane == 0) tk = atomicAdd(a.ticket, CH);
+  if (lane == 0) tk = atomicAdd(a.ticket, NS);
   tk = __builtin_amdgcn_readfirstlane(tk);
   while ((uint64_t)tk < a.d.total_units) {
-    uint32_t x[CH], n[CH], fl[CH];
-    uint32_t cb[CH], sb[CH];  // LDS byte address of the stage and of the next word
+    // per chain (this lane's half): unit, plane, block, symbols
+    uint32_t x[CH], n[CH];
     const uint4* tab[CH];
     uint64_t blk[CH];
     int kp[CH];
     bool act[CH];
     const uint8_t* src[CH];
+    // per stream: LDS byte address of the stage and of the next word, words
+    // moved to the slot
+    uint32_t cb[NS], sb[NS], fl[NS];
     uint16_t* const wslot = spill + slot * SLOT;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      const uint64_t unit = (uint64_t)tk + c;
+      const uint64_t unit = (uint64_t)tk + half * CH + c;
       act[c] = unit < a.d.total_units;
       int k = 0;
       uint64_t b = 0;
@@ -467,14 +480,18 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
       const LanePlane pl = s_pl[k];
       const uint64_t start = b * BLOCK;
       n[c] = act[c] ? (uint32_t)((pl.nsym - start) < BLOCK ? (pl.nsym - start) : BLOCK) : 0u;
-      n[c] = __builtin_amdgcn_readfirstlane(n[c]);
+      if (L == 64) n[c] = __builtin_amdgcn_readfirstlane(n[c]);
       src[c] = pl.src;
       x[c] = kMinState;
+      // S > 1 only on element arrays of S planes: chain c is plane c % S
+      // (both halves: CH is a multiple of P)
+      tab[c] = s_tab + (S > 1 ? c % S : (L == 32 ? 0 : k)) * kNumSymbols;
+    }
+#pragma unroll
+    for (int c = 0; c < NS; ++c) {
       fl[c] = 0;
       cb[c] = lds_addr(wstage + c * REG);
       sb[c] = cb[c];
-      // S > 1 only on element arrays of S planes: chain c is plane c % S
-      tab[c] = s_tab + (S > 1 ? c % S : k) * kNumSymbols;
     }
     uint32_t n_min = n[0], n_max = n[0];
 #pragma unroll
@@ -482,31 +499,45 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
       n_min = n[c] < n_min ? n[c] : n_min;
       n_max = n[c] > n_max ? n[c] : n_max;
     }
+    if (L == 32) {
+      const uint32_t a0 = __builtin_amdgcn_readlane(n_min, 0), a1 = __builtin_amdgcn_readlane(n_min, 32);
+      const uint32_t b0 = __builtin_amdgcn_readlane(n_max, 0), b1 = __builtin_amdgcn_readlane(n_max, 32);
+      n_min = a0 < a1 ? a0 : a1;
+      n_max = b0 > b1 ? b0 : b1;
+    }
     const uint32_t all = (uint32_t)div_up(n_max, L);
 
-    // stage upkeep: a chain with H or more unspilled words moves its first H
-    // to the slot and the rest (fewer than 256: at most 64 words per step,
+    // stage upkeep: a stream with H or more unspilled words moves its first
+    // H to the slot and the rest (fewer than 256: at most 64 words per step,
     // at most 4 steps apart) to the front
     auto spill_full = [&]() {
 #pragma unroll
-      for (int c = 0; c < CH; ++c) {
+      for (int c = 0; c < NS; ++c) {
         const uint32_t un = (sb[c] - cb[c]) >> 1;  // unspilled words
         if (un >= H) {
           uint16_t* st = wstage + c * REG;
-          if ((uint32_t)lane < H4) ((uint4*)(wslot + c * CSTAGE + fl[c]))[lane] = ((const uint4*)st)[lane];
+          if ((uint32_t)lane < H4) ((uint4*)(wslot + c * SSTAGE + fl[c]))[lane] = ((const uint4*)st)[lane];
           if (4u * (uint32_t)lane < un - H) ((uint2*)st)[lane] = ((const uint2*)(st + H))[lane];
           fl[c] += H;
           sb[c] -= 2u * H;
         }
       }
     };
+    auto step_full = [&](int c, const uint4& e) {
+      if constexpr (L == 64) enc_step64f(x[c], sb[c], dummy_addr, e);
+      else enc_step32f(x[c], sb[c], sb[CH + c], half, e);
+    };
+    auto step_part = [&](int c, const uint4& e, bool v) {
+      if constexpr (L == 64) enc_step64(x[c], sb[c], dummy_addr, e, v);
+      else enc_step32(x[c], sb[c], sb[CH + c], half, dummy_addr, e, v);
+    };
 
     {
       // Per step each lane takes one element of each of the ticket's BPT
-      // blocks; byte b of block d's element feeds chain d * S + b.  With
-      // S > 1 the first chain of a block is its plane 0, whose pointer is the
-      // element array; with S == 1 each chain has its own byte stream
-      // (single streams, concatenated planes).
+      // blocks (of its half); byte b of block d's element feeds chain
+      // d * S + b.  With S > 1 the first chain of a block is its plane 0,
+      // whose pointer is the element array; with S == 1 each chain has its
+      // own byte stream (single streams, concatenated planes).
       const uint8_t* eb[BPT];
 #pragma unroll
       for (int d = 0; d < BPT; ++d) eb[d] = src[d * S] + blk[d * S] * BLOCK * S;
@@ -518,7 +549,7 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
       auto load = [&](uint32_t (&buf)[BPT][KB], uint32_t s0) {
 #pragma unroll
         for (int d = 0; d < BPT; ++d) {
-          const gptr<EL> bp = (gptr<EL>)eb[d] + lane + (uint64_t)s0 * L;
+          const gptr<EL> bp = (gptr<EL>)eb[d] + lin + (uint64_t)s0 * L;
 #pragma unroll
           for (int j = 0; j < KB; ++j) buf[d][j] = bp[j * L];
         }
@@ -547,7 +578,7 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
           uint4 en[CH];
           if (j + 1 < KB) lookup(en, buf, j + 1);
 #pragma unroll
-          for (int c = 0; c < CH; ++c) enc_step64f(x[c], sb[c], dummy_addr, e[c]);
+          for (int c = 0; c < CH; ++c) step_full(c, e[c]);
           if ((j & 3) == 3) spill_full();
           if (j + 1 < KB) {
 #pragma unroll
@@ -577,55 +608,80 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
         }
       }
       for (; s < all; ++s) {
-        const uint32_t i = s * L + (uint32_t)lane;
+        const uint32_t i = s * L + lin;
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
           const bool v = i < n[c];
           const uint32_t sym = v ? (uint32_t)eb[c / S][(uint64_t)i * S + (c % S)] : 0u;
-          enc_step64(x[c], sb[c], dummy_addr, tab[c][sym], v);
+          step_part(c, tab[c][sym], v);
         }
         spill_full();
       }
     }
 
-    // ---- per chain: pad, move the stage's remainder to the slot, states and
-    // word count; ticket aggregate
+    // ---- per stream: pad, move the stage's remainder to the slot; per chain
+    // and half: states and block words; ticket aggregate
     uint64_t agg = 0;
-    uint32_t pad[CH];
-    uint64_t bwo[CH];
+    uint32_t pad[NS];
+    uint64_t bwo[NS];
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
+    for (int c = 0; c < NS; ++c) {
       const uint32_t rem = (sb[c] - cb[c]) >> 1;  // words still in the stage
       const uint32_t cnt = fl[c] + rem;
       const uint32_t padded = (uint32_t)round_up(cnt, 8);
-      if (cnt > CSTAGE && lane == 0) atomicOr(a.err, kErrStageOverflow);
+      if (cnt > SSTAGE && lane == 0) atomicOr(a.err, kErrStageOverflow);
       uint16_t* st = wstage + c * REG;
       if ((uint32_t)lane < padded - cnt) st[rem + lane] = 0;
       const uint32_t nr = (padded - fl[c]) / 8;
-      for (uint32_t j = lane; j < nr; j += L) ((uint4*)(wslot + c * CSTAGE + fl[c]))[j] = ((const uint4*)st)[j];
-      const LanePlane& pl = s_pl[kp[c]];
-      if (act[c]) ((uint32_t*)(a.out + pl.section + kSectionFixed))[blk[c] * L + lane] = x[c];
-      bwo[c] = act[c] ? pl.section + kSectionFixed + 4ull * L * pl.nb + 8ull * blk[c] : 0ull;
-      if (act[c] && lane == 0) ((uint32_t*)(a.out + bwo[c]))[0] = (n[c] << 16) | cnt;
-      pad[c] = act[c] ? padded : 0u;
-      agg += pad[c];
-#pragma unroll
-      for (int k = 0; k < kMaxPlanes; ++k) ptot[k] += kp[c] == k ? pad[c] : 0u;
+      for (uint32_t j = lane; j < nr; j += 64) ((uint4*)(wslot + c * SSTAGE + fl[c]))[j] = ((const uint4*)st)[j];
+      pad[c] = padded;  // set to 0 below for inactive units
     }
-    publish_aggregate(a.status, tk / CH, agg, lane);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const LanePlane& pl = s_pl[kp[c]];
+      if (act[c]) ((uint32_t*)(a.out + pl.section + kSectionFixed))[blk[c] * L + lin] = x[c];
+      // this lane's half: stream h * CH + c
+      uint32_t cnt_h = 0;
+#pragma unroll
+      for (int h = 0; h < HALVES; ++h) {
+        const int sidx = h * CH + c;
+        const uint32_t cnt = fl[sidx] + ((sb[sidx] - cb[sidx]) >> 1);
+        cnt_h = (uint32_t)h == half ? cnt : cnt_h;
+      }
+      const uint64_t bw = pl.section + kSectionFixed + 4ull * L * pl.nb + 8ull * blk[c];
+      if (act[c] && lin == 0) ((uint32_t*)(a.out + bw))[0] = (n[c] << 16) | cnt_h;
+#pragma unroll
+      for (int h = 0; h < HALVES; ++h) {
+        const int sidx = h * CH + c;
+        const uint32_t src_lane = (uint32_t)h * 32u;
+        const bool ah = L == 32 ? (bool)__builtin_amdgcn_readlane((uint32_t)act[c], src_lane) : act[c];
+        const uint32_t kh = L == 32 ? __builtin_amdgcn_readlane((uint32_t)kp[c], src_lane) : (uint32_t)kp[c];
+        const uint64_t bwh =
+            L == 32 ? ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(bw >> 32), src_lane) << 32) |
+                          __builtin_amdgcn_readlane((uint32_t)bw, src_lane)
+                    : bw;
+        pad[sidx] = ah ? pad[sidx] : 0u;
+        bwo[sidx] = ah ? bwh : 0ull;
+#pragma unroll
+        for (int k = 0; k < kMaxPlanes; ++k) ptot[k] += kh == (uint32_t)k ? pad[sidx] : 0u;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NS; ++c) agg += pad[c];
+    publish_aggregate(a.status, tk / NS, agg, lane);
     if (pend) finish_pending();
     pend = true;
     p_tk = tk;
     p_slot = slot;
     p_agg = agg;
 #pragma unroll
-    for (int c = 0; c < CH; ++c) p_pad[c] = pad[c], p_bw[c] = bwo[c];
+    for (int c = 0; c < NS; ++c) p_pad[c] = pad[c], p_bw[c] = bwo[c];
     slot ^= 1u;
     // the next ticket only once this one is published: tickets start in
     // ticket order, and every ticket a look-back waits on belongs to a wave
     // that is coding it
     __threadfence_block();
-    if (lane == 0) tk = atomicAdd(a.ticket, CH);
+    if (lane == 0) tk = atomicAdd(a.ticket, NS);
     tk = __builtin_amdgcn_readfirstlane(tk);
   }
   if (pend) finish_pending();
@@ -637,65 +693,90 @@ __global__ __launch_bounds__(64 * NW, 4) void encode64_kernel(EncodeArgs a) {
   }
 }
 
-template <int L, int PB, int S, int CH, int NW, int HB>
-hipError_t launch_t(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
+// The whole-unit staged 32-lane kernel (encode_kernel): CH == S chains, no
+// spill areas.  The multi-chain kernel (encode_mc_kernel): BPT blocks per
+// half-wave ticket, spill areas.
+template <int L, int PB, int S, int CH, int NW>
+hipError_t launch_staged(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
-  constexpr uint32_t REG = L == 64 ? (1u << HB) + 256u : CSTAGE;
-  // dynamic LDS: the staging areas, after the tables (static in the 64-lane kernel)
-  const size_t lds = (L == 64 ? 0 : (size_t)a.d.num_planes * kNumSymbols * sizeof(uint4)) +
-                     (size_t)NW * (CH * REG + 128) * 2u;
+  const size_t lds = (size_t)a.d.num_planes * kNumSymbols * sizeof(uint4) + (size_t)NW * (CH * CSTAGE + 128) * 2u;
   int per_cu = 0;
-  hipError_t e;
-  if constexpr (L == 64)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode64_kernel<PB, S, CH / S, NW, HB>, 64 * NW, lds);
-  else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S, CH, NW>, 64 * NW, lds);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_kernel<L, PB, S, CH, NW>, 64 * NW, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
-  constexpr uint32_t UPW = CH * (L == 32 ? 2u : 1u);
-  const uint64_t waves = div_up(a.d.total_units, UPW);
+  const uint64_t waves = div_up(a.d.total_units, CH * 2u);
   uint64_t grid = div_up(waves, NW);
   const uint64_t cap = (uint64_t)cu * (uint64_t)per_cu;
   if (grid > cap) grid = cap;
   if (grid < 1) grid = 1;
-  // two slots per wave of CH chains (64 lanes)
-  const uint64_t spill_bytes = L == 64 ? grid * NW * 2ull * CH * CSTAGE * 2ull : 0ull;
+  if (need_spill) {
+    *need_spill = 0;
+    return hipSuccess;
+  }
+  hipLaunchKernelGGL((encode_kernel<L, PB, S, CH, NW>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+  return hipGetLastError();
+}
+
+template <int L, int PB, int S, int BPT, int NW, int HB>
+hipError_t launch_mc(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  constexpr int CH = S * BPT, NS = CH * (L == 32 ? 2 : 1);
+  constexpr uint32_t REG = (1u << HB) + 256u;
+  // dynamic LDS: the staging areas (the tables are static)
+  const size_t lds = (size_t)NW * (NS * REG + 128) * 2u;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, encode_mc_kernel<L, PB, S, BPT, NW, HB>,
+                                                              64 * NW, lds);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  const uint64_t waves = div_up(a.d.total_units, NS);
+  uint64_t grid = div_up(waves, NW);
+  const uint64_t cap = (uint64_t)cu * (uint64_t)per_cu;
+  if (grid > cap) grid = cap;
+  if (grid < 1) grid = 1;
+  // two slots per wave of CH chains of 64 lanes
+  const uint64_t spill_bytes = grid * NW * 2ull * CH * CSTAGE * 2ull;
   if (need_spill) {
     *need_spill = spill_bytes;
     return hipSuccess;
   }
-  if (spill_bytes > a.spill_bytes) return hipErrorInvalidValue;
-  if constexpr (L == 64)
-    hipLaunchKernelGGL((encode64_kernel<PB, S, CH / S, NW, HB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
-  else
-    hipLaunchKernelGGL((encode_kernel<L, PB, S, CH, NW>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+  if (!a.spill || spill_bytes > a.spill_bytes) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((encode_mc_kernel<L, PB, S, BPT, NW, HB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
+// staged: the 32-lane whole-unit kernel (no spill areas: the allocation-free
+// entry point, and ADM planes concatenated at 32 lanes).
 template <int L, int PB>
-hipError_t launch_s(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
+hipError_t launch_s(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill, bool staged) {
   const int P = a.d.num_planes;
   const uint32_t S = a.d.plane[0].stride;
   if constexpr (L == 64) {
     // four chains per lane, linear stages of 256 + 256 words: 4.3 KiB of
     // LDS per wave
-    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 4, 4, 8>(a, cu, s, need_spill);
-    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 4, 8>(a, cu, s, need_spill);
-    if (S == 1) return launch_t<L, PB, 1, 4, 4, 8>(a, cu, s, need_spill);
+    if (a.d.order == 0 && P == 2 && S == 2) return launch_mc<64, PB, 2, 2, 4, 8>(a, cu, s, need_spill);
+    if (a.d.order == 0 && P == 4 && S == 4) return launch_mc<64, PB, 4, 1, 4, 8>(a, cu, s, need_spill);
+    if (S == 1) return launch_mc<64, PB, 1, 4, 4, 8>(a, cu, s, need_spill);
   } else {
-    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 2, 0>(a, cu, s, need_spill);
-    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 1, 0>(a, cu, s, need_spill);
-    if (S == 1) return launch_t<L, PB, 1, 1, 4, 0>(a, cu, s, need_spill);
+    if (!staged && a.d.order == 0) {
+      // four chains per lane, two units per chain (one per half-wave)
+      if (P == 2 && S == 2) return launch_mc<32, PB, 2, 2, 4, 8>(a, cu, s, need_spill);
+      if (P == 4 && S == 4) return launch_mc<32, PB, 4, 1, 4, 8>(a, cu, s, need_spill);
+      if (P == 1 && S == 1) return launch_mc<32, PB, 1, 4, 4, 8>(a, cu, s, need_spill);
+    }
+    if (a.d.order == 0 && P == 2 && S == 2) return launch_staged<L, PB, 2, 2, 2>(a, cu, s, need_spill);
+    if (a.d.order == 0 && P == 4 && S == 4) return launch_staged<L, PB, 4, 4, 1>(a, cu, s, need_spill);
+    if (S == 1) return launch_staged<L, PB, 1, 1, 4>(a, cu, s, need_spill);
   }
   return hipErrorInvalidValue;
 }
 
 template <int L>
-hipError_t launch_p(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
+hipError_t launch_p(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill, bool staged) {
   switch (a.d.prob_bits) {
-    case 9: return launch_s<L, 9>(a, cu, s, need_spill);
-    case 10: return launch_s<L, 10>(a, cu, s, need_spill);
-    case 11: return launch_s<L, 11>(a, cu, s, need_spill);
+    case 9: return launch_s<L, 9>(a, cu, s, need_spill, staged);
+    case 10: return launch_s<L, 10>(a, cu, s, need_spill, staged);
+    case 11: return launch_s<L, 11>(a, cu, s, need_spill, staged);
   }
   return hipErrorInvalidValue;
 }
@@ -705,12 +786,14 @@ hipError_t launch_p(const EncodeArgs& a, int cu, hipStream_t s, uint64_t* need_s
 hipError_t encode_spill_bytes(const EncodeArgs& a, int cu, uint64_t* bytes) {
   *bytes = 0;
   if (a.d.total_units == 0) return hipSuccess;
-  return a.d.lanes == 64 ? launch_p<64>(a, cu, nullptr, bytes) : launch_p<32>(a, cu, nullptr, bytes);
+  return a.d.lanes == 64 ? launch_p<64>(a, cu, nullptr, bytes, false) : launch_p<32>(a, cu, nullptr, bytes, false);
 }
 
+// a.spill == nullptr selects the 32-lane whole-unit kernel (no spill areas).
 hipError_t launch_encode(const EncodeArgs& a, int cu, hipStream_t s) {
   if (a.d.total_units == 0) return hipSuccess;
-  return a.d.lanes == 64 ? launch_p<64>(a, cu, s, nullptr) : launch_p<32>(a, cu, s, nullptr);
+  if (a.d.lanes == 64) return launch_p<64>(a, cu, s, nullptr, false);
+  return launch_p<32>(a, cu, s, nullptr, a.spill == nullptr);
 }
 
 }  // namespace mans
