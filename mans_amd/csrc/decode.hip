@@ -48,7 +48,8 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
   constexpr int BPT = CH / S;  // element blocks per wave iteration
   constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
   constexpr uint32_t HALVES = L == 32 ? 2u : 1u;
-  constexpr uint32_t UPW = CH * HALVES;
+  constexpr uint32_t UPW = CH * HALVES;  // units per wave iteration: one stream each
+  constexpr int NSTR = CH * (int)HALVES;
   constexpr uint32_t W = 1u << PB;
   constexpr bool RING = RB > 0;
   constexpr uint32_t R = RING ? 1u << RB : 0u;  // ring words
@@ -59,10 +60,11 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
   constexpr uint32_t HSTAGE = CSTAGE / HALVES;
   constexpr uint32_t REG = RING ? R + 64u : CSTAGE;  // LDS words per chain
   constexpr uint32_t RK = H >= 512 ? 4u : 2u;          // steps between ring checks
-  static_assert(!RING || 2u * 64u * RK <= H, "a chunk's load starts a check before its commit");
+  static_assert(!RING || 2u * L * RK <= H, "a chunk's load starts a check before its commit");
   constexpr uint32_t UNIT_CAP = L * lane_max_words(BLOCK / L, PB);
-  static_assert(CH % S == 0 && (L == 64 || CH == S) && CH * 8 <= 32, "chain d * S + b decodes byte b of block d");
-  static_assert(!RING || (L == 64 && H >= 128 && H4 <= 64), "ring: 64 lanes, 128 <= R/2 <= 512 words");
+  static_assert(CH % S == 0 && (L == 64 || RING || CH == S) && CH * 8 <= 32,
+                "chain d * S + b decodes byte b of block d");
+  static_assert(!RING || (H >= 128 && H4 <= 64), "ring: 128 <= R/2 <= 512 words");
   extern __shared__ uint4 smem[];
   __shared__ DPlane s_pl[kMaxPlanes];
   __shared__ uint64_t s_ubase[kMaxPlanes + 1];
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
   // read one slot past its unit's words)
   const uint32_t wave_u = __builtin_amdgcn_readfirstlane((uint32_t)wave);
   uint16_t* wstage =
-      (uint16_t*)(smem + div_up((uint64_t)P * W * 8u, 16u)) + wave_u * (CH * REG + (RING ? 0u : 64u));
+      (uint16_t*)(smem + div_up((uint64_t)P * W * 8u, 16u)) + wave_u * ((RING ? NSTR : CH) * REG + (RING ? 0u : 64u));
   const uint64_t total_units = s_total_units;
   const uint64_t data_off = s_data_off, data_words = s_data_words;
   const uint64_t nwaves = (uint64_t)gridDim.x * NW;
@@ -196,12 +198,14 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
     uint64_t blk[CH];
     const uint2* lut[CH];
     const uint16_t* stage[CH];
-    // ring state (wave-uniform): lowest word index held in LDS, whether a
-    // refill is in flight, and its registers
-    uint32_t lo[CH];
-    bool pend[CH];
-    uint4 rv[CH];
-    const uint4* src4[CH];
+    // ring state per stream (unit), wave-uniform: lowest word index held in
+    // LDS, whether a refill is in flight, and its registers
+    uint32_t lo[NSTR];
+    bool pend[NSTR];
+    uint4 rv[NSTR];
+    const uint4* src4[CH];  // this lane's unit (staged units)
+    const uint4* ssrc4[NSTR];  // each stream's words (rings)
+    uint32_t swords[NSTR];
     uint8_t* dstp[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
@@ -248,28 +252,16 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
         y = __builtin_amdgcn_readfirstlane(y);
       }
       src4[c] = (const uint4*)(a.in + data_off + 2ull * y);  // y is a multiple of 8 words
-      uint16_t* reg = wstage + c * REG + (RING ? 0u : half * HSTAGE);
+      // a ring per stream h * CH + c; whole units at half * HSTAGE of the chain's area
+      uint16_t* reg = RING ? wstage + (half * CH + c) * REG : wstage + c * REG + half * HSTAGE;
       stage[c] = reg;
-      lo[c] = 0;
-      pend[c] = false;
-      rv[c] = make_uint4(0, 0, 0, 0);
       if constexpr (RING) {
-        // the top one or two chunks of the unit: chunk j holds words
-        // [j H, (j + 1) H) at ring half j & 1; half 0's first 64 words are
-        // mirrored after the ring
-        if (words > 0) {
-          const uint32_t jt = (words - 1) / H;
-          const uint32_t j0 = jt > 0 ? jt - 1 : 0;
-          lo[c] = j0 * H;
-          for (uint32_t j = j0; j <= jt; ++j) {
-            const uint32_t left = words - j * H;
-            const uint32_t n4 = (uint32_t)div_up(left < H ? left : H, 8);
-            if ((uint32_t)lane < n4) {
-              const uint4 q = src4[c][j * H4 + lane];
-              ((uint4*)(reg + (j & 1) * H))[lane] = q;
-              if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + R))[lane] = q;
-            }
-          }
+#pragma unroll
+        for (int h = 0; h < (int)HALVES; ++h) {
+          const int sidx = h * CH + c;
+          swords[sidx] = L == 32 ? __builtin_amdgcn_readlane(words, (uint32_t)h * 32u) : words;
+          const uint32_t yh = L == 32 ? __builtin_amdgcn_readlane(y, (uint32_t)h * 32u) : y;
+          ssrc4[sidx] = (const uint4*)(a.in + data_off + 2ull * yh);
         }
       } else {
         // whole unit -> LDS (16-byte loads)
@@ -281,6 +273,33 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
       ptr[c] = words;
       wds[c] = words;
       taken[c] = 0;
+    }
+    if constexpr (RING) {
+      // the top one or two chunks of each stream: chunk j holds words
+      // [j H, (j + 1) H) at ring half j & 1; half 0's first 64 words are
+      // mirrored after the ring (all 64 lanes load each stream's chunks)
+#pragma unroll
+      for (int sidx = 0; sidx < NSTR; ++sidx) {
+        uint16_t* reg = wstage + sidx * REG;
+        const uint32_t words = swords[sidx];
+        lo[sidx] = 0;
+        pend[sidx] = false;
+        rv[sidx] = make_uint4(0, 0, 0, 0);
+        if (words > 0) {
+          const uint32_t jt = (words - 1) / H;
+          const uint32_t j0 = jt > 0 ? jt - 1 : 0;
+          lo[sidx] = j0 * H;
+          for (uint32_t j = j0; j <= jt; ++j) {
+            const uint32_t left = words - j * H;
+            const uint32_t n4 = (uint32_t)div_up(left < H ? left : H, 8);
+            if ((uint32_t)lane < n4) {
+              const uint4 q = ssrc4[sidx][j * H4 + lane];
+              ((uint4*)(reg + (j & 1) * H))[lane] = q;
+              if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + R))[lane] = q;
+            }
+          }
+        }
+      }
     }
     // wait for the state loads here, once: a first use inside the step loop
     // would put a wait for all outstanding loads (the ring refills) and
@@ -298,19 +317,22 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
     auto refill = [&]() {
       if constexpr (RING) {
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-          if (!pend[c] && lo[c] > 0 && (int32_t)ptr[c] <= (int32_t)(lo[c] + H)) {
-            const uint32_t j = lo[c] / H - 1;
-            if ((uint32_t)lane < H4) rv[c] = src4[c][j * H4 + lane];
-            pend[c] = true;
+        for (int sidx = 0; sidx < NSTR; ++sidx) {
+          const int c = sidx % CH;
+          const int32_t p = (int32_t)(L == 32 ? __builtin_amdgcn_readlane(ptr[c], (uint32_t)(sidx / CH) * 32u)
+                                              : ptr[c]);
+          if (!pend[sidx] && lo[sidx] > 0 && p <= (int32_t)(lo[sidx] + H)) {
+            const uint32_t j = lo[sidx] / H - 1;
+            if ((uint32_t)lane < H4) rv[sidx] = ssrc4[sidx][j * H4 + lane];
+            pend[sidx] = true;
           }
-          if (pend[c] && (int32_t)ptr[c] < (int32_t)(lo[c] + 64u * RK)) {
-            const uint32_t j = lo[c] / H - 1;
-            uint16_t* reg = wstage + c * REG;
-            if ((uint32_t)lane < H4) ((uint4*)(reg + (j & 1) * H))[lane] = rv[c];
-            if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + R))[lane] = rv[c];
-            lo[c] -= H;
-            pend[c] = false;
+          if (pend[sidx] && p < (int32_t)(lo[sidx] + L * RK)) {
+            const uint32_t j = lo[sidx] / H - 1;
+            uint16_t* reg = wstage + sidx * REG;
+            if ((uint32_t)lane < H4) ((uint4*)(reg + (j & 1) * H))[lane] = rv[sidx];
+            if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + R))[lane] = rv[sidx];
+            lo[sidx] -= H;
+            pend[sidx] = false;
           }
         }
       }
@@ -394,8 +416,9 @@ template <int L, int PB, int S, int CH, int NW, int RB>
 hipError_t launch_t(const DecodeArgs& a, int cu, hipStream_t s) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
   constexpr uint32_t REG = RB > 0 ? (1u << RB) + 64u : CSTAGE;
+  constexpr uint32_t NSTR = CH * (L == 32 ? 2u : 1u);
   const size_t lds = (size_t)round_up((uint64_t)a.d.num_planes * (1u << PB) * 8u, 16u) +
-                     (size_t)NW * (CH * REG + (RB > 0 ? 0u : 64u)) * 2u;
+                     (size_t)NW * ((RB > 0 ? NSTR : CH) * REG + (RB > 0 ? 0u : 64u)) * 2u;
   int per_cu = 0;
   hipError_t e =
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decode_kernel<L, PB, S, CH, NW, RB>, 64 * NW, lds);
@@ -424,10 +447,12 @@ hipError_t launch_s(const DecodeArgs& a, int cu, hipStream_t s) {
     if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 8, 9>(a, cu, s);
     if (S == 1) return launch_t<L, PB, 1, 1, 8, 10>(a, cu, s);
   } else {
-    // whole-unit staging; wavefronts per group chosen so LUTs + staging fit
-    // 64 KiB at p <= 10
-    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 2, 0>(a, cu, s);
-    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 1, 0>(a, cu, s);
+    // two units per chain (one per half-wave), a ring of 1024 words per unit
+    if (a.d.order == 0 && P == 2 && S == 2) return launch_t<L, PB, 2, 2, 4, 10>(a, cu, s);
+    if (a.d.order == 0 && P == 4 && S == 4) return launch_t<L, PB, 4, 4, 2, 9>(a, cu, s);
+    if (a.d.order == 0 && P == 1 && S == 1) return launch_t<L, PB, 1, 2, 4, 10>(a, cu, s);
+    // concatenated planes (ADM frames): whole-unit staging; wavefronts per
+    // group chosen so LUTs + staging fit 64 KiB at p <= 10
     if (S == 1) return launch_t<L, PB, 1, 1, 4, 0>(a, cu, s);
   }
   return hipErrorInvalidValue;

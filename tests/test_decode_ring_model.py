@@ -11,7 +11,9 @@ word a step reads sits in the slot (ring index, or mirror) it reads:
   64 KB)): fetch right after the previous write, write once ptr <= lo + H;
 * single-buffer decode, checks at most RK = 4 steps apart (decode.hip; the
   model checks only every RK steps, the sparsest case): fetch once
-  ptr <= lo + H, write when ptr < lo + 64 RK, H = 512."""
+  ptr <= lo + H, write when ptr < lo + L RK, H = 512 (or 256 for four
+  chains), L = 64 lanes, or L = 32 for the reference format's units (at most
+  32 words per step, one ring per unit of a half-wave)."""
 import random
 
 import pytest
@@ -85,17 +87,17 @@ def run(cons, full, kb, early, h, commit_gap):
     assert st["ptr"] == 0
 
 
-def cases(seed, n):
+def cases(seed, n, lanes=64):
     rng = random.Random(seed)
     for _ in range(n):
         mode = rng.random()
         if mode < 0.3:
-            cons = [64] * 128
+            cons = [lanes] * 128
         elif mode < 0.6:
             p = rng.random()
-            cons = [sum(rng.random() < p for _ in range(64)) for _ in range(128)]
+            cons = [sum(rng.random() < p for _ in range(lanes)) for _ in range(128)]
         else:
-            cons = [rng.choice([0, 64, rng.randrange(65)]) for _ in range(128)]
+            cons = [rng.choice([0, lanes, rng.randrange(lanes + 1)]) for _ in range(128)]
         yield cons, rng.choice([0, 128, rng.randrange(129)])
 
 
@@ -109,6 +111,12 @@ def test_batched_ring_schedules(kb):
 def test_single_buffer_ring_schedule():
     for cons, full in cases(99, 150):
         run(cons, full, 4, early=False, h=512, commit_gap=64 * 4)
+
+
+@pytest.mark.parametrize("h", [512, 256])
+def test_single_buffer_ring_schedule_32_lanes(h):
+    for cons, full in cases(31 + h, 150, lanes=32):
+        run(cons, full, 4, early=False, h=h, commit_gap=32 * 4)
 
 
 def test_model_catches_a_late_write():
