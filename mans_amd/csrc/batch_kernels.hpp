@@ -43,6 +43,14 @@ constexpr uint64_t kValue48 = (1ull << kGenShift) - 1;
 constexpr uint32_t kTabWords = 2 * 256 * 4;
 constexpr uint32_t kRingWords = 2 * kR / 2;
 constexpr uint32_t kWaveWords = kTabWords + kRingWords + 64;
+// Queue items a decode wavefront takes at once: consecutive tickets of a
+// buffer then run on one wavefront, which keeps the buffer's LUTs in its
+// LDS.  (Encode takes one at a time: a ticket's look-back waits for its
+// predecessor's count, and with runs that predecessor would be the last of
+// another wavefront's run, so the runs of a buffer would follow one another.)
+// An item waits only on lower items, each held by a running wavefront that
+// reaches it, so the launch drains.
+constexpr uint32_t kRun = 4;
 
 template <typename T>
 using gptr = const __attribute__((address_space(1))) T*;
@@ -96,49 +104,80 @@ __device__ uint64_t look_back_gen(uint64_t* status, uint64_t idx, uint64_t first
 
 // ---- histogram item -------------------------------------------------------
 // Counts bytes [c0, c1) of buffer B (plane = byte offset % P) into this
-// wave's LDS counters [P][256][REP] (REP = 8 / P replicas, lane l adds into
-// replica l % REP), then adds them to the buffer's accumulators.
+// wave's LDS counters and adds them to the buffer's accumulators.  The
+// counters are 16-bit pairs: [P][128 bin pairs][16 / P replicas] words (bins
+// 2i and 2i + 1 in the low and high half of word i), lane l adding into
+// replica l % (16 / P).  A replica counts at most 64 / (16 / P) lanes x
+// chunk / 64 / P symbols of a plane (16384 for 256 KiB chunks), so the halves
+// cannot carry into each other.  Loads: eight 16-byte loads per lane in
+// flight while the previous eight are counted.
+static_assert(kBatchHistChunk / 16u < 65536u, "16-bit pair counters");
+static_assert(4u * 128u * (16u / 4u) <= kTabWords, "counters fit the wave's table area");
+constexpr int kHistLoads = 8;
+
 __device__ void hist_chunk(const BatchBuf& B, uint32_t b, uint64_t c0, uint64_t c1, uint32_t* cnt,
                            uint32_t* ghist) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t P = B.eb;
-  const uint32_t rep = 8u / P;
-  const uint32_t r = lane % rep;
+  const uint32_t rep = 16u / P;
+  uint32_t* c = cnt + lane % rep;
   const uint4* p4 = (const uint4*)(B.in + c0);
   const uint64_t n16 = (c1 - c0) / 16u;
+  auto add = [&](uint32_t plane, uint32_t byte) {
+    atomicAdd(c + (plane * 128u + (byte >> 1)) * rep, (byte & 1u) ? 0x10000u : 1u);
+  };
   auto count16 = [&](const uint4& v) {
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const uint32_t plane = (uint32_t)(4 * k + m) % P;
-        const uint32_t byte = __builtin_amdgcn_ubfe(w[k], 8u * (uint32_t)m, 8u);
-        atomicAdd(cnt + (plane * 256u + byte) * rep + r, 1u);
-      }
+      for (int m = 0; m < 4; ++m) add((uint32_t)(4 * q + m) % P, __builtin_amdgcn_ubfe(w[q], 8u * (uint32_t)m, 8u));
   };
-  uint64_t i = lane;
-  for (; i + 192 < n16; i += 256) {
-    const uint4 a0 = p4[i], a1 = p4[i + 64], a2 = p4[i + 128], a3 = p4[i + 192];
-    count16(a0);
-    count16(a1);
-    count16(a2);
-    count16(a3);
-  }
-  for (; i < n16; i += 64) count16(p4[i]);
-  const uint64_t tail = c0 + 16u * n16;
-  if (tail + lane < c1) {
-    const uint32_t plane = (uint32_t)((tail - c0 + lane) % P);  // c0 is a multiple of 16
-    atomicAdd(cnt + (plane * 256u + B.in[tail + lane]) * rep + r, 1u);
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  for (uint32_t idx = lane; idx < P * 256u; idx += 64) {
-    uint32_t s = 0;
-    for (uint32_t q = 0; q < rep; ++q) {
-      s += cnt[idx * rep + q];
-      cnt[idx * rep + q] = 0;
+  // rows of kHistLoads uint4 per lane, two buffers; loads are unconditional
+  // (indices past the end re-read uint4 0 and are not counted)
+  constexpr uint64_t ROW = 64ull * kHistLoads;
+  auto load = [&](uint4 (&buf)[kHistLoads], uint64_t base) {
+#pragma unroll
+    for (int k = 0; k < kHistLoads; ++k) {
+      const uint64_t i = base + 64ull * k + lane;
+      buf[k] = p4[i < n16 ? i : 0];
     }
-    if (s) atomicAdd(ghist + (uint64_t)b * 1024u + idx, s);
+  };
+  auto count = [&](const uint4 (&buf)[kHistLoads], uint64_t base) {
+#pragma unroll
+    for (int k = 0; k < kHistLoads; ++k)
+      if (base + 64ull * k + lane < n16) count16(buf[k]);
+  };
+  if (n16 > 0) {
+    uint4 bufA[kHistLoads], bufB[kHistLoads];
+    uint64_t r = 0;
+    load(bufA, 0);
+    for (;;) {
+      load(bufB, r + ROW);
+      count(bufA, r);
+      r += ROW;
+      if (r >= n16) break;
+      load(bufA, r + ROW);
+      count(bufB, r);
+      r += ROW;
+      if (r >= n16) break;
+    }
+  }
+  const uint64_t tail = c0 + 16u * n16;
+  if (tail + lane < c1) add((uint32_t)((tail - c0 + lane) % P), B.in[tail + lane]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  // merge the replicas: each lane a bin pair of a plane per round
+  for (uint32_t pr = lane; pr < P * 128u; pr += 64) {
+    uint32_t lo = 0, hi = 0;
+    for (uint32_t q = 0; q < rep; ++q) {
+      const uint32_t v = cnt[pr * rep + q];
+      cnt[pr * rep + q] = 0;
+      lo += v & 0xffffu;
+      hi += v >> 16;
+    }
+    uint32_t* g = ghist + (uint64_t)b * 1024u + (pr / 128u) * 256u + 2u * (pr % 128u);
+    if (lo) atomicAdd(g, lo);
+    if (hi) atomicAdd(g + 1, hi);
   }
 }
 
@@ -162,8 +201,8 @@ __device__ void finish_buffer(const BatchBuf& B, uint32_t b, const BatchEncodeAr
     uint16_t pr[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      // the encoder's form: {pdf << (31 - p), magic, shift | cdf << 8, 2^p - pdf}
-      const uint4 f = make_uint4(e[j].x << (31 - PB), e[j].z, e[j].w | (e[j].y << 8), (1u << PB) - e[j].x);
+      // the encoder's form (enc64_entry)
+      const uint4 f = enc64_entry(e[j].x, e[j].y, PB);
       atomic_st64(t2 + 2 * (4 * lane + j), ((uint64_t)f.y << 32) | f.x);
       atomic_st64(t2 + 2 * (4 * lane + j) + 1, ((uint64_t)f.w << 32) | f.z);
       pr[j] = (uint16_t)e[j].x;
@@ -196,171 +235,245 @@ __device__ void finish_buffer(const BatchBuf& B, uint32_t b, const BatchEncodeAr
 }
 
 // ---- encode ticket --------------------------------------------------------
-// Two chains (c = 0, 1) over units u0, u0 + 1 of buffer B.  SAME: both units
-// are planes of the same element block (P >= 2), fed by one element stream.
-// tkey: which tables the wave's LDS holds ((buffer << 4) | (plane of chain 0
-// << 2) | plane of chain 1, ~0 for none); a ticket of the same buffer and
-// planes reuses them.
-template <int PB, bool SAME, int KB>
+// An encode ticket of a buffer of EB-byte elements is kRounds rounds of
+// enc_chains(EB) units: four chains per lane (four blocks of a uint8 buffer,
+// or both planes of two uint16 blocks) or two (two planes of one uint32
+// block), so a lane runs four (two) interleaved chains per round.  One queue
+// item, one table load and one look-back serve the whole ticket.
+constexpr uint32_t kRounds = 4;
+__host__ __device__ constexpr uint32_t enc_chains(uint32_t eb) { return eb == 4 ? 2u : 4u; }
+__host__ __device__ constexpr uint32_t enc_ticket_units(uint32_t eb) { return kRounds * enc_chains(eb); }
+
+// Ticket lt of buffer B: units u0 .. u0 + UT - 1, u0 = UT lt, coded round
+// by round with the single-buffer launch's multi-chain step (enc_step64f, a
+// linear LDS stage of H + 256 words per chain, spilled to the wave's slot in
+// chunks of H; unit j of the ticket has slot area j).  uint32 tickets code
+// planes 0-1 of both blocks, then planes 2-3, so the tables change once.
+// The ticket's offset comes from a look-back within its buffer once all its
+// rounds are coded; then block word offsets and words go to their place, in
+// unit order.  tkey: which tables the wave's LDS holds ((buffer << 4) |
+// first plane, ~0 for none).  KBB: elements loaded ahead per block.
+template <int PB, int EB, int KBB>
 __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
                               uint32_t gen, uint32_t* wl, uint16_t* spill, uint64_t& tkey) {
+  constexpr int CH = (int)enc_chains(EB);        // chains per round
+  constexpr int UT = (int)enc_ticket_units(EB);  // units per ticket
+  constexpr int SPB = EB == 4 ? 2 : EB;          // chains per element block (planes coded together)
+  constexpr int BPT = CH / SPB;                  // element blocks per round
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
-  static_assert(kBlock / kL % KB == 0, "KB steps per prefetch batch");
+  constexpr uint32_t H = 256, H4 = H / 8, REG = H + 256;
+  static_assert(kBlock / kL % KBB == 0 && KBB % 4 == 0, "KBB steps per prefetch batch, checks every 4");
+  static_assert((uint32_t)CH * REG <= 2 * kRingWords, "stages fit the wave's stage area");
+  using T = typename ElemOf<EB>::T;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t P = B.eb;
+  const uint32_t P = EB;
   const uint64_t U = (uint64_t)P * B.nb;
-  const uint64_t u0 = 2ull * lt;
-  uint32_t x[2], cnt[2], n[2], fl[2], kp[2], soff[2];
-  uint64_t blk[2];
-  bool act[2];
-  const uint4* tab[2];
-  uint4* s_tab = (uint4*)wl;
+  const uint64_t u0 = (uint64_t)UT * lt;
+  uint4* s_tab = (uint4*)wl;  // [SPB <= 2][256]
   uint16_t* wstage = (uint16_t*)(wl + kTabWords);
-  const uint32_t doff = 2u * kR + 2u * lane;  // dummy slot, from wstage
+  const uint32_t dummy_addr = lds_addr(wstage + CH * REG + 2u * lane);
+  uint32_t padj[UT], cntj[UT];  // per unit of the ticket (wave-uniform)
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const uint64_t u = u0 + c;
-    act[c] = u < U;
-    kp[c] = act[c] ? (uint32_t)(u % P) : 0u;
-    blk[c] = act[c] ? u / P : 0u;
-    const uint64_t start = blk[c] * kBlock;
-    n[c] = act[c] ? (uint32_t)((B.n - start) < kBlock ? (B.n - start) : kBlock) : 0u;
-    x[c] = kMinState;
-    cnt[c] = 0;
-    fl[c] = 0;
-    soff[c] = (uint32_t)c * kR;
-    tab[c] = s_tab + c * 256;
-  }
-  // tables of the chains' planes, published by the finisher
-  const uint64_t key = ((uint64_t)b << 4) | (kp[0] << 2) | kp[1];
-  if (key != tkey) {
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const uint64_t* g = (const uint64_t*)(a.tables + ((uint64_t)b * 4u + kp[c]) * 256u);
-      uint64_t* d = (uint64_t*)(s_tab + c * 256);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) d[lane + 64 * j] = atomic_ld64(g + lane + 64 * j);
-    }
-    tkey = key;
-  }
-  const uint32_t n_min = wave_u32(n[0] < n[1] || !act[1] ? n[0] : n[1]);
-  const uint32_t n_max = wave_u32(n[0] > n[1] ? n[0] : n[1]);
-  const uint32_t all = (uint32_t)div_up(n_max, kL);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  for (int j = 0; j < UT; ++j) padj[j] = 0, cntj[j] = 0;
 
-  auto spill_full = [&]() {
+#pragma unroll 1
+  for (uint32_t r = 0; r < kRounds; ++r) {
+    // the round's units: position j0 + c in the ticket (uint32: block r % 2,
+    // planes 2 (r / 2) + c)
+    const uint32_t j0 = EB == 4 ? 4u * (r % 2u) + 2u * (r / 2u) : (uint32_t)CH * r;
+    if (u0 + j0 >= U) continue;  // (uint32: the second block's units may be missing)
+    const uint32_t p0 = EB == 4 ? 2u * (r / 2u) : 0u;  // plane of chain 0
+    uint32_t x[CH], n[CH], fl[CH], cb[CH], sb[CH];
+    uint64_t blk[CH];
+    bool act[CH];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      if (cnt[c] - fl[c] >= kH) {
-        const uint4 q = ((const uint4*)(wstage + soff[c] + (fl[c] & kMask)))[lane];
-        ((uint4*)(spill + c * CSTAGE + fl[c]))[lane] = q;
-        fl[c] += kH;
+    for (int c = 0; c < CH; ++c) {
+      const uint64_t u = u0 + j0 + c;
+      act[c] = u < U;
+      blk[c] = act[c] ? u / P : 0u;
+      const uint64_t start = blk[c] * kBlock;
+      n[c] = act[c] ? (uint32_t)((B.n - start) < kBlock ? (B.n - start) : kBlock) : 0u;
+      x[c] = kMinState;
+      fl[c] = 0;
+      cb[c] = lds_addr(wstage + c * REG);
+      sb[c] = cb[c];
+    }
+    // tables of the round's planes, published by the finisher
+    const uint64_t key = ((uint64_t)b << 4) | p0;
+    if (key != tkey) {
+      drain();  // the previous round's table reads are done
+#pragma unroll
+      for (int q = 0; q < SPB; ++q) {
+        const uint64_t* g = (const uint64_t*)(a.tables + ((uint64_t)b * 4u + p0 + q) * 256u);
+        uint64_t* d = (uint64_t*)(s_tab + q * 256);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) d[lane + 64 * k] = atomic_ld64(g + lane + 64 * k);
       }
+      tkey = key;
     }
-  };
+    uint32_t n_min = n[0], n_max = n[0];
+#pragma unroll
+    for (int c = 1; c < CH; ++c) {
+      if (act[c]) n_min = n[c] < n_min ? n[c] : n_min;
+      n_max = n[c] > n_max ? n[c] : n_max;
+    }
+    n_min = wave_u32(n_min);
+    n_max = wave_u32(n_max);
+    const bool all_act = act[CH - 1];
+    const uint32_t all = (uint32_t)div_up(n_max, kL);
+    drain();
+    uint16_t* const rspill = spill + (uint64_t)j0 * CSTAGE;  // chain c: slot area j0 + c
 
-  uint32_t s = 0;
-  {
-    // element streams: chain c reads byte kp[c] of element lane + 64 s of
-    // its block through the aligned dword holding it
-    const uint8_t* e0 = B.in + blk[0] * kBlock * P;
-    const uint8_t* e1 = B.in + blk[1] * kBlock * P;
-    const gptr<uint32_t> dp0 = (gptr<uint32_t>)e0 + ((lane * P) >> 2);
-    const gptr<uint32_t> dp1 = (gptr<uint32_t>)e1 + ((lane * P) >> 2);
-    const uint32_t sh0 = 8u * ((lane * P) & 3u) + 8u * kp[0];
-    const uint32_t sh1 = 8u * ((lane * P) & 3u) + 8u * kp[1];
-    const uint32_t dstep = kL * P / 4;  // dwords per step
-    const bool full = n_min == kBlock && act[1];
-    if (full) {
-      const uint32_t nbatch = kBlock / kL / KB;
-      const uint32_t last = (nbatch - 1) * KB;
-      uint32_t bA0[KB], bB0[KB], bA1[SAME ? 1 : KB], bB1[SAME ? 1 : KB];
-      auto load = [&](uint32_t (&b0)[KB], uint32_t (&b1)[SAME ? 1 : KB], uint32_t s0) {
+    auto spill_full = [&]() {
 #pragma unroll
-        for (int j = 0; j < KB; ++j) b0[j] = dp0[(uint64_t)(s0 + j) * dstep];
-        if constexpr (!SAME) {
+      for (int c = 0; c < CH; ++c) {
+        const uint32_t un = (sb[c] - cb[c]) >> 1;  // unspilled words
+        if (un >= H) {
+          uint16_t* st = wstage + c * REG;
+          if (lane < H4) ((uint4*)(rspill + c * CSTAGE + fl[c]))[lane] = ((const uint4*)st)[lane];
+          if (4u * lane < un - H) ((uint2*)st)[lane] = ((const uint2*)(st + H))[lane];
+          fl[c] += H;
+          sb[c] -= 2u * H;
+        }
+      }
+    };
+
+    // block d of the round: its elements; chain d * SPB + q codes byte
+    // p0 + q of each (uint32 planes 2-3: the element is shifted down 16 bits)
+    const T* eb[BPT];
 #pragma unroll
-          for (int j = 0; j < KB; ++j) b1[j] = dp1[(uint64_t)(s0 + j) * dstep];
+    for (int d = 0; d < BPT; ++d) eb[d] = (const T*)B.in + blk[d * SPB] * kBlock;
+    uint32_t s = 0;
+    if (n_min == kBlock && all_act) {
+      auto load = [&](uint32_t (&buf)[BPT][KBB], uint32_t s0) {
+#pragma unroll
+        for (int d = 0; d < BPT; ++d) {
+          const gptr<T> bp = (gptr<T>)eb[d] + lane + (uint64_t)s0 * kL;
+#pragma unroll
+          for (int k = 0; k < KBB; ++k) buf[d][k] = bp[k * kL];
         }
       };
-      auto code = [&](const uint32_t (&b0)[KB], const uint32_t (&b1)[SAME ? 1 : KB]) {
-        auto sym = [&](int j, int c) -> uint32_t {
-          if constexpr (SAME) return __builtin_amdgcn_ubfe(b0[j], c ? sh1 : sh0, 8u);
-          else return c ? __builtin_amdgcn_ubfe(b1[j], sh1, 8u) : __builtin_amdgcn_ubfe(b0[j], sh0, 8u);
-        };
-        uint4 e[2];
-        e[0] = tab[0][sym(0, 0)];
-        e[1] = tab[1][sym(0, 1)];
+      auto lookup = [&](uint4 (&e)[CH], const uint32_t (&buf)[BPT][KBB], int k) {
 #pragma unroll
-        for (int j = 0; j < KB; ++j) {
-          uint4 en[2];
-          if (j + 1 < KB) {
-            en[0] = tab[0][sym(j + 1, 0)];
-            en[1] = tab[1][sym(j + 1, 1)];
-          }
+        for (int d = 0; d < BPT; ++d) {
+          const uint32_t w = EB == 4 ? buf[d][k] >> (8u * p0) : buf[d][k];
+          e[d * SPB] = *(const uint4*)((const char*)s_tab + byte_x16<0>(w));
+          if constexpr (SPB > 1) e[d * SPB + 1] = *(const uint4*)((const char*)(s_tab + 256) + byte_x16<1>(w));
+        }
+      };
+      auto code = [&](const uint32_t (&buf)[BPT][KBB]) {
+        uint4 e[CH];
+        lookup(e, buf, 0);
 #pragma unroll
-          for (int c = 0; c < 2; ++c) enc_step<kL, kMask>(x[c], cnt[c], wstage, soff[c], doff, e[c], true, 0u);
-          spill_full();
-          if (j + 1 < KB) {
-            e[0] = en[0];
-            e[1] = en[1];
+        for (int k = 0; k < KBB; ++k) {
+          uint4 en[CH];
+          if (k + 1 < KBB) lookup(en, buf, k + 1);
+#pragma unroll
+          for (int c = 0; c < CH; ++c) enc_step64f(x[c], sb[c], dummy_addr, e[c]);
+          if ((k & 3) == 3) spill_full();
+          if (k + 1 < KBB) {
+#pragma unroll
+            for (int c = 0; c < CH; ++c) e[c] = en[c];
           }
         }
       };
-      load(bA0, bA1, 0);
+      const uint32_t last = kBlock / kL - KBB;
+      uint32_t bA[BPT][KBB], bB[BPT][KBB];
+      load(bA, 0);
       for (;;) {
-        load(bB0, bB1, s + KB <= last ? s + KB : last);
-        code(bA0, bA1);
-        s += KB;
+        load(bB, s + KBB <= last ? s + KBB : last);
+        code(bA);
+        s += KBB;
         if (s > last) break;
-        load(bA0, bA1, s + KB <= last ? s + KB : last);
-        code(bB0, bB1);
-        s += KB;
+        load(bA, s + KBB <= last ? s + KBB : last);
+        code(bB);
+        s += KBB;
         if (s > last) break;
       }
     }
     for (; s < all; ++s) {
       const uint32_t i = s * kL + lane;
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
+      for (int c = 0; c < CH; ++c) {
         const bool v = i < n[c];
-        const uint8_t* eb = c ? e1 : e0;
-        const uint32_t sy = v ? (uint32_t)eb[(uint64_t)i * P + kp[c]] : 0u;
-        const uint4 e = tab[c][sy];
-        enc_step<kL, kMask>(x[c], cnt[c], wstage, soff[c], doff, e, v, 0u);
+        const uint32_t w = v ? (uint32_t)eb[c / SPB][i] : 0u;
+        const uint32_t sy = (w >> (8u * (p0 + (uint32_t)(c % SPB)))) & 0xffu;
+        enc_step64(x[c], sb[c], dummy_addr, s_tab[(c % SPB) * 256 + sy], v);
       }
       spill_full();
     }
+
+    // pad, stage remainder to the slot, states, block sizes
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const uint32_t rem = (sb[c] - cb[c]) >> 1;
+      const uint32_t cnt = fl[c] + rem;
+      const uint32_t padded = act[c] ? (uint32_t)round_up(cnt, 8) : 0u;
+      if (cnt > CSTAGE && lane == 0) atomicOr(a.err, kErrStageOverflow);
+      uint16_t* st = wstage + c * REG;
+      if (lane < padded - cnt) st[rem + lane] = 0;
+      const uint32_t nr = padded > fl[c] ? (padded - fl[c]) / 8 : 0u;
+      for (uint32_t k = lane; k < nr; k += kL) ((uint4*)(rspill + c * CSTAGE + fl[c]))[k] = ((const uint4*)st)[k];
+      const uint32_t kp = p0 + (uint32_t)(c % SPB);
+      if (act[c]) {
+        ((uint32_t*)(B.out + B.section[kp] + kSectionFixed))[blk[c] * kL + lane] = x[c];
+        if (lane == 0) {
+          ((uint32_t*)(B.out + B.section[kp] + kSectionFixed + 4ull * kL * B.nb + 8ull * blk[c]))[0] =
+              (n[c] << 16) | cnt;
+          atomicAdd(a.ptot + 4ull * b + kp, padded);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < UT; ++j)
+        if ((uint32_t)j == j0 + (uint32_t)c) padj[j] = padded, cntj[j] = cnt;
+    }
   }
 
-  // pad, states, aggregate
-  uint32_t padded[2];
+  // the ticket's offset, then block word offsets and words in unit order
   uint64_t agg = 0;
+  uint32_t qb[UT + 1];  // first uint4 of each unit in the ticket's output
+  qb[0] = 0;
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    padded[c] = (uint32_t)round_up(cnt[c], 8);
-    if (cnt[c] > CSTAGE && lane == 0) atomicOr(a.err, kErrStageOverflow);
-    if (lane < padded[c] - cnt[c]) wstage[soff[c] + ((cnt[c] + lane) & kMask)] = 0;
-    if (act[c]) ((uint32_t*)(B.out + B.section[kp[c]] + kSectionFixed))[blk[c] * kL + lane] = x[c];
-    agg += wave_u32(padded[c]);
+  for (int j = 0; j < UT; ++j) {
+    agg += padj[j];
+    qb[j + 1] = qb[j] + padj[j] / 8;
   }
   const uint64_t excl = look_back_gen(a.status, tix, tix - lt, agg, gen, (int)lane);
-  drain();  // spilled words are read back by the lanes that wrote them
-  uint64_t y = excl;
+  drain();  // the slot was written by other lanes of this wave
+  if (lane == 0) {
+    uint64_t y = excl;
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    if (act[c] && lane == 0) {
-      uint8_t* sec = B.out + B.section[kp[c]];
-      uint32_t* bw = (uint32_t*)(sec + kSectionFixed + 4ull * kL * B.nb + 8ull * blk[c]);
-      bw[0] = (n[c] << 16) | cnt[c];
-      bw[1] = (uint32_t)y;
-      atomicAdd(a.ptot + 4ull * b + kp[c], padded[c]);
+    for (int j = 0; j < UT; ++j) {
+      const uint64_t u = u0 + j;
+      if (u < U) {
+        const uint32_t kp = (uint32_t)(u % P);
+        ((uint32_t*)(B.out + B.section[kp] + kSectionFixed + 4ull * kL * B.nb + 8ull * (u / P)))[1] = (uint32_t)y;
+      }
+      y += padj[j];
     }
-    uint4* dst = (uint4*)(B.out + B.data_off + 2ull * y);
-    const uint4* ring4 = (const uint4*)(wstage + soff[c]);
-    const uint4* spill4 = (const uint4*)(spill + c * CSTAGE);
-    for (uint32_t j = lane; j < padded[c] / 8; j += kL) dst[j] = 8 * j < fl[c] ? spill4[j] : ring4[((8 * j) & kMask) / 8];
-    y += padded[c];
+  }
+  {
+    const uint32_t nq = qb[UT];
+    const uint4* s4 = (const uint4*)spill;
+    uint4* dst = (uint4*)(B.out + B.data_off + 2ull * excl);
+    constexpr int CBQ = 8;
+    for (uint32_t j0 = 0; j0 < nq; j0 += CBQ * kL) {
+      uint4 q[CBQ];
+#pragma unroll
+      for (int u = 0; u < CBQ; ++u) {
+        const uint32_t j = j0 + u * kL + lane;
+        const uint32_t jj = j < nq ? j : 0u;
+        uint32_t dd = 0;
+#pragma unroll
+        for (int k = 1; k < UT; ++k) dd = jj >= qb[k] ? k * (CSTAGE / 8) - qb[k] : dd;
+        q[u] = s4[jj + dd];
+      }
+#pragma unroll
+      for (int u = 0; u < CBQ; ++u) {
+        const uint32_t j = j0 + u * kL + lane;
+        if (j < nq) dst[j] = q[u];
+      }
+    }
   }
   if (lt + 1 == B.nticks && lane == 0) {
     const uint64_t total_words = excl + agg;
@@ -389,7 +502,7 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = wave_u32(threadIdx.x >> 6);
   uint32_t* wl = lds + wave * kWaveWords;
-  uint16_t* const spill = a.spill + ((uint64_t)blockIdx.x * NW + wave) * (2ull * CSTAGE);
+  uint16_t* const spill = a.spill + ((uint64_t)blockIdx.x * NW + wave) * (16ull * CSTAGE);
   for (uint32_t i = lane; i < kTabWords; i += 64) wl[i] = 0;  // histogram counters start at zero
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   // this launch's generation (every wavefront reads it before the last one
@@ -401,12 +514,18 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
   if (lane == 0) it = atomicAdd(a.queue, 1u);
   it = wave_u32(it);
   while (it < a.nitems) {
-    const uint32_t b = wave_u32(a.item_buf[it]);
+    const uint2 itm = a.items[it];
+    const uint32_t b = wave_u32(itm.x), li = wave_u32(itm.y);
     const BatchBuf& B = a.bufs[b];
-    if (it < a.nhist_items) {
+    if (!(li & kItemTicket)) {
+      if (tkey != ~0ull) {
+        // the area held a ticket's tables: the counters start at zero
+        for (uint32_t i = lane; i < kTabWords / 4; i += 64) ((uint4*)wl)[i] = make_uint4(0, 0, 0, 0);
+        drain();
+      }
       tkey = ~0ull;
       const uint64_t bytes = B.n * B.eb;
-      const uint64_t c0 = (uint64_t)(it - B.hist_first) * kBatchHistChunk;
+      const uint64_t c0 = (uint64_t)li * kBatchHistChunk;
       const uint64_t c1 = c0 + kBatchHistChunk < bytes ? c0 + kBatchHistChunk : bytes;
       if (c0 < c1) hist_chunk(B, b, c0, c1, wl, a.hist);
       publish();
@@ -414,13 +533,14 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
       if (lane == 0) d = atomicAdd(a.done + b, 1u);
       if (wave_u32(d) + 1 == B.nhist) finish_buffer<PB>(B, b, a, gen, wl + kTabWords);
     } else {
-      const uint32_t tix = it - a.nhist_items;
-      const uint32_t lt = tix - B.tick_first;
+      const uint32_t lt = li & ~kItemTicket;
+      const uint32_t tix = B.tick_first + lt;
       while (wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u) != gen) __builtin_amdgcn_s_sleep(2);
       // (the tables share LDS with the histogram counters; items start in
       // queue order, so no histogram item follows a ticket in a wave)
-      if (B.eb == 1) encode_ticket<PB, false, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
-      else encode_ticket<PB, true, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
+      if (B.eb == 1) encode_ticket<PB, 1, (KB / 4 < 4 ? 4 : KB / 4)>(B, b, lt, tix, a, gen, wl, spill, tkey);
+      else if (B.eb == 2) encode_ticket<PB, 2, (KB / 2 < 4 ? 4 : KB / 2)>(B, b, lt, tix, a, gen, wl, spill, tkey);
+      else encode_ticket<PB, 4, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     // the next item only once this one is done (items start in order)
@@ -456,7 +576,7 @@ hipError_t launch_enc_shape(const BatchEncodeArgs& a, int cu, hipStream_t s, uin
     const uint64_t want = div_up(a.nitems, NW);
     if (want < grid) grid = want;
     if (grid < 1) grid = 1;
-    const uint64_t spill_bytes = grid * NW * 2ull * CSTAGE * 2ull;
+    const uint64_t spill_bytes = grid * NW * 16ull * CSTAGE * 2ull;  // a slot area per unit of a ticket
     if (need_spill) {
       *need_spill = spill_bytes;
       return hipSuccess;
@@ -790,10 +910,11 @@ __global__ __launch_bounds__(64 * NW) void batch_decode_kernel(BatchDecodeArgs a
   const uint32_t gen = wave_u32(lane == 0 ? atomic_ld32(a.queue + 2) : 0u);
   uint64_t tkey = ~0ull;  // LUTs held in this wave's LDS
   uint32_t it = 0;
-  if (lane == 0) it = atomicAdd(a.queue, 1u);
+  if (lane == 0) it = atomicAdd(a.queue, kRun);
   it = wave_u32(it);
+  uint32_t run_end = it + kRun;
   while (it < a.nitems) {
-    const uint32_t b = wave_u32(a.item_buf[it]);
+    const uint32_t b = wave_u32(a.items[it].x);
     const BatchBuf& B = a.bufs[b];
     if (it < a.nbufs) {
       build_luts<PB>(B, b, a, gen, wl);  // uses the LUT area as scratch
@@ -809,9 +930,12 @@ __global__ __launch_bounds__(64 * NW) void batch_decode_kernel(BatchDecodeArgs a
       }
     }
     drain();
-    uint32_t nxt = 0;
-    if (lane == 0) nxt = atomicAdd(a.queue, 1u);
-    it = wave_u32(nxt);
+    if (++it == run_end) {
+      uint32_t nxt = 0;
+      if (lane == 0) nxt = atomicAdd(a.queue, kRun);
+      it = wave_u32(nxt);
+      run_end = it + kRun;
+    }
   }
   if (lane == 0) {
     const uint32_t total = gridDim.x * NW;
@@ -835,7 +959,7 @@ hipError_t launch_dec_shape(const BatchDecodeArgs& a, int cu, hipStream_t s) {
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     uint64_t grid = (uint64_t)cu * (uint64_t)per_cu;
-    const uint64_t want = div_up(a.nitems, NW);
+    const uint64_t want = div_up(a.nitems, (uint64_t)NW * kRun);
     if (want < grid) grid = want;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL((batch_decode_kernel<PB, NW, KB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);

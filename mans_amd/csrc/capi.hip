@@ -999,7 +999,7 @@ struct mans_batch {
   // current plan
   uint32_t nbufs = 0, nhist = 0, nitems = 0, nticks = 0;
   BatchBuf* bufs = nullptr;   // device [cap_bufs]
-  uint32_t* item_buf = nullptr;  // device [cap_items]
+  uint2* item_buf = nullptr;  // device [cap_items] queue items
   uint32_t* hist = nullptr;   // [cap_bufs][4][256]
   uint32_t* done = nullptr;   // [cap_bufs]
   uint32_t* ready = nullptr;  // [cap_bufs]
@@ -1010,7 +1010,7 @@ struct mans_batch {
   uint32_t* queue = nullptr;  // [3] next item, exited waves, generation
   uint32_t* luts = nullptr;   // decode: [cap_bufs][4][2^p]
   BatchBuf* h_bufs = nullptr; // pinned staging for asynchronous plans (or null)
-  uint32_t* h_items = nullptr;
+  uint2* h_items = nullptr;
   // the batch's own encoder spill areas, sized at create for every launch
   // shape, so neither a shape change nor the workspace can move them under
   // a captured run
@@ -1084,7 +1084,9 @@ mans_status plan_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t 
     B.hist_first = (uint32_t)nhist;
     B.nhist = (uint32_t)std::max<uint64_t>(1, div_up(m.n_elems * eb, kBatchHistChunk));
     B.tick_first = (uint32_t)nticks;
-    B.nticks = (uint32_t)div_up(eb * nb, 2);
+    // decode tickets: two units; encode tickets: four rounds of four chains
+    // (two for uint32), as enc_ticket_units in batch_kernels.hpp
+    B.nticks = (uint32_t)div_up(eb * nb, decode ? 2u : (eb == 4 ? 8u : 16u));
     B.in_bytes = decode ? m.in_bytes : 0;
     frame_header(B.frame_hdr, m.n_elems, eb, MANS_MODE_PLANES, f, p->prob_bits, d, 0, 0);
     put64(B.frame_hdr, 56, off);  // total size of an empty frame
@@ -1104,7 +1106,7 @@ mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, ui
   b->cap_items = cap_items;
   b->cap_ticks = cap_ticks;
   HIP_TRY(hipMalloc(&b->bufs, nb1 * sizeof(BatchBuf)));
-  HIP_TRY(hipMalloc(&b->item_buf, std::max<size_t>(1, cap_items) * 4));
+  HIP_TRY(hipMalloc(&b->item_buf, std::max<size_t>(1, cap_items) * sizeof(uint2)));
   HIP_TRY(hipMalloc(&b->ready, nb1 * 4));
   HIP_TRY(hipMalloc(&b->queue, 12));
   HIP_TRY(hipMemset(b->ready, 0, nb1 * 4));
@@ -1130,7 +1132,7 @@ mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, ui
   }
   if (pinned) {
     HIP_TRY(hipHostMalloc((void**)&b->h_bufs, nb1 * sizeof(BatchBuf), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&b->h_items, std::max<size_t>(1, cap_items) * 4, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&b->h_items, std::max<size_t>(1, cap_items) * sizeof(uint2), hipHostMallocDefault));
   }
   return MANS_OK;
 }
@@ -1145,26 +1147,44 @@ mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h
   if (st != MANS_OK) return st;
   const uint64_t nitems = (b->decode ? (uint64_t)count : nhist) + nticks;
   if ((uint32_t)count > b->cap_bufs || nitems > b->cap_items || nticks > b->cap_ticks) return MANS_ERR_TOO_LARGE;
-  std::vector<uint32_t> items;
-  uint32_t* dst = b->h_items;
+  std::vector<uint2> items;
+  uint2* dst = b->h_items;
   if (!dst) {
     items.resize(std::max<size_t>(1, nitems));
     dst = items.data();
   }
   uint64_t k = 0;
-  for (int32_t i = 0; i < count; ++i) {
-    const uint32_t reps = b->decode ? 1u : bufs[(size_t)i].nhist;
-    for (uint32_t r = 0; r < reps; ++r) dst[k++] = (uint32_t)i;
+  if (b->decode) {
+    // every buffer's LUT item, then the tickets in buffer order
+    for (int32_t i = 0; i < count; ++i) dst[k++] = make_uint2((uint32_t)i, 0u);
+    for (int32_t i = 0; i < count; ++i)
+      for (uint32_t r = 0; r < bufs[(size_t)i].nticks; ++r) dst[k++] = make_uint2((uint32_t)i, 0u);
+  } else {
+    // histogram chunks in buffer order; a buffer's tickets follow once
+    // `lag` more items have been queued after its last chunk (by then its
+    // chunks are counted, normally), so counting and coding overlap
+    const uint64_t lag = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)b->ws->cu * 16u, nhist / 4));
+    std::vector<std::pair<uint64_t, int32_t>> pending;  // (item after the buffer's chunks, buffer)
+    size_t head = 0;
+    auto emit_tickets = [&](int32_t i) {
+      for (uint32_t r = 0; r < bufs[(size_t)i].nticks; ++r) dst[k++] = make_uint2((uint32_t)i, kItemTicket | r);
+    };
+    for (int32_t i = 0; i < count; ++i) {
+      for (uint32_t r = 0; r < bufs[(size_t)i].nhist; ++r) {
+        dst[k++] = make_uint2((uint32_t)i, r);
+        while (head < pending.size() && k >= pending[head].first + lag) emit_tickets(pending[head++].second);
+      }
+      pending.emplace_back(k, i);
+    }
+    while (head < pending.size()) emit_tickets(pending[head++].second);
   }
-  for (int32_t i = 0; i < count; ++i)
-    for (uint32_t r = 0; r < bufs[(size_t)i].nticks; ++r) dst[k++] = (uint32_t)i;
   if (b->h_bufs) {
     if (count > 0) memcpy(b->h_bufs, bufs.data(), bufs.size() * sizeof(BatchBuf));
     if (count > 0) HIP_TRY(hipMemcpyAsync(b->bufs, b->h_bufs, bufs.size() * sizeof(BatchBuf), hipMemcpyHostToDevice, s));
-    if (nitems > 0) HIP_TRY(hipMemcpyAsync(b->item_buf, dst, nitems * 4, hipMemcpyHostToDevice, s));
+    if (nitems > 0) HIP_TRY(hipMemcpyAsync(b->item_buf, dst, nitems * sizeof(uint2), hipMemcpyHostToDevice, s));
   } else {
     if (count > 0) HIP_TRY(hipMemcpy(b->bufs, bufs.data(), bufs.size() * sizeof(BatchBuf), hipMemcpyHostToDevice));
-    if (nitems > 0) HIP_TRY(hipMemcpy(b->item_buf, dst, nitems * 4, hipMemcpyHostToDevice));
+    if (nitems > 0) HIP_TRY(hipMemcpy(b->item_buf, dst, nitems * sizeof(uint2), hipMemcpyHostToDevice));
   }
   b->pb = p->prob_bits;
   b->total_bytes = 0;
@@ -1193,7 +1213,7 @@ BatchEncodeArgs batch_args(const mans_batch* b, uint64_t* d_sizes) {
   BatchEncodeArgs a;
   memset(&a, 0, sizeof(a));
   a.bufs = b->bufs;
-  a.item_buf = b->item_buf;
+  a.items = b->item_buf;
   a.nbufs = b->nbufs;
   a.nhist_items = b->nhist;
   a.nitems = b->nitems;
@@ -1462,7 +1482,7 @@ mans_status mans_batch_decode_run(mans_batch* b, void* stream) {
   BatchDecodeArgs a;
   memset(&a, 0, sizeof(a));
   a.bufs = b->bufs;
-  a.item_buf = b->item_buf;
+  a.items = b->item_buf;
   a.nbufs = b->nbufs;
   a.nitems = b->nitems;
   a.ready = b->ready;

@@ -220,12 +220,14 @@ struct BatchBuf {
   uint32_t frame_hdr[32];        // frame header (data_words / total_size set by the kernel)
 };
 
+// A queue item: {buffer, kItemTicket | ticket, or histogram chunk}.
+constexpr uint32_t kItemTicket = 1u << 31;
 struct BatchEncodeArgs {
   const BatchBuf* bufs;
-  const uint32_t* item_buf;      // [nitems] item -> buffer
+  const uint2* items;            // [nitems] histogram chunks and encode tickets
   uint32_t nbufs;
-  uint32_t nhist_items;          // items [0, nhist_items) are histogram chunks
-  uint32_t nitems;               // then encode tickets
+  uint32_t nhist_items;          // histogram chunks among the items
+  uint32_t nitems;               // items: a buffer's chunks come before its tickets
   uint32_t* hist;                // [nbufs][4][256] accumulators (left zeroed)
   uint32_t* done;                // [nbufs] histogram items finished (left zeroed)
   uint32_t* ready;               // [nbufs] generation whose tables are ready
@@ -245,7 +247,7 @@ struct BatchEncodeArgs {
 // [0, nbufs) build buffer b's LUTs, then come the decode tickets.
 struct BatchDecodeArgs {
   const BatchBuf* bufs;
-  const uint32_t* item_buf;      // [nitems] item -> buffer
+  const uint2* items;            // [nitems] {buffer, 0}: LUT items, then tickets
   uint32_t nbufs;
   uint32_t nitems;
   uint32_t* ready;               // [nbufs] generation whose LUTs are built (| kBatchBad: frame rejected)
@@ -256,7 +258,7 @@ struct BatchDecodeArgs {
 };
 constexpr uint32_t kBatchBad = 1u << 31;
 constexpr uint32_t kBatchGenMask = 0x3fffu;     // 14-bit generations
-constexpr uint32_t kBatchHistChunk = 1u << 18;  // bytes per histogram item
+constexpr uint32_t kBatchHistChunk = 1u << 19;  // bytes per histogram item
 
 // ---- archive: the frames of a batch in one container (archive.hip) --------
 constexpr uint32_t kArchiveMagic = 0x414E414Du;  // "MANA"
