@@ -471,8 +471,9 @@ def served_bench(args):
         submitted at once to the server, against the same requests called
         back to back on the plain codec (aggregate GB/s = request bytes /
         makespan).
-    Every served output is compared byte for byte with the plain codec's
-    output for its pool buffer (frame, or elements)."""
+    Served outputs are compared byte for byte with the plain codec's output
+    for their pool buffer (frame, or elements): every 16th under arrivals,
+    every one in the burst."""
     import threading
 
     import torch
@@ -556,12 +557,16 @@ def served_bench(args):
     t_last = [0.0]
 
     def collect():
+        # every 16th result is compared here (each comparison synchronises
+        # this thread with the device, and a result's arena memory is only
+        # reused after its release, so a slow collector would throttle
+        # admission); the burst below compares every result
         for i in range(n_req):
             ready.acquire()
             r = srv.wait(ids[i], 300_000)
             t_last[0] = time.perf_counter()
             lat_srv.append(r.latency_ms if r is not None else float("inf"))
-            oks.append(check(i, r))
+            oks.append(check(i, r) if i % 16 == 0 else (r is not None and r.status == 0))
             srv.release(ids[i])
 
     th = threading.Thread(target=collect)

@@ -115,6 +115,24 @@ mans_status mans_encode_noalloc(const mans_params* p, const void* d_in, uint64_t
 mans_status mans_decode(const mans_params* p, const void* d_in, uint64_t c, void* d_out,
                         uint64_t out_capacity, mans_workspace* ws, void* stream);
 
+/* ---- many single streams, DietGPU-style pointer arrays ----------------
+ * The reference keeps DietGPU's multi-input form only as commented-out
+ * vestiges (InProvider / OutProvider, a batch on blockIdx.y:
+ * GpuANSStatistics.h:138-151, GpuANSDecode.h:202-214,304-317).  These take
+ * host arrays of device pointers: input i (h_in_size[i] bytes) is coded
+ * exactly as mans_encode codes it alone (p->lanes 32: the reference
+ * container, byte-identical to ansEncodeBatch of that input), its size
+ * written to d_out_sizes[i] (device uint32).  The inputs are coded one after
+ * another on the stream with the workspace's buffers; the calls allocate
+ * nothing beyond what the largest input needs (mans_workspace_reserve).
+ * Decode: each container at h_in[i] into h_out[i]; a malformed one raises
+ * the sticky error read by mans_workspace_check. */
+mans_status mans_encode_multi(const mans_params* p, int32_t count, const void* const* h_in,
+                              const uint32_t* h_in_size, void* const* h_out, const uint64_t* h_out_capacity,
+                              uint32_t* d_out_sizes, mans_workspace* ws, void* stream);
+mans_status mans_decode_multi(const mans_params* p, int32_t count, const void* const* h_in, void* const* h_out,
+                              const uint64_t* h_out_capacity, mans_workspace* ws, void* stream);
+
 /* ---- stage entry points (each reference kernel on its own) ------------- */
 /* 256-bin histogram of the n bytes d_in[i*stride] (ansHistogramBatch,
  * GpuANSStatistics.h:409-446).  d_hist is overwritten. */

@@ -74,6 +74,9 @@ SIGNATURES = {
     "mans_encode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, C.POINTER(_u64), _vp, _vp]),
     "mans_encode_noalloc": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, _vp]),
     "mans_decode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, _vp]),
+    "mans_encode_multi": (C.c_int, [_P, _i32, C.POINTER(_vp), C.POINTER(_u32), C.POINTER(_vp), C.POINTER(_u64), _vp,
+                                    _vp, _vp]),
+    "mans_decode_multi": (C.c_int, [_P, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_u64), _vp, _vp]),
     "mans_histogram": (C.c_int, [_vp, _u64, _u32, _vp, _vp, _vp]),
     "mans_normalize": (C.c_int, [_vp, _u32, _i32, _vp, _vp, _vp]),
     "mans_adm_map": (C.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _u64, C.POINTER(_u64), C.POINTER(_i32), _vp, _vp]),
@@ -240,6 +243,33 @@ def encode_noalloc(data, p: Params, out, d_size, stream=None):
                                    _stream(stream))
     _check(rc, "mans_encode_noalloc")
     return out, d_size
+
+
+def encode_multi(inputs, p: Params, ws: Workspace, stream=None):
+    """mans_encode_multi over uint8 CUDA tensors: (containers, device sizes)."""
+    torch = _torch()
+    k = len(inputs)
+    outs = [torch.empty(max_compressed_size(x.numel(), p), dtype=torch.uint8, device="cuda") for x in inputs]
+    sizes = torch.zeros(max(1, k), dtype=torch.int32, device="cuda")
+    ins = (C.c_void_p * max(1, k))(*[_ptr(x) for x in inputs])
+    ns = (C.c_uint32 * max(1, k))(*[x.numel() for x in inputs])
+    ops = (C.c_void_p * max(1, k))(*[o.data_ptr() for o in outs])
+    caps = (C.c_uint64 * max(1, k))(*[o.numel() for o in outs])
+    _check(lib().mans_encode_multi(C.byref(p), k, ins, ns, ops, caps, sizes.data_ptr(), ws.handle, _stream(stream)),
+           "mans_encode_multi")
+    return outs, sizes
+
+
+def decode_multi(comps, sizes_n, p: Params, ws: Workspace, stream=None):
+    """mans_decode_multi: containers -> uint8 CUDA tensors of the given lengths."""
+    torch = _torch()
+    k = len(comps)
+    outs = [torch.empty(max(1, n), dtype=torch.uint8, device="cuda") for n in sizes_n]
+    ins = (C.c_void_p * max(1, k))(*[c.data_ptr() for c in comps])
+    ops = (C.c_void_p * max(1, k))(*[o.data_ptr() for o in outs])
+    caps = (C.c_uint64 * max(1, k))(*[o.numel() for o in outs])
+    _check(lib().mans_decode_multi(C.byref(p), k, ins, ops, caps, ws.handle, _stream(stream)), "mans_decode_multi")
+    return [o[:n] for o, n in zip(outs, sizes_n)]
 
 
 def decode(comp, n: int, p: Params, ws: Workspace, out=None, stream=None, c: int | None = None):

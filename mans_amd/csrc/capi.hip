@@ -552,6 +552,42 @@ mans_status mans_decode(const mans_params* p, const void* d_in, uint64_t c, void
   return MANS_OK;
 }
 
+mans_status mans_encode_multi(const mans_params* p, int32_t count, const void* const* h_in,
+                              const uint32_t* h_in_size, void* const* h_out, const uint64_t* h_out_capacity,
+                              uint32_t* d_out_sizes, mans_workspace* ws, void* stream) {
+  mans_status st = check_params(p);
+  if (st != MANS_OK) return st;
+  if (!ws || count < 0 || (count > 0 && (!h_in || !h_in_size || !h_out || !h_out_capacity || !d_out_sizes)))
+    return MANS_ERR_INVALID_ARG;
+  // every argument first: on an error nothing is launched
+  for (int32_t i = 0; i < count; ++i) {
+    if (!h_out[i] || (h_in_size[i] > 0 && !h_in[i])) return MANS_ERR_INVALID_ARG;
+    if (!aligned(h_out[i], 16) || !aligned(h_in[i], 4)) return MANS_ERR_ALIGNMENT;
+    if (h_out_capacity[i] < mans_max_compressed_size(h_in_size[i], p)) return MANS_ERR_CAPACITY;
+  }
+  for (int32_t i = 0; i < count; ++i) {
+    st = mans_encode(p, h_in[i], h_in_size[i], h_out[i], h_out_capacity[i], d_out_sizes + i, nullptr, ws, stream);
+    if (st != MANS_OK) return st;
+  }
+  return MANS_OK;
+}
+
+mans_status mans_decode_multi(const mans_params* p, int32_t count, const void* const* h_in, void* const* h_out,
+                              const uint64_t* h_out_capacity, mans_workspace* ws, void* stream) {
+  mans_status st = check_params(p);
+  if (st != MANS_OK) return st;
+  if (!ws || count < 0 || (count > 0 && (!h_in || !h_out || !h_out_capacity))) return MANS_ERR_INVALID_ARG;
+  for (int32_t i = 0; i < count; ++i) {
+    if (!h_in[i] || !h_out[i]) return MANS_ERR_INVALID_ARG;
+    if (!aligned(h_in[i], 16)) return MANS_ERR_ALIGNMENT;
+  }
+  for (int32_t i = 0; i < count; ++i) {
+    st = mans_decode(p, h_in[i], 0, h_out[i], h_out_capacity[i], ws, stream);
+    if (st != MANS_OK) return st;
+  }
+  return MANS_OK;
+}
+
 mans_status mans_histogram(const void* d_in, uint64_t n, uint32_t stride, uint32_t* d_hist, mans_workspace* ws,
                            void* stream) {
   if (!ws || !d_hist || (n > 0 && !d_in)) return MANS_ERR_INVALID_ARG;

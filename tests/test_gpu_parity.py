@@ -358,3 +358,27 @@ def test_compress_cli_round_trip():
         assert r.returncode == 0
         assert (np.fromfile(dst, dtype=np.uint16) == x).all()
         assert (oracle.frame_decode(np.fromfile(tmp, dtype=np.uint8)) == x).all()
+
+
+# ------------------------------------------------------------------ multi-input
+@pytest.mark.parametrize("lanes", [32, 64])
+def test_multi_input_streams_match_single(ws, lanes):
+    # DietGPU-style pointer arrays: every container equals mans_encode of
+    # that input alone (and, for the reference format, the oracle's)
+    rng = np.random.default_rng(50 + lanes)
+    xs = [rng.integers(0, 256, n).astype(np.uint8) for n in (0, 1, 4095, 4096, 70_001)]
+    xs += [datasets(51)["laplace_bytes"], datasets(52)["skewed"]]
+    prm = mans_amd.params(10, lanes)
+    ts = [dev(x) if x.size else torch.empty(0, dtype=torch.uint8, device="cuda") for x in xs]
+    outs, sizes = mans_amd.encode_multi(ts, prm, ws)
+    sizes = host(sizes).tolist()
+    for i, (x, t) in enumerate(zip(xs, ts)):
+        ref, ref_size = mans_amd.encode(t, prm, ws)
+        assert sizes[i] == ref_size and (host(outs[i][:ref_size]) == host(ref[:ref_size])).all(), i
+        if lanes == 32:
+            o = oracle.stream_encode(x, 10, REF)
+            assert o.size == ref_size and (host(outs[i][:ref_size]) == o).all(), i
+    backs = mans_amd.decode_multi(outs, [x.size for x in xs], prm, ws)
+    for i, x in enumerate(xs):
+        assert (host(backs[i])[:x.size] == x).all(), i
+    ws.check()
