@@ -553,7 +553,7 @@ def served_bench(args):
     # thread waits, checks and releases in submission order
     ids = [0] * n_req
     ready = threading.Semaphore(0)
-    lat_srv, oks = [], []
+    lat_srv, run_srv, oks = [], [], []
     t_last = [0.0]
 
     def collect():
@@ -566,6 +566,7 @@ def served_bench(args):
             r = srv.wait(ids[i], 300_000)
             t_last[0] = time.perf_counter()
             lat_srv.append(r.latency_ms if r is not None else float("inf"))
+            run_srv.append(r.latency_ms - r.queue_ms if r is not None else float("inf"))
             oks.append(check(i, r) if i % 16 == 0 else (r is not None and r.status == 0))
             srv.release(ids[i])
 
@@ -621,6 +622,7 @@ def served_bench(args):
                   "p99_ms": pct(lat_plain, 99), "span_s": round(plain_span, 3)},
         "served": {"GBps": round(total / served_span / 1e9, 2), "p50_ms": pct(lat_srv, 50),
                    "p99_ms": pct(lat_srv, 99), "span_s": round(served_span, 3),
+                   "in_launch_p50_ms": pct(run_srv, 50), "in_launch_p99_ms": pct(run_srv, 99),
                    "max_submit_lag_ms": round(late * 1e3, 3)},
         "capacity": {"requests": nb, "bytes": int(burst_bytes),
                      "plain_GBps": round(burst_bytes / plain_burst_s / 1e9, 2),

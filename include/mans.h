@@ -302,6 +302,17 @@ mans_status mans_batch_set_shape(mans_batch* b, const mans_batch_shape* shape);
  * two, element widths present, compute units). */
 mans_status mans_batch_tune(mans_batch* b, void* stream, int32_t reps, mans_tune_entry* h_entries,
                             int32_t max_entries, int32_t* n_entries);
+/* Work-item grain of the batches planned after this call, process-wide: the
+ * bytes of a histogram item (a power of two in [64 KiB, 512 KiB]), the
+ * rounds of an encode ticket (1, 2 or 4; a ticket codes rounds x 4 byte-plane
+ * blocks, rounds x 2 for uint32) and the decode items a wavefront takes per
+ * queue grab (1 or 4).  0 leaves a value to the planner, which picks the
+ * largest grain that still gives every compute unit tens of items (one
+ * request of tens of MB: 64 KiB, 1, 1; thousands of buffers of several GB:
+ * 512 KiB, 4, 4).  The grain changes no output byte; it is a tuning and test
+ * knob.  MANS_ERR_INVALID_ARG for other values.  No counterpart in the
+ * reference (its kernels take one buffer per launch). */
+mans_status mans_batch_set_grain(uint32_t hist_chunk, uint32_t rounds, uint32_t run);
 /* create + run + destroy (synchronises the stream before freeing). */
 mans_status mans_encode_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, uint64_t* d_sizes,
                               mans_workspace* ws, void* stream);

@@ -102,6 +102,7 @@ SIGNATURES = {
     "mans_batch_get_shape": (C.c_int, [_vp, C.POINTER(BatchShape)]),
     "mans_batch_set_shape": (C.c_int, [_vp, C.POINTER(BatchShape)]),
     "mans_batch_tune": (C.c_int, [_vp, _vp, _i32, C.POINTER(TuneEntry), _i32, C.POINTER(_i32)]),
+    "mans_batch_set_grain": (C.c_int, [_u32, _u32, _u32]),
     "mans_encode_batch": (C.c_int, [_P, _B, _i32, _vp, _vp, _vp]),
     "mans_decode_batch": (C.c_int, [_P, _B, _i32, _vp, _vp]),
     "mans_server_create": (C.c_int, [C.POINTER(_vp), _P, _u64, _i32]),
@@ -527,6 +528,14 @@ class DecodeBatch(_Tunable):
 
 
 _NP_DT = {1: "int8", 2: "int16", 4: "int32"}
+
+
+def set_batch_grain(hist_chunk: int = 0, rounds: int = 0, run: int = 0) -> None:
+    """Force the work-item grain of batches planned from now on (process-wide;
+    0 = the planner's pick): histogram item bytes (power of two, 64 KiB ..
+    512 KiB), encode ticket rounds (1, 2, 4), decode items per grab (1, 4).
+    Output bytes do not depend on it."""
+    _check(lib().mans_batch_set_grain(hist_chunk, rounds, run), "mans_batch_set_grain")
 
 
 def archive_index(archive) -> list[tuple[int, int, int, int]]:

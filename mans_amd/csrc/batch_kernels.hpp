@@ -43,14 +43,13 @@ constexpr uint64_t kValue48 = (1ull << kGenShift) - 1;
 constexpr uint32_t kTabWords = 2 * 256 * 4;
 constexpr uint32_t kRingWords = 2 * kR / 2;
 constexpr uint32_t kWaveWords = kTabWords + kRingWords + 64;
-// Queue items a decode wavefront takes at once: consecutive tickets of a
-// buffer then run on one wavefront, which keeps the buffer's LUTs in its
-// LDS.  (Encode takes one at a time: a ticket's look-back waits for its
-// predecessor's count, and with runs that predecessor would be the last of
-// another wavefront's run, so the runs of a buffer would follow one another.)
-// An item waits only on lower items, each held by a running wavefront that
-// reaches it, so the launch drains.
-constexpr uint32_t kRun = 4;
+// Queue items a decode wavefront takes at once (BatchDecodeArgs::run, 1 or
+// 4): consecutive tickets of a buffer then run on one wavefront, which keeps
+// the buffer's LUTs in its LDS.  (Encode takes one at a time: a ticket's
+// look-back waits for its predecessor's count, and with runs that
+// predecessor would be the last of another wavefront's run, so the runs of a
+// buffer would follow one another.)  An item waits only on lower items, each
+// held by a running wavefront that reaches it, so the launch drains.
 
 template <typename T>
 using gptr = const __attribute__((address_space(1))) T*;
@@ -111,7 +110,7 @@ __device__ uint64_t look_back_gen(uint64_t* status, uint64_t idx, uint64_t first
 // chunk / 64 / P symbols of a plane (16384 for 256 KiB chunks), so the halves
 // cannot carry into each other.  Loads: eight 16-byte loads per lane in
 // flight while the previous eight are counted.
-static_assert(kBatchHistChunk / 16u < 65536u, "16-bit pair counters");
+static_assert(kBatchHistChunkMax / 16u < 65536u, "16-bit pair counters");
 static_assert(4u * 128u * (16u / 4u) <= kTabWords, "counters fit the wave's table area");
 constexpr int kHistLoads = 8;
 
@@ -235,20 +234,25 @@ __device__ void finish_buffer(const BatchBuf& B, uint32_t b, const BatchEncodeAr
 }
 
 // ---- encode ticket --------------------------------------------------------
-// An encode ticket of a buffer of EB-byte elements is kRounds rounds of
-// enc_chains(EB) units: four chains per lane (four blocks of a uint8 buffer,
-// or both planes of two uint16 blocks) or two (two planes of one uint32
-// block), so a lane runs four (two) interleaved chains per round.  One queue
-// item, one table load and one look-back serve the whole ticket.
-constexpr uint32_t kRounds = 4;
+// An encode ticket of a buffer of EB-byte elements is `rounds` (1, 2 or 4,
+// BatchEncodeArgs::rounds) rounds of enc_chains(EB) units: four chains per
+// lane (four blocks of a uint8 buffer, or both planes of two uint16 blocks)
+// or two (two planes of one uint32 block), so a lane runs four (two)
+// interleaved chains per round.  One queue item, one table load and one
+// look-back serve the whole ticket.
+constexpr uint32_t kMaxRounds = 4;
 __host__ __device__ constexpr uint32_t enc_chains(uint32_t eb) { return eb == 4 ? 2u : 4u; }
-__host__ __device__ constexpr uint32_t enc_ticket_units(uint32_t eb) { return kRounds * enc_chains(eb); }
+__host__ __device__ constexpr uint32_t enc_ticket_units(uint32_t eb, uint32_t rounds) {
+  return rounds * enc_chains(eb);
+}
 
-// Ticket lt of buffer B: units u0 .. u0 + UT - 1, u0 = UT lt, coded round
-// by round with the single-buffer launch's multi-chain step (enc_step64f, a
-// linear LDS stage of H + 256 words per chain, spilled to the wave's slot in
-// chunks of H; unit j of the ticket has slot area j).  uint32 tickets code
-// planes 0-1 of both blocks, then planes 2-3, so the tables change once.
+// Ticket lt of buffer B: units u0 .. u0 + UT - 1, u0 = UT lt, UT = rounds x
+// chains, coded round by round with the single-buffer launch's multi-chain
+// step (enc_step64f, a linear LDS stage of H + 256 words per chain, spilled
+// to the wave's slot in chunks of H; unit j of the ticket has slot area j).
+// uint32 tickets of four rounds code planes 0-1 of both blocks, then planes
+// 2-3, so the tables change once; of two rounds, planes 0-1 then 2-3 of one
+// block; of one round, two planes of a block.
 // The ticket's offset comes from a look-back within its buffer once all its
 // rounds are coded; then block word offsets and words go to their place, in
 // unit order.  tkey: which tables the wave's LDS holds ((buffer << 4) |
@@ -256,8 +260,9 @@ __host__ __device__ constexpr uint32_t enc_ticket_units(uint32_t eb) { return kR
 template <int PB, int EB, int KBB>
 __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
                               uint32_t gen, uint32_t* wl, uint16_t* spill, uint64_t& tkey) {
-  constexpr int CH = (int)enc_chains(EB);        // chains per round
-  constexpr int UT = (int)enc_ticket_units(EB);  // units per ticket
+  constexpr int CH = (int)enc_chains(EB);                    // chains per round
+  constexpr int UT = (int)enc_ticket_units(EB, kMaxRounds);  // units per ticket, at most
+  const uint32_t R = a.rounds, UTR = R * (uint32_t)CH;      // rounds, units of this ticket
   constexpr int SPB = EB == 4 ? 2 : EB;          // chains per element block (planes coded together)
   constexpr int BPT = CH / SPB;                  // element blocks per round
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
@@ -268,7 +273,7 @@ __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t P = EB;
   const uint64_t U = (uint64_t)P * B.nb;
-  const uint64_t u0 = (uint64_t)UT * lt;
+  const uint64_t u0 = (uint64_t)UTR * lt;
   uint4* s_tab = (uint4*)wl;  // [SPB <= 2][256]
   uint16_t* wstage = (uint16_t*)(wl + kTabWords);
   const uint32_t dummy_addr = lds_addr(wstage + CH * REG + 2u * lane);
@@ -277,12 +282,12 @@ __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64
   for (int j = 0; j < UT; ++j) padj[j] = 0, cntj[j] = 0;
 
 #pragma unroll 1
-  for (uint32_t r = 0; r < kRounds; ++r) {
-    // the round's units: position j0 + c in the ticket (uint32: block r % 2,
-    // planes 2 (r / 2) + c)
-    const uint32_t j0 = EB == 4 ? 4u * (r % 2u) + 2u * (r / 2u) : (uint32_t)CH * r;
+  for (uint32_t r = 0; r < R; ++r) {
+    // the round's units: position j0 + c in the ticket (uint32 of four
+    // rounds: block r % 2, planes 2 (r / 2) + c)
+    const uint32_t j0 = EB == 4 && R == 4 ? 4u * (r % 2u) + 2u * (r / 2u) : (uint32_t)CH * r;
     if (u0 + j0 >= U) continue;  // (uint32: the second block's units may be missing)
-    const uint32_t p0 = EB == 4 ? 2u * (r / 2u) : 0u;  // plane of chain 0
+    const uint32_t p0 = EB == 4 ? (uint32_t)((u0 + j0) % 4u) : 0u;  // plane of chain 0
     uint32_t x[CH], n[CH], fl[CH], cb[CH], sb[CH];
     uint64_t blk[CH];
     bool act[CH];
@@ -445,7 +450,7 @@ __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64
 #pragma unroll
     for (int j = 0; j < UT; ++j) {
       const uint64_t u = u0 + j;
-      if (u < U) {
+      if ((uint32_t)j < UTR && u < U) {
         const uint32_t kp = (uint32_t)(u % P);
         ((uint32_t*)(B.out + B.section[kp] + kSectionFixed + 4ull * kL * B.nb + 8ull * (u / P)))[1] = (uint32_t)y;
       }
@@ -525,8 +530,8 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
       }
       tkey = ~0ull;
       const uint64_t bytes = B.n * B.eb;
-      const uint64_t c0 = (uint64_t)li * kBatchHistChunk;
-      const uint64_t c1 = c0 + kBatchHistChunk < bytes ? c0 + kBatchHistChunk : bytes;
+      const uint64_t c0 = (uint64_t)li * a.hist_chunk;
+      const uint64_t c1 = c0 + a.hist_chunk < bytes ? c0 + a.hist_chunk : bytes;
       if (c0 < c1) hist_chunk(B, b, c0, c1, wl, a.hist);
       publish();
       uint32_t d = 0;
@@ -910,9 +915,10 @@ __global__ __launch_bounds__(64 * NW) void batch_decode_kernel(BatchDecodeArgs a
   const uint32_t gen = wave_u32(lane == 0 ? atomic_ld32(a.queue + 2) : 0u);
   uint64_t tkey = ~0ull;  // LUTs held in this wave's LDS
   uint32_t it = 0;
-  if (lane == 0) it = atomicAdd(a.queue, kRun);
+  const uint32_t run = a.run;
+  if (lane == 0) it = atomicAdd(a.queue, run);
   it = wave_u32(it);
-  uint32_t run_end = it + kRun;
+  uint32_t run_end = it + run;
   while (it < a.nitems) {
     const uint32_t b = wave_u32(a.items[it].x);
     const BatchBuf& B = a.bufs[b];
@@ -932,9 +938,9 @@ __global__ __launch_bounds__(64 * NW) void batch_decode_kernel(BatchDecodeArgs a
     drain();
     if (++it == run_end) {
       uint32_t nxt = 0;
-      if (lane == 0) nxt = atomicAdd(a.queue, kRun);
+      if (lane == 0) nxt = atomicAdd(a.queue, run);
       it = wave_u32(nxt);
-      run_end = it + kRun;
+      run_end = it + run;
     }
   }
   if (lane == 0) {
@@ -959,7 +965,7 @@ hipError_t launch_dec_shape(const BatchDecodeArgs& a, int cu, hipStream_t s) {
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     uint64_t grid = (uint64_t)cu * (uint64_t)per_cu;
-    const uint64_t want = div_up(a.nitems, (uint64_t)NW * kRun);
+    const uint64_t want = div_up(a.nitems, (uint64_t)NW * a.run);
     if (want < grid) grid = want;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL((batch_decode_kernel<PB, NW, KB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);

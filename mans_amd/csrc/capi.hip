@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstddef>
 #include <cmath>
 #include <cstdio>
@@ -17,6 +18,8 @@
 
 #include "../../include/mans.h"
 #include "mans_internal.hpp"
+
+#include <atomic>
 
 using namespace mans;
 
@@ -1045,8 +1048,10 @@ struct mans_batch {
   uint32_t* fin = nullptr;    // [cap_bufs]
   uint32_t* queue = nullptr;  // [3] next item, exited waves, generation
   uint32_t* luts = nullptr;   // decode: [cap_bufs][4][2^p]
-  BatchBuf* h_bufs = nullptr; // pinned staging for asynchronous plans (or null)
-  uint2* h_items = nullptr;
+  BatchBuf* h_bufs = nullptr; // pinned staging for asynchronous plans (or null):
+                              // descriptors then items, one upload
+  const uint2* items_dev = nullptr;  // the plan's items (item_buf, or after the
+                                     // descriptors in `bufs` for pinned plans)
   // the batch's own encoder spill areas, sized at create for every launch
   // shape, so neither a shape change nor the workspace can move them under
   // a captured run
@@ -1060,6 +1065,7 @@ struct mans_batch {
   BatchShape shape = kBatchEncodeDefault;
   uint64_t total_bytes = 0;
   uint32_t width_mask = 0;
+  BatchGrain grain = {kBatchHistChunkMax, 4, 4};  // work-item grain of the plan
 };
 
 namespace {
@@ -1070,14 +1076,47 @@ void batch_free(mans_batch* b) {
   for (void* q : dev)
     if (q) (void)hipFree(q);
   if (b->h_bufs) (void)hipHostFree(b->h_bufs);
-  if (b->h_items) (void)hipHostFree(b->h_items);
   delete b;
 }
 
+// A grain forced by mans_batch_set_grain (0: picked per plan).
+std::atomic<uint32_t> g_force_chunk{0}, g_force_rounds{0}, g_force_run{0};
+
+// The work-item grain of a batch of `bytes` uncompressed bytes in `units`
+// byte-plane blocks on a chip of `cu` compute units: the largest items that
+// still leave about 32 histogram chunks, 32 encode tickets and 128 decode
+// runs per CU (a launch keeps 8-16 wavefronts per CU busy and the queue
+// balances their tails), within the bounds of each.  4096 buffers of 8 GB
+// take the largest grain; one request of a few tens of MB the smallest.
+BatchGrain batch_grain(uint64_t bytes, uint64_t units, int cu) {
+  const uint64_t c = (uint64_t)std::max(cu, 1);
+  BatchGrain g{kBatchHistChunkMin, 1, 1};
+  while (g.hist_chunk < kBatchHistChunkMax && bytes / (2ull * g.hist_chunk) >= 32 * c) g.hist_chunk *= 2;
+  while (g.rounds < 4 && units / (4ull * 2 * g.rounds) >= 32 * c) g.rounds *= 2;
+  if (units / (2ull * 4) >= 128 * c) g.run = 4;
+  const uint32_t fc = g_force_chunk.load(), fr = g_force_rounds.load(), fu = g_force_run.load();
+  if (fc) g.hist_chunk = fc;
+  if (fr) g.rounds = fr;
+  if (fu) g.run = fu;
+  return g;
+}
+
+// The grain of a batch of these buffers (elem_bytes 0 counts as 1).
+BatchGrain grain_of(const mans_buffer* h_bufs, int32_t count, int cu) {
+  uint64_t bytes = 0, units = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    const uint64_t eb = (uint64_t)std::max<int32_t>(h_bufs[i].elem_bytes, 1);
+    bytes += h_bufs[i].n_elems * eb;
+    units += eb * div_up(h_bufs[i].n_elems, (uint64_t)format_for_lanes(64).block_bytes);
+  }
+  return batch_grain(bytes, units, cu);
+}
+
 // Lays out every buffer of a batch as the 64-lane byte-plane frame of
-// mans_frame_encode and counts histogram items and tickets (two units each).
+// mans_frame_encode and counts its histogram items (g.hist_chunk bytes) and
+// tickets (decode: two units; encode: g.rounds rounds of enc_chains units).
 mans_status plan_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, bool decode,
-                       std::vector<BatchBuf>& bufs, uint64_t& nhist, uint64_t& nticks) {
+                       const BatchGrain& g, std::vector<BatchBuf>& bufs, uint64_t& nhist, uint64_t& nticks) {
   const Format f = format_for_lanes(64);
   bufs.assign((size_t)count, BatchBuf{});
   nhist = nticks = 0;
@@ -1118,11 +1157,11 @@ mans_status plan_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t 
     B.data_off = off;
     B.eb = eb;
     B.hist_first = (uint32_t)nhist;
-    B.nhist = (uint32_t)std::max<uint64_t>(1, div_up(m.n_elems * eb, kBatchHistChunk));
+    B.nhist = (uint32_t)std::max<uint64_t>(1, div_up(m.n_elems * eb, g.hist_chunk));
     B.tick_first = (uint32_t)nticks;
-    // decode tickets: two units; encode tickets: four rounds of four chains
-    // (two for uint32), as enc_ticket_units in batch_kernels.hpp
-    B.nticks = (uint32_t)div_up(eb * nb, decode ? 2u : (eb == 4 ? 8u : 16u));
+    // decode tickets: two units; encode tickets: g.rounds rounds of four
+    // chains (two for uint32), as enc_ticket_units in batch_kernels.hpp
+    B.nticks = (uint32_t)div_up(eb * nb, decode ? 2u : g.rounds * (eb == 4 ? 2u : 4u));
     B.in_bytes = decode ? m.in_bytes : 0;
     frame_header(B.frame_hdr, m.n_elems, eb, MANS_MODE_PLANES, f, p->prob_bits, d, 0, 0);
     put64(B.frame_hdr, 56, off);  // total size of an empty frame
@@ -1141,8 +1180,16 @@ mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, ui
   b->cap_bufs = cap_bufs;
   b->cap_items = cap_items;
   b->cap_ticks = cap_ticks;
-  HIP_TRY(hipMalloc(&b->bufs, nb1 * sizeof(BatchBuf)));
-  HIP_TRY(hipMalloc(&b->item_buf, std::max<size_t>(1, cap_items) * sizeof(uint2)));
+  // pinned plans upload descriptors and items in one copy: the items follow
+  // the plan's descriptors in the same allocation
+  const size_t blob = nb1 * sizeof(BatchBuf) + 16 + std::max<size_t>(1, cap_items) * sizeof(uint2);
+  if (pinned) {
+    HIP_TRY(hipMalloc(&b->bufs, blob));
+  } else {
+    HIP_TRY(hipMalloc(&b->bufs, nb1 * sizeof(BatchBuf)));
+    HIP_TRY(hipMalloc(&b->item_buf, std::max<size_t>(1, cap_items) * sizeof(uint2)));
+    b->items_dev = b->item_buf;
+  }
   HIP_TRY(hipMalloc(&b->ready, nb1 * 4));
   HIP_TRY(hipMalloc(&b->queue, 12));
   HIP_TRY(hipMemset(b->ready, 0, nb1 * 4));
@@ -1167,8 +1214,7 @@ mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, ui
     HIP_TRY(hipMemset(b->fin, 0, nb1 * 4));
   }
   if (pinned) {
-    HIP_TRY(hipHostMalloc((void**)&b->h_bufs, nb1 * sizeof(BatchBuf), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&b->h_items, std::max<size_t>(1, cap_items) * sizeof(uint2), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&b->h_bufs, blob, hipHostMallocDefault));
   }
   return MANS_OK;
 }
@@ -1179,12 +1225,14 @@ mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, ui
 mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h_bufs, int32_t count, hipStream_t s) {
   std::vector<BatchBuf> bufs;
   uint64_t nhist = 0, nticks = 0;
-  mans_status st = plan_batch(p, h_bufs, count, b->decode, bufs, nhist, nticks);
+  const BatchGrain grain = grain_of(h_bufs, count, b->ws->cu);
+  mans_status st = plan_batch(p, h_bufs, count, b->decode, grain, bufs, nhist, nticks);
   if (st != MANS_OK) return st;
   const uint64_t nitems = (b->decode ? (uint64_t)count : nhist) + nticks;
   if ((uint32_t)count > b->cap_bufs || nitems > b->cap_items || nticks > b->cap_ticks) return MANS_ERR_TOO_LARGE;
   std::vector<uint2> items;
-  uint2* dst = b->h_items;
+  const size_t items_off = round_up((uint64_t)count * sizeof(BatchBuf), 16);
+  uint2* dst = b->h_bufs ? (uint2*)((char*)b->h_bufs + items_off) : nullptr;
   if (!dst) {
     items.resize(std::max<size_t>(1, nitems));
     dst = items.data();
@@ -1216,8 +1264,9 @@ mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h
   }
   if (b->h_bufs) {
     if (count > 0) memcpy(b->h_bufs, bufs.data(), bufs.size() * sizeof(BatchBuf));
-    if (count > 0) HIP_TRY(hipMemcpyAsync(b->bufs, b->h_bufs, bufs.size() * sizeof(BatchBuf), hipMemcpyHostToDevice, s));
-    if (nitems > 0) HIP_TRY(hipMemcpyAsync(b->item_buf, dst, nitems * sizeof(uint2), hipMemcpyHostToDevice, s));
+    const size_t up = items_off + nitems * sizeof(uint2);
+    if (up > 0) HIP_TRY(hipMemcpyAsync(b->bufs, b->h_bufs, up, hipMemcpyHostToDevice, s));
+    b->items_dev = (const uint2*)((const char*)b->bufs + items_off);
   } else {
     if (count > 0) HIP_TRY(hipMemcpy(b->bufs, bufs.data(), bufs.size() * sizeof(BatchBuf), hipMemcpyHostToDevice));
     if (nitems > 0) HIP_TRY(hipMemcpy(b->item_buf, dst, nitems * sizeof(uint2), hipMemcpyHostToDevice));
@@ -1239,6 +1288,7 @@ mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h
     b->archive_max += round_up(fm, 16);
   }
   b->nbufs = (uint32_t)count;
+  b->grain = grain;
   b->nhist = b->decode ? 0u : (uint32_t)nhist;
   b->nticks = (uint32_t)nticks;
   b->nitems = (uint32_t)nitems;
@@ -1249,7 +1299,7 @@ BatchEncodeArgs batch_args(const mans_batch* b, uint64_t* d_sizes) {
   BatchEncodeArgs a;
   memset(&a, 0, sizeof(a));
   a.bufs = b->bufs;
-  a.items = b->item_buf;
+  a.items = b->items_dev;
   a.nbufs = b->nbufs;
   a.nhist_items = b->nhist;
   a.nitems = b->nitems;
@@ -1265,6 +1315,8 @@ BatchEncodeArgs batch_args(const mans_batch* b, uint64_t* d_sizes) {
   a.sizes = d_sizes;
   a.spill = b->spill_buf;
   a.spill_bytes = b->spill_cap;
+  a.hist_chunk = b->grain.hist_chunk;
+  a.rounds = b->grain.rounds;
   return a;
 }
 
@@ -1420,7 +1472,7 @@ mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buff
   const mans_buffer* h_bufs = given.data();
   std::vector<BatchBuf> bufs;
   uint64_t nhist = 0, nticks = 0;
-  st = plan_batch(p, h_bufs, count, decode, bufs, nhist, nticks);
+  st = plan_batch(p, h_bufs, count, decode, grain_of(h_bufs, count, ws->cu), bufs, nhist, nticks);
   if (st != MANS_OK) return st;
   DeviceGuard g(ws->device);
   mans_batch* b = new mans_batch();
@@ -1459,10 +1511,12 @@ mans_status batch_slot_create(mans_batch** out, const mans_params* p, bool decod
   b->decode = decode;
   b->shape = decode ? kBatchDecodeDefault : kBatchEncodeDefault;
   // units of cap_bytes in 8 KiB blocks, plus a partial block per plane and
-  // buffer; tickets hold two units
+  // buffer; tickets hold two units or more.  Histogram chunks: batch_grain
+  // keeps chunks below kBatchHistChunkMax only while there are fewer than
+  // 64 per CU, plus a partial chunk per buffer.
   const uint64_t units = cap_bytes / 8192 + 4ull * cap_bufs;
   const uint64_t ticks = units / 2 + cap_bufs;
-  const uint64_t hist = cap_bytes / kBatchHistChunk + cap_bufs;
+  const uint64_t hist = cap_bytes / kBatchHistChunkMax + 64ull * (uint64_t)std::max(ws->cu, 1) + cap_bufs;
   st = batch_alloc(b, cap_bufs, (decode ? cap_bufs : hist) + ticks, ticks, true);
   if (st == MANS_OK && !decode) st = batch_alloc_spill(b, 0x7fffffffu, &b->shape);
   if (st != MANS_OK) {
@@ -1484,6 +1538,13 @@ mans_status batch_slot_plan(mans_batch* b, const mans_params* p, const mans_buff
 // the workspace's sticky error word that only decode launches raise.
 const uint32_t* batch_slot_bad(const mans_batch* b) { return b ? b->bad : nullptr; }
 uint32_t batch_decode_error_bits() { return kErrMagic | kErrBlockWords | kErrFinalState; }
+// The workspace's sticky error word, and its reset enqueued on stream s.
+const uint32_t* workspace_error_word(const mans_workspace* ws) { return ws->counters + 3; }
+mans_status workspace_clear_error(mans_workspace* ws, hipStream_t s) {
+  DeviceGuard g(ws->device);
+  HIP_TRY(launch_fill_u32(ws->counters + 3, 1, 0, s));
+  return MANS_OK;
+}
 
 extern "C" {
 
@@ -1518,7 +1579,7 @@ mans_status mans_batch_decode_run(mans_batch* b, void* stream) {
   BatchDecodeArgs a;
   memset(&a, 0, sizeof(a));
   a.bufs = b->bufs;
-  a.items = b->item_buf;
+  a.items = b->items_dev;
   a.nbufs = b->nbufs;
   a.nitems = b->nitems;
   a.ready = b->ready;
@@ -1526,12 +1587,24 @@ mans_status mans_batch_decode_run(mans_batch* b, void* stream) {
   a.queue = b->queue;
   a.err = ws->counters + 3;
   a.bad = b->bad;
+  a.run = b->grain.run;
   // per-buffer flags start at zero in every run (a kernel, so the run can
   // be captured)
   HIP_TRY(launch_fill_u32(b->bad, b->nbufs, 0, s));
   prof_mark(ws, s, "batch_decode");
   HIP_TRY(launch_batch_decode(a, b->pb, b->shape, ws->cu, s));
   prof_mark(ws, s, nullptr);
+  return MANS_OK;
+}
+
+mans_status mans_batch_set_grain(uint32_t hist_chunk, uint32_t rounds, uint32_t run) {
+  const bool chunk_ok = hist_chunk == 0 || (hist_chunk >= kBatchHistChunkMin && hist_chunk <= kBatchHistChunkMax &&
+                                           (hist_chunk & (hist_chunk - 1)) == 0);
+  if (!chunk_ok || (rounds != 0 && rounds != 1 && rounds != 2 && rounds != 4) || (run != 0 && run != 1 && run != 4))
+    return MANS_ERR_INVALID_ARG;
+  g_force_chunk.store(hist_chunk);
+  g_force_rounds.store(rounds);
+  g_force_run.store(run);
   return MANS_OK;
 }
 

@@ -242,6 +242,8 @@ struct BatchEncodeArgs {
   uint64_t* sizes;               // [nbufs] frame sizes out
   uint16_t* spill;               // per-wave spill areas
   uint64_t spill_bytes;
+  uint32_t hist_chunk;           // bytes per histogram item (BatchGrain)
+  uint32_t rounds;               // rounds per encode ticket (BatchGrain)
 };
 // Batched decode: BatchBuf.in is the frame, .out the elements; items
 // [0, nbufs) build buffer b's LUTs, then come the decode tickets.
@@ -255,10 +257,22 @@ struct BatchDecodeArgs {
   uint32_t* queue;               // [3] as BatchEncodeArgs::queue
   uint32_t* err;
   uint32_t* bad;                 // [nbufs] the error flags of each buffer (zeroed before the launch)
+  uint32_t run;                  // items a wavefront takes per queue grab (BatchGrain)
 };
 constexpr uint32_t kBatchBad = 1u << 31;
 constexpr uint32_t kBatchGenMask = 0x3fffu;     // 14-bit generations
-constexpr uint32_t kBatchHistChunk = 1u << 19;  // bytes per histogram item
+// Work-item grain of a batched launch, picked per plan from the batch's
+// size against the chip (batch_grain in capi.hip): a large batch takes big
+// items (fewer queue atomics, table loads and look-backs per byte), a small
+// one small items, so that a few requests still spread over every CU.  The
+// grain changes no output byte.
+constexpr uint32_t kBatchHistChunkMin = 1u << 16;  // bytes per histogram item
+constexpr uint32_t kBatchHistChunkMax = 1u << 19;
+struct BatchGrain {
+  uint32_t hist_chunk;  // power of two in [kBatchHistChunkMin, kBatchHistChunkMax]
+  uint32_t rounds;      // encode ticket rounds: 1, 2 or 4
+  uint32_t run;         // decode items per grab: 1 or 4
+};
 
 // ---- archive: the frames of a batch in one container (archive.hip) --------
 constexpr uint32_t kArchiveMagic = 0x414E414Du;  // "MANA"

@@ -38,6 +38,8 @@ mans_status batch_slot_plan(mans_batch* b, const mans_params* p, const mans_buff
                             hipStream_t s);
 const uint32_t* batch_slot_bad(const mans_batch* b);
 uint32_t batch_decode_error_bits();
+const uint32_t* workspace_error_word(const mans_workspace* ws);
+mans_status workspace_clear_error(mans_workspace* ws, hipStream_t s);
 
 namespace {
 
@@ -112,6 +114,7 @@ struct Slot {
   uint64_t* d_sizes = nullptr;
   uint64_t* h_sizes = nullptr;  // pinned
   uint32_t* h_bad = nullptr;    // pinned: per-request error flags of the decode batch
+  uint32_t* h_err = nullptr;    // pinned: the slot workspace's error word after the launch
   hipEvent_t done = nullptr;
   std::vector<Req*> enc_reqs, dec_reqs;
   bool busy = false;
@@ -194,6 +197,12 @@ void launch_slot(mans_server* S, Slot& sl) {
     if (st != MANS_OK)
       for (Req* q : sl.dec_reqs) q->status = st;
   }
+  // the error word comes back with the results: finishing the slot needs no
+  // further copy or synchronisation
+  if (hipMemcpyAsync(sl.h_err, workspace_error_word(sl.ws), 4, hipMemcpyDeviceToHost, sl.stream) != hipSuccess) {
+    for (Req* q : sl.enc_reqs) q->status = MANS_ERR_HIP;
+    for (Req* q : sl.dec_reqs) q->status = MANS_ERR_HIP;
+  }
   (void)hipEventRecord(sl.done, sl.stream);
 }
 
@@ -202,9 +211,12 @@ void launch_slot(mans_server* S, Slot& sl) {
 // batch's per-buffer flags); the slot's sticky error word fails the
 // compress requests only for bits an encode launch raises.
 void finish_slot(mans_server* S, Slot& sl) {
-  uint32_t flags = 0;
-  const mans_status chk = mans_workspace_check(sl.ws, sl.stream, &flags);
-  const bool enc_bad = chk != MANS_OK && (flags & ~batch_decode_error_bits()) != 0;
+  const uint32_t flags = *sl.h_err;
+  // reset before the slot's next launch (same stream)
+  // (were the reset to fail, the next launch's compress requests would fail
+  // too: never a wrong success)
+  if (flags != 0) (void)workspace_clear_error(sl.ws, sl.stream);
+  const bool enc_bad = (flags & ~batch_decode_error_bits()) != 0;
   const auto now = Clock::now();
   for (size_t i = 0; i < sl.enc_reqs.size(); ++i) {
     Req* q = sl.enc_reqs[i];
@@ -314,7 +326,8 @@ mans_status mans_server_create(mans_server** out, const mans_params* p, uint64_t
     if (st == MANS_OK) st = batch_slot_create(&sl.dec, p, true, kSlotBufs, kSlotBytes, sl.ws);
     if (st == MANS_OK && (hipMalloc(&sl.d_sizes, kSlotBufs * 8) != hipSuccess ||
                           hipHostMalloc((void**)&sl.h_sizes, kSlotBufs * 8, hipHostMallocDefault) != hipSuccess ||
-                          hipHostMalloc((void**)&sl.h_bad, kSlotBufs * 4, hipHostMallocDefault) != hipSuccess))
+                          hipHostMalloc((void**)&sl.h_bad, kSlotBufs * 4, hipHostMallocDefault) != hipSuccess ||
+                          hipHostMalloc((void**)&sl.h_err, 4, hipHostMallocDefault) != hipSuccess))
       st = MANS_ERR_HIP;
   }
   if (st == MANS_OK) {
@@ -451,6 +464,7 @@ mans_status mans_server_destroy(mans_server* S) {
     if (sl.d_sizes) (void)hipFree(sl.d_sizes);
     if (sl.h_sizes) (void)hipHostFree(sl.h_sizes);
     if (sl.h_bad) (void)hipHostFree(sl.h_bad);
+    if (sl.h_err) (void)hipHostFree(sl.h_err);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }

@@ -354,6 +354,47 @@ def test_batch_every_shape_is_bit_identical(ws, p):
     enc.close()
 
 
+GRAINS = [(1 << 16, 1, 1), (1 << 17, 2, 4), (1 << 18, 4, 1), (1 << 19, 4, 4), (1 << 19, 1, 4), (1 << 16, 2, 1)]
+
+
+@pytest.mark.parametrize("grain", GRAINS, ids=lambda g: f"chunk{g[0] >> 10}k-r{g[1]}-run{g[2]}")
+def test_batch_every_grain_is_bit_identical(ws, grain):
+    """Every work-item grain (histogram chunk, encode ticket rounds, decode
+    run) gives the frames of the single-buffer encode, and decodes back."""
+    rng = np.random.default_rng(50)
+    sizes = EDGE_SIZES + [int(v) for v in np.exp(rng.uniform(np.log(1 << 10), np.log(3 << 20), 24))]
+    xs, ts = make_batch(50, sizes)
+    prm = mans_amd.params(10, 64)
+    print(f"grain {grain} buffers={len(xs)}", flush=True)
+    mans_amd.set_batch_grain(*grain)
+    try:
+        enc = mans_amd.EncodeBatch(ts, prm, ws)
+        got_sizes = enc.run().cpu().tolist()
+        ws.check()
+        for i, (x, t) in enumerate(zip(xs, ts)):
+            q = mans_amd.params(10, 64, x.itemsize, mans_amd.MODE_PLANES)
+            ref, n = mans_amd.frame_encode(t, q, ws)
+            assert got_sizes[i] == n, (grain, i, x.size, x.itemsize)
+            assert torch.equal(enc.outs[i][:n], ref[:n]), (grain, i, x.size, x.itemsize)
+        dec = mans_amd.DecodeBatch(enc.outs, got_sizes, ts, prm, ws)
+        outs = dec.run()
+        torch.cuda.synchronize()
+        ws.check()
+        for i, x in enumerate(xs):
+            assert (outs[i].cpu().numpy().view(x.dtype) == x).all(), (grain, i, x.size, x.itemsize)
+        dec.close()
+        enc.close()
+    finally:
+        mans_amd.set_batch_grain(0, 0, 0)
+
+
+def test_batch_grain_rejects_bad_values():
+    for g in [(1 << 15, 0, 0), (3 << 16, 0, 0), (1 << 20, 0, 0), (0, 3, 0), (0, 8, 0), (0, 0, 2)]:
+        with pytest.raises(mans_amd.MansError):
+            mans_amd.set_batch_grain(*g)
+    mans_amd.set_batch_grain(0, 0, 0)
+
+
 def test_batch_tune_picks_the_fastest_and_caches_it(ws):
     rng = np.random.default_rng(40)
     sizes = [int(v) for v in np.exp(rng.uniform(np.log(4096), np.log(1 << 21), 96))]
