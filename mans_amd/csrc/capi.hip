@@ -27,7 +27,8 @@ struct mans_workspace {
   int device = 0;
   int cu = 0;
   uint32_t* hist = nullptr;      // [4][256] self-cleaning accumulators
-  uint32_t* counters = nullptr;  // 0 done, 1 encode ticket, 2 adm ticket, 3 err, 4.. scratch scalars
+  uint32_t* counters = nullptr;  // 0 done, 1 encode ticket, 2 adm ticket, 3 err, 4-5 scratch scalars,
+                                 // 8-9 histogram row grabs
   uint4* table = nullptr;        // [4][256]
   uint32_t* hist_out = nullptr;  // [4][256]
   uint64_t* status = nullptr;
@@ -225,6 +226,7 @@ mans_status launch_encode_pair(const CodecDesc& d, const HistSource* src, int ns
   }
   h.hist = sc.hist;
   h.done = sc.counters + 0;
+  h.grab = sc.counters + 8;
   h.table = sc.table;
   h.out = out;
   h.version = f.version;
@@ -297,7 +299,7 @@ constexpr uint64_t kNoAllocMaxBytes = 1ull << 30;
 constexpr uint64_t kNoAllocStatus = kNoAllocMaxBytes / 4096 / 2;  // one look-back word per two blocks
 struct NoAllocLayout {
   uint32_t hist[4 * 256];    // histogram accumulators (cleared by each call)
-  uint32_t counters[64];     // 0 done, 1 ticket, 3 error flags (cleared by each call)
+  uint32_t counters[64];     // 0 done, 1 ticket, 3 error flags, 8-9 row grabs (cleared by each call)
   uint4 table[4 * 256];      // normalised tables
   uint64_t status[kNoAllocStatus];  // look-back words (cleared by the histogram launch)
 };
@@ -609,6 +611,7 @@ mans_status mans_histogram(const void* d_in, uint64_t n, uint32_t stride, uint32
   for (uint32_t k = 0; k < stride; ++k) h.plane_nsym[k] = n;
   h.hist = ws->hist;
   h.done = ws->counters + 0;
+  h.grab = ws->counters + 8;
   h.table = ws->table;
   h.hist_out = ws->hist_out;
   h.ticket = ws->counters + 1;
@@ -635,6 +638,7 @@ mans_status mans_normalize(const uint32_t* d_hist, uint32_t total, int32_t prob_
   h.plane_nsym[0] = total;
   h.hist = ws->hist;
   h.done = ws->counters + 0;
+  h.grab = ws->counters + 8;
   h.table = ws->table;
   h.ticket = ws->counters + 1;
   prof_mark(ws, s, "normalize");

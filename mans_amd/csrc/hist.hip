@@ -123,9 +123,57 @@ __device__ __forceinline__ void count_u4(uint32_t* c, const uint4& v, uint32_t j
       atomicAdd(c + __builtin_amdgcn_ubfe(w[k], 8u * (j0 + (uint32_t)(q * S)), 8u) * kRep, 1u);
 }
 
+// Whole rows [r0, r1) of a source, counted by one wavefront: row r is the
+// 64 kLoads uint4s from r * 64 kLoads, lane l loading uint4s l, l + 64, ...
+// of it.  Two register buffers, so one row loads while the other is
+// counted; row indices are wave-uniform (scalar row base, one lane offset
+// for every load).
 template <int S>
-__device__ void count_source(const HistSource& src, uint32_t* c /* lane's replica base */, uint32_t j0,
-                             uint64_t slot, uint64_t nslots, bool edges) {
+__device__ void count_rows(const uint4* body, uint32_t r0, uint32_t r1, uint32_t* c, uint32_t j0) {
+  constexpr uint64_t row = 64ull * kLoads;
+  const uint32_t lane = threadIdx.x & 63;
+  uint4 bufA[kLoads], bufB[kLoads];
+  // (32-bit row indices: their compares stay scalar)
+  auto load = [&](uint4 (&buf)[kLoads], uint32_t r) {
+    const uint4* rp = body + (uint64_t)r * row;
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) buf[k] = rp[lane + 64u * (uint32_t)k];
+    // issue the row's loads here, ahead of the counting that follows (the
+    // scheduler would otherwise move counting of the row in flight above them)
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto count = [&](const uint4 (&buf)[kLoads]) {
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) count_u4<S>(c, buf[k], j0);
+  };
+  // (the prefetch past the last row re-reads that row: straight-line code,
+  // so the compiler keeps it in flight while the current row is counted)
+  uint32_t r = r0;
+  load(bufA, r);
+  for (;;) {
+    load(bufB, r + 1 < r1 ? r + 1 : r);
+    count(bufA);
+    if (++r >= r1) break;
+    load(bufA, r + 1 < r1 ? r + 1 : r);
+    count(bufB);
+    if (++r >= r1) break;
+  }
+}
+
+// Rows per group and grab.  Workgroups take their rows in chunks from a
+// device-wide counter instead of a fixed share: workgroups on different CUs
+// stream at different rates (a fixed share ends 1.6x apart from the fastest
+// to the slowest), so a static split leaves the launch waiting on its
+// slowest workgroup.
+constexpr uint32_t kChunkRows = 8;
+
+// One source, every wavefront of the workgroup taking part: the workgroup
+// grabs one chunk per group, each wave of the group counts its plane's bytes
+// of the group's chunk.  The next grab is taken while the current chunk is
+// counted (s_grab double-buffered, one barrier per chunk).
+template <int S>
+__device__ void count_source(const HistSource& src, uint32_t* c, uint32_t j0, bool active, uint32_t group,
+                             uint32_t groups, bool edges, uint32_t* grab, uint32_t* s_grab) {
   const uint8_t* p = src.ptr;
   const uint64_t n = src.nbytes;
   const uint32_t mis = (uint32_t)((uintptr_t)p & 15u);
@@ -134,41 +182,24 @@ __device__ void count_source(const HistSource& src, uint32_t* c /* lane's replic
   const uint64_t tail0 = head + 16u * n4;
   const uint4* body = (const uint4*)(p + head);
   const uint32_t lane = threadIdx.x & 63;
-  // lane-strided rows of kLoads uint4 per lane; two buffers so one row loads
-  // while the other is counted
-  const uint64_t row = 64ull * kLoads;
-  uint64_t r0 = slot * row + lane;
-  const uint64_t rstep = nslots * row;
-  uint4 bufA[kLoads], bufB[kLoads];
-  // loads are unconditional (indices past the end re-read the last uint4 and
-  // are not counted) so the wait before counting a row covers only that row
-  auto load = [&](uint4 (&buf)[kLoads], uint64_t base) {
-#pragma unroll
-    for (int k = 0; k < kLoads; ++k) {
-      const uint64_t i = base + 64ull * k;
-      buf[k] = body[i < n4 ? i : n4 - 1];
+  const uint64_t nrows = n4 / (64ull * kLoads);  // whole rows; the rest with the edges
+  const uint64_t nchunks = div_up(nrows, (uint64_t)kChunkRows);
+  if (threadIdx.x == 0) s_grab[0] = atomicAdd(grab, groups);
+  __syncthreads();
+  for (uint32_t k = 0;; ++k) {
+    const uint32_t cb = s_grab[k & 1u];
+    if (cb >= nchunks) break;
+    if (threadIdx.x == 0) s_grab[(k + 1) & 1u] = atomicAdd(grab, groups);
+    const uint64_t ch = (uint64_t)cb + group;
+    if (active && ch < nchunks) {
+      const uint64_t r1 = (ch + 1) * kChunkRows;
+      count_rows<S>(body, (uint32_t)(ch * kChunkRows), (uint32_t)(r1 < nrows ? r1 : nrows), c, j0);
     }
-  };
-  auto count = [&](const uint4 (&buf)[kLoads], uint64_t base) {
-#pragma unroll
-    for (int k = 0; k < kLoads; ++k)
-      if (base + 64ull * k < n4) count_u4<S>(c, buf[k], j0);
-  };
-  if (slot * row < n4) {
-    load(bufA, r0);
-    for (;;) {
-      const uint64_t r1 = r0 + rstep;
-      load(bufB, r1);
-      count(bufA, r0);
-      if (r1 - lane >= n4) break;
-      const uint64_t r2 = r1 + rstep;
-      load(bufA, r2);
-      count(bufB, r1);
-      if (r2 - lane >= n4) break;
-      r0 = r2;
-    }
+    __syncthreads();
   }
-  if (edges) {
+  if (edges && active) {
+    // the uint4s after the whole rows (fewer than a row)
+    for (uint64_t i = nrows * 64ull * kLoads + lane; i < n4; i += 64) count_u4<S>(c, body[i], j0);
     // unaligned head and tail (< 16 bytes each); offsets from the
     // element-aligned source start give the plane
     if (lane < head && lane % S == j0) atomicAdd(c + p[lane] * kRep, 1u);
@@ -183,7 +214,9 @@ __global__ __launch_bounds__(kThreads) void hist_kernel(HistArgs a) {
   uint32_t* s_keys = smem;                 // [256]  (finisher)
   uint32_t* s_tmp = smem + kNumSymbols;    // [kWaves]
   int* s_last = (int*)(smem + kNumSymbols + kWaves);
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  // (the wave index through readfirstlane: plane, group and every row index
+  // derived from them stay in scalar registers)
+  const int t = threadIdx.x, lane = t & 63, w = (int)__builtin_amdgcn_readfirstlane((uint32_t)t >> 6);
   const int P = a.num_planes;
   const int groups = kWaves / P;           // P divides kWaves (1, 2 or 4)
   const int plane = w % P, group = w / P;
@@ -198,16 +231,15 @@ __global__ __launch_bounds__(kThreads) void hist_kernel(HistArgs a) {
   for (int i = lane; i < kCountWords / 4; i += 64) ((uint4*)wc)[i] = make_uint4(0, 0, 0, 0);
   __threadfence_block();
   uint32_t* c = wc + (lane % kRep);
-  const uint64_t slot = (uint64_t)blockIdx.x * groups + group;
-  const uint64_t nslots = (uint64_t)gridDim.x * groups;
+  __shared__ uint32_t s_grab[2];
   for (int s = 0; s < a.nsrc; ++s) {
     const HistSource& src = a.src[s];
-    if (plane < (int)src.plane0 || plane >= (int)(src.plane0 + src.stride)) continue;
-    const uint32_t j0 = (uint32_t)plane - src.plane0;
+    const bool active = plane >= (int)src.plane0 && plane < (int)(src.plane0 + src.stride);
+    const uint32_t j0 = active ? (uint32_t)plane - src.plane0 : 0u;
     const bool edges = blockIdx.x == 0 && group == 0;
-    if (src.stride == 1) count_source<1>(src, c, j0, slot, nslots, edges);
-    else if (src.stride == 2) count_source<2>(src, c, j0, slot, nslots, edges);
-    else count_source<4>(src, c, j0, slot, nslots, edges);
+    if (src.stride == 1) count_source<1>(src, c, j0, active, group, groups, edges, a.grab + s, s_grab);
+    else if (src.stride == 2) count_source<2>(src, c, j0, active, group, groups, edges, a.grab + s, s_grab);
+    else count_source<4>(src, c, j0, active, group, groups, edges, a.grab + s, s_grab);
   }
   __threadfence_block();
   // merge the wave's replicas and add to the device histogram of its plane
@@ -257,6 +289,8 @@ __global__ __launch_bounds__(kThreads) void hist_kernel(HistArgs a) {
   }
   if (t == 0) {
     *a.done = 0;
+    a.grab[0] = 0;
+    a.grab[1] = 0;
     if (a.ticket) *a.ticket = 0;
   }
   if (a.write_headers && a.has_frame && t < 32) ((uint32_t*)a.out)[t] = a.frame_hdr[t];

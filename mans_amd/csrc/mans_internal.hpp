@@ -131,6 +131,7 @@ struct HistArgs {
   uint64_t plane_section[kMaxPlanes];
   uint32_t* hist;                // [P][256] self-cleaning accumulators
   uint32_t* done;                // self-cleaning WG counter
+  uint32_t* grab;                // [2] self-cleaning row-chunk counters, one per source
   uint4* table;                  // [P][256] {pdf, cdf, magic, shift}
   uint32_t* hist_out;            // optional copy of the final histogram
   uint8_t* out;                  // container base (headers), may be null

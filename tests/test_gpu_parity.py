@@ -50,7 +50,10 @@ def datasets(seed=0):
 @pytest.mark.parametrize("stride", [1, 2, 4])
 def test_histogram(ws, stride):
     rng = np.random.default_rng(stride)
-    for n in [0, 1, 15, 16, 17, 1000, 65_536 + 7, 1_000_003]:
+    # (8 KiB rows of 512 uint4s are counted in chunks of 8 rows: sizes on,
+    # around and between those boundaries)
+    for n in [0, 1, 15, 16, 17, 1000, 8192 // stride, 65_536 // stride, 65_536 + 7, 8192 * 43 // stride,
+              1_000_003]:
         raw = rng.integers(0, 256, n * stride + 64).astype(np.uint8)
         raw[: n * stride: 3] = 7  # a hot bin
         for off in ([0, 4, 8] if stride > 1 else [0, 1, 5, 15]):
@@ -225,6 +228,29 @@ def test_adm_map_matches_oracle_random(ws):
         assert ok == rok
         assert (host(offs) == roffs).all() and (host(cen).view(np.uint16) == rcen).all()
         assert (host(codes) == rcodes).all() and (host(bits) == rbits).all()
+
+
+def test_adm_map_multiword_lanes_match_oracle(ws):
+    # wide spreads: a lane's 16 elements take up to ~200 bits, so its bit
+    # string spans several 32-bit words (and pads inside a later word); whole
+    # and ragged 16-group tickets
+    rng = np.random.default_rng(22)
+    # (b = 1200 leaves some lanes beyond 256 bits: only the verdict is
+    # defined there)
+    for n, b in [(3 * 8192, 300.0), (5 * 8192 + 1000, 600.0), (8192 * 40 + 17, 700.0), (8192 * 8, 1200.0)]:
+        x = np.clip(32768 + np.round(rng.laplace(0, b, n)), 0, 65535).astype(np.uint16)
+        print(f"adm multiword n={n} b={b}", flush=True)
+        ok, offs, cen, codes, bits = mans_amd.adm_map(u16_dev(x), ws)
+        rok, roffs, rcen, rcodes, rbits = oracle.adm_map(x)
+        assert ok == rok
+        if b > 1000:
+            assert not ok
+            continue
+        assert ok
+        assert (host(offs) == roffs).all() and (host(cen).view(np.uint16) == rcen).all()
+        assert (host(codes) == rcodes).all() and (host(bits) == rbits).all()
+        back = host(mans_amd.adm_unmap(offs, cen, codes, bits, n, ws)).view(np.uint16)
+        assert (back == x).all()
 
 
 def test_adm_domain_flag(ws):
