@@ -423,7 +423,8 @@ class _Tunable:
 
 class EncodeBatch(_Tunable):
     """A planned batch of independent buffers, each coded as a byte-plane
-    frame (mans_batch_encode_create).  `inputs`: CUDA tensors of 1/2/4-byte
+    frame in p.lanes' format, 64 (native) or 32 (reference-format streams)
+    (mans_batch_encode_create).  `inputs`: CUDA tensors of 1/2/4-byte
     elements (16-byte aligned storage); frames go to `outs` (allocated at
     frame_max_size when not given).  run() only launches, so it can be
     captured in a CUDA/HIP graph; sizes are left on the device."""
@@ -435,7 +436,7 @@ class EncodeBatch(_Tunable):
         if outs is None:
             outs = []
             for x in self.inputs:
-                q = Params(p.prob_bits, 64, x.element_size(), MODE_PLANES)
+                q = Params(p.prob_bits, p.lanes, x.element_size(), MODE_PLANES)
                 outs.append(torch.empty(frame_max_size(x.numel(), q), dtype=torch.uint8, device=x.device))
         self.outs = list(outs)
         arr = (Buffer * max(1, len(self.inputs)))()
@@ -482,7 +483,7 @@ class EncodeBatch(_Tunable):
 
 class DecodeBatch(_Tunable):
     """A planned batch decode (mans_batch_decode_create) of byte-plane frames
-    (64 lanes): `frames` are uint8 CUDA tensors holding each frame (at least
+    of p.lanes' format: `frames` are uint8 CUDA tensors holding each frame (at least
     its size in bytes), `like` gives each buffer's element count and dtype
     (tensors, only their numel / dtype are used; with from_headers the
     library reads them from the frames and `like` only sizes the outputs).

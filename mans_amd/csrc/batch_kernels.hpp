@@ -211,7 +211,7 @@ __device__ void finish_buffer(const BatchBuf& B, uint32_t b, const BatchEncodeAr
         make_uint2((uint32_t)pr[0] | ((uint32_t)pr[1] << 16), (uint32_t)pr[2] | ((uint32_t)pr[3] << 16));
     if (lane < 8) {
       uint32_t f = 0;
-      if (lane == 0) f = (kAnsMagic << 16) | 2u;  // native format version
+      if (lane == 0) f = (kAnsMagic << 16) | (B.lanes == 32 ? 1u : 2u);  // reference / native format version
       if (lane == 1) f = (uint32_t)B.nb;
       if (lane == 2) f = (uint32_t)B.n;
       if (lane == 4) f = (uint32_t)PB;
@@ -219,7 +219,7 @@ __device__ void finish_buffer(const BatchBuf& B, uint32_t b, const BatchEncodeAr
       // ticket to finish; no other store may touch its bytes before that
       if (lane != 3 || B.nticks == 0) ((uint32_t*)sec)[lane] = f;
     }
-    if ((B.nb & 1) && lane < 2) ((uint32_t*)(sec + kSectionFixed + 4ull * kL * B.nb + 8ull * B.nb))[lane] = 0;
+    if ((B.nb & 1) && lane < 2) ((uint32_t*)(sec + kSectionFixed + 4ull * B.lanes * B.nb + 8ull * B.nb))[lane] = 0;
   }
   // frame header; data_words / total_size (bytes 48..63) belong to the
   // buffer's last ticket unless there is none
@@ -500,7 +500,201 @@ __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64
   }
 }
 
-template <int PB, int NW, int KB>
+// ---- 32-lane (reference-format) encode ticket ------------------------------
+// Units u0 = 2 lt and u0 + 1 of buffer B in the 32-lane format (4096-symbol
+// blocks): lanes 0-31 code the first and lanes 32-63 the second, one chain
+// per lane, with the single-buffer 32-lane step (enc_step32f: one ballot,
+// words compacted per half).  Each unit has a linear LDS stage of H + 256
+// words (checked every 4 steps, at most 32 words per step) and a slot area of
+// CSTAGE / 2 words in the wave's spill area.  The pair's planes (both planes
+// of a uint16 block, two of a uint32 block, or two blocks of uint8 plane 0)
+// have their tables at s_tab[kp - p0].  One look-back per ticket, as for the
+// 64-lane tickets.
+template <int PB, int EB, int KBB>
+__device__ void encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
+                                uint32_t gen, uint32_t* wl, uint16_t* spill, uint64_t& tkey) {
+  constexpr uint32_t L = 32, BLOCK = 4096;
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  constexpr uint32_t SSTAGE = CSTAGE / 2;  // a 32-lane unit's words at most, rounded
+  constexpr uint32_t H = 256, H4 = H / 8, REG = H + 256;
+  constexpr uint32_t NT = EB == 1 ? 1u : 2u;  // tables of the pair
+  static_assert(BLOCK / L % KBB == 0 && KBB % 4 == 0, "KBB steps per prefetch batch, checks every 4");
+  static_assert(2 * REG + 128 <= 2 * (kRingWords + 64), "stages and dummy slots fit the wave's stage area");
+  static_assert(SSTAGE % 8 == 0 && 2 * SSTAGE <= 16 * CSTAGE, "slot areas");
+  using T = typename ElemOf<EB>::T;
+  const uint32_t lane = threadIdx.x & 63u, half = lane >> 5, lin = lane & 31u;
+  const uint32_t P = EB;
+  const uint64_t U = (uint64_t)P * B.nb;
+  const uint64_t u0 = 2ull * lt;
+  // this lane's unit (its half's)
+  const uint64_t u = u0 + half;
+  const bool act = u < U;
+  const uint32_t kp = act ? (uint32_t)(u % P) : 0u;
+  const uint64_t blk = act ? u / P : 0u;
+  const uint32_t n = act ? (uint32_t)((B.n - blk * BLOCK) < BLOCK ? (B.n - blk * BLOCK) : BLOCK) : 0u;
+  const uint32_t p0 = (uint32_t)(u0 % P);
+  const bool act1 = u0 + 1 < U;  // the second unit exists (wave-uniform)
+  uint4* s_tab = (uint4*)wl;     // [NT][256]
+  uint16_t* wstage = (uint16_t*)(wl + kTabWords);
+  const uint32_t dummy_addr = lds_addr(wstage + 2 * REG + 2u * lane);
+  const uint64_t key = ((uint64_t)b << 4) | p0;
+  if (key != tkey) {
+    drain();  // the previous ticket's table reads are done
+#pragma unroll
+    for (uint32_t q = 0; q < NT; ++q) {
+      const uint64_t* g = (const uint64_t*)(a.tables + ((uint64_t)b * 4u + p0 + q) * 256u);
+      uint64_t* d = (uint64_t*)(s_tab + q * 256);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) d[lane + 64 * k] = atomic_ld64(g + lane + 64 * k);
+    }
+    tkey = key;
+  }
+  const uint4* tab = s_tab + (kp - p0) * 256u;
+  uint32_t x = kMinState;
+  // per unit h: LDS byte address of the stage and of the next word, words
+  // moved to the slot area
+  uint32_t cb[2], sb[2], fl[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    cb[h] = wave_u32(lds_addr(wstage + h * REG));
+    sb[h] = cb[h];
+    fl[h] = 0;
+  }
+  const uint32_t n0 = __builtin_amdgcn_readlane(n, 0), n1 = __builtin_amdgcn_readlane(n, 32);
+  const uint32_t n_min = act1 ? (n0 < n1 ? n0 : n1) : 0u;
+  const uint32_t n_max = n0 > n1 ? n0 : n1;
+  const uint32_t all = (uint32_t)div_up(n_max, L);
+  drain();
+
+  // a stage with H or more unspilled words moves its first H to its slot
+  // area and the rest (fewer than 256) to the front
+  auto spill_full = [&]() {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t un = wave_u32((sb[h] - cb[h]) >> 1);  // unspilled words
+      if (un >= H) {
+        uint16_t* st = wstage + h * REG;
+        if (lane < H4) ((uint4*)(spill + h * SSTAGE + fl[h]))[lane] = ((const uint4*)st)[lane];
+        if (4u * lane < un - H) ((uint2*)st)[lane] = ((const uint2*)(st + H))[lane];
+        fl[h] += H;
+        sb[h] = wave_u32(sb[h] - 2u * H);
+      }
+    }
+  };
+
+  const T* ep = (const T*)B.in + blk * BLOCK;
+  uint32_t s = 0;
+  if (n_min == BLOCK) {
+    auto load = [&](uint32_t (&buf)[KBB], uint32_t s0) {
+      const gptr<T> bp = (gptr<T>)ep + lin + (uint64_t)s0 * L;
+#pragma unroll
+      for (int k = 0; k < KBB; ++k) buf[k] = bp[k * L];
+    };
+    auto code = [&](const uint32_t (&buf)[KBB]) {
+#pragma unroll
+      for (int k = 0; k < KBB; ++k) {
+        const uint4 e = tab[(buf[k] >> (8u * kp)) & 0xffu];
+        enc_step32f(x, sb[0], sb[1], half, e);
+        if ((k & 3) == 3) spill_full();
+      }
+    };
+    const uint32_t last = BLOCK / L - KBB;
+    uint32_t bA[KBB], bB[KBB];
+    load(bA, 0);
+    for (;;) {
+      load(bB, s + KBB <= last ? s + KBB : last);
+      code(bA);
+      s += KBB;
+      if (s > last) break;
+      load(bA, s + KBB <= last ? s + KBB : last);
+      code(bB);
+      s += KBB;
+      if (s > last) break;
+    }
+  }
+  for (; s < all; ++s) {
+    const uint32_t i = s * L + lin;
+    const bool v = i < n;
+    const uint32_t w = v ? (uint32_t)ep[i] : 0u;
+    enc_step32(x, sb[0], sb[1], half, dummy_addr, tab[(w >> (8u * kp)) & 0xffu], v);
+    spill_full();
+  }
+
+  // pad each unit, its stage remainder to its slot area, states, block sizes
+  uint32_t pad[2], cnt[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t rem = (sb[h] - cb[h]) >> 1;
+    const bool acth = h ? act1 : true;
+    cnt[h] = fl[h] + rem;
+    pad[h] = acth ? (uint32_t)round_up(cnt[h], 8) : 0u;
+    if (cnt[h] > SSTAGE && lane == 0) atomicOr(a.err, kErrStageOverflow);
+    uint16_t* st = wstage + h * REG;
+    if (lane < pad[h] - cnt[h]) st[rem + lane] = 0;
+    const uint32_t nr = pad[h] > fl[h] ? (pad[h] - fl[h]) / 8 : 0u;
+    for (uint32_t k = lane; k < nr; k += 64) ((uint4*)(spill + h * SSTAGE + fl[h]))[k] = ((const uint4*)st)[k];
+  }
+  if (act) {
+    uint8_t* sec = B.out + B.section[kp] + kSectionFixed;
+    ((uint32_t*)sec)[blk * L + lin] = x;
+    if (lin == 0) {
+      const uint32_t c = half ? cnt[1] : cnt[0];
+      ((uint32_t*)(sec + 4ull * L * B.nb + 8ull * blk))[0] = (n << 16) | c;
+      atomicAdd(a.ptot + 4ull * b + kp, half ? pad[1] : pad[0]);
+    }
+  }
+
+  // the ticket's offset, then block word offsets and words in unit order
+  const uint64_t agg = (uint64_t)pad[0] + pad[1];
+  const uint64_t excl = look_back_gen(a.status, tix, tix - lt, agg, gen, (int)lane);
+  drain();  // the slot areas were written by other lanes of this wave
+  if (act && lin == 0) {
+    const uint8_t* sec = B.out + B.section[kp] + kSectionFixed;
+    ((uint32_t*)(sec + 4ull * L * B.nb + 8ull * blk))[1] = (uint32_t)(excl + (half ? pad[0] : 0u));
+  }
+  {
+    const uint32_t q0 = pad[0] / 8, nq = q0 + pad[1] / 8;
+    const uint4* s4 = (const uint4*)spill;
+    uint4* dst = (uint4*)(B.out + B.data_off + 2ull * excl);
+    constexpr int CBQ = 8;
+    for (uint32_t j0 = 0; j0 < nq; j0 += CBQ * 64) {
+      uint4 q[CBQ];
+#pragma unroll
+      for (int k = 0; k < CBQ; ++k) {
+        const uint32_t j = j0 + k * 64 + lane;
+        const uint32_t jj = j < nq ? j : 0u;
+        q[k] = s4[jj >= q0 ? jj - q0 + SSTAGE / 8 : jj];
+      }
+#pragma unroll
+      for (int k = 0; k < CBQ; ++k) {
+        const uint32_t j = j0 + k * 64 + lane;
+        if (j < nq) dst[j] = q[k];
+      }
+    }
+  }
+  if (lt + 1 == B.nticks && lane == 0) {
+    const uint64_t total_words = excl + agg;
+    const uint64_t size = B.data_off + 2ull * total_words;
+    a.sizes[b] = size;
+    ((uint64_t*)B.out)[6] = total_words;  // data_words at byte 48
+    ((uint64_t*)B.out)[7] = size;         // total_size at byte 56
+  }
+  // the last ticket of the buffer to finish writes each plane's
+  // totalCompressedWords (section header field 3)
+  publish();
+  uint32_t f = 0;
+  if (lane == 0) f = atomicAdd(a.fin + b, 1u);
+  if (wave_u32(f) + 1 == B.nticks && lane < B.eb) {
+    const uint32_t v = atomic_ld32(a.ptot + 4ull * b + lane);
+    atomic_st32(a.ptot + 4ull * b + lane, 0u);
+    ((uint32_t*)(B.out + B.section[lane]))[3] = v;
+    if (lane == 0) atomic_st32(a.fin + b, 0u);
+  }
+}
+
+// L: every buffer's frame format, 64 or 32 lanes (a kernel of its own for
+// each, so neither ticket coder shares a kernel body with the other).
+template <int PB, int NW, int KB, int L>
 __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
   extern __shared__ uint32_t lds[];
@@ -543,9 +737,15 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
       while (wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u) != gen) __builtin_amdgcn_s_sleep(2);
       // (the tables share LDS with the histogram counters; items start in
       // queue order, so no histogram item follows a ticket in a wave)
-      if (B.eb == 1) encode_ticket<PB, 1, (KB / 4 < 4 ? 4 : KB / 4)>(B, b, lt, tix, a, gen, wl, spill, tkey);
-      else if (B.eb == 2) encode_ticket<PB, 2, (KB / 2 < 4 ? 4 : KB / 2)>(B, b, lt, tix, a, gen, wl, spill, tkey);
-      else encode_ticket<PB, 4, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
+      if constexpr (L == 32) {
+        if (B.eb == 1) encode_ticket32<PB, 1, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
+        else if (B.eb == 2) encode_ticket32<PB, 2, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
+        else encode_ticket32<PB, 4, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
+      } else {
+        if (B.eb == 1) encode_ticket<PB, 1, (KB / 4 < 4 ? 4 : KB / 4)>(B, b, lt, tix, a, gen, wl, spill, tkey);
+        else if (B.eb == 2) encode_ticket<PB, 2, (KB / 2 < 4 ? 4 : KB / 2)>(B, b, lt, tix, a, gen, wl, spill, tkey);
+        else encode_ticket<PB, 4, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     // the next item only once this one is done (items start in order)
@@ -574,7 +774,9 @@ hipError_t launch_enc_shape(const BatchEncodeArgs& a, int cu, hipStream_t s, uin
     return hipErrorInvalidConfiguration;
   } else {
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_encode_kernel<PB, NW, KB>, 64 * NW, lds);
+    hipError_t e = a.lanes == 32
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_encode_kernel<PB, NW, KB, 32>, 64 * NW, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_encode_kernel<PB, NW, KB, 64>, 64 * NW, lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     uint64_t grid = (uint64_t)cu * (uint64_t)per_cu;
@@ -587,7 +789,8 @@ hipError_t launch_enc_shape(const BatchEncodeArgs& a, int cu, hipStream_t s, uin
       return hipSuccess;
     }
     if (spill_bytes > a.spill_bytes) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+    if (a.lanes == 32) hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB, 32>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+    else hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB, 64>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
     return hipGetLastError();
   }
 }
@@ -643,7 +846,7 @@ __device__ void build_luts(const BatchBuf& B, uint32_t b, const BatchDecodeArgs&
     // magic, version, element width, mode 0, lanes, precision, planes, order,
     // block bytes, n, data offset and section offsets must match the plan
     bad = fh[0] != kFrameMagic || fb[4] != kFrameVersion || fb[5] != 0 || fb[6] != P || fb[7] != 0 ||
-          fb[8] != kL || fb[9] != PB || fb[10] != P || fb[11] != 0 || fh[3] != kBlock ||
+          fb[8] != B.lanes || fb[9] != PB || fb[10] != P || fb[11] != 0 || fh[3] != 128u * B.lanes ||
           ((const uint64_t*)B.in)[2] != B.n || ((const uint64_t*)B.in)[5] != B.data_off;
     for (uint32_t k = 0; k < P && !bad; ++k) bad = ((const uint64_t*)B.in)[8 + k] != B.section[k];
     const uint64_t data_words = ((const uint64_t*)B.in)[6];
@@ -653,7 +856,8 @@ __device__ void build_luts(const BatchBuf& B, uint32_t b, const BatchDecodeArgs&
   for (uint32_t k = 0; k < P && !bad; ++k) {
     const uint8_t* sec = B.in + B.section[k];
     const uint32_t* h = (const uint32_t*)sec;
-    if ((h[0] >> 16) != kAnsMagic || (h[0] & 0xffffu) != 2u || (h[4] & 0xfu) != (uint32_t)PB || h[1] != B.nb ||
+    if ((h[0] >> 16) != kAnsMagic || (h[0] & 0xffffu) != (B.lanes == 32 ? 1u : 2u) || (h[4] & 0xfu) != (uint32_t)PB ||
+        h[1] != B.nb ||
         h[2] != (uint32_t)B.n) {
       bad = true;
       break;
@@ -904,7 +1108,190 @@ __device__ void decode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, const 
   }
 }
 
-template <int PB, int NW, int KB>
+// Decode ticket of the 32-lane format: units u0 = 2 lt and u0 + 1 of buffer
+// B, lanes 0-31 decoding the first and lanes 32-63 the second (one chain per
+// lane, each unit with its own ring and LUT slot; a unit consumes at most 32
+// words per step, inside the ring schedule's 64).  Each half stores its
+// plane's bytes.
+template <int PB, int KB>
+__device__ void decode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, const BatchDecodeArgs& a, uint32_t* wl,
+                                uint64_t& tkey) {
+  constexpr uint32_t L = 32, BLOCK = 4096;
+  constexpr uint32_t W = 1u << PB;
+  constexpr uint32_t kH = DecRing<KB>::H, kR = DecRing<KB>::R, kMask = DecRing<KB>::MASK;
+  constexpr uint32_t kDecRing = DecRing<KB>::WORDS;
+  constexpr uint32_t H4 = kH / 8;    // uint4 per half ring
+  constexpr int QH = (int)(H4 / 64);  // uint4 per lane per half ring
+  constexpr uint32_t UNIT_CAP = L * lane_max_words(BLOCK / L, PB);
+  const uint32_t lane = threadIdx.x & 63u, half = lane >> 5, lin = lane & 31u;
+  const uint32_t P = B.eb;
+  const uint64_t U = (uint64_t)P * B.nb;
+  const uint64_t u0 = 2ull * lt;
+  const uint64_t data_words = ((const uint64_t*)B.in)[6];
+  uint32_t* s_lut = wl;                        // [2][W]
+  uint16_t* wstage = (uint16_t*)(wl + 2 * W);  // [2][R + 64]
+  // per unit (stream h, wave-uniform): validity, plane, block, symbols,
+  // words, where its words are, ring state
+  bool acth[2], pend[2];
+  uint32_t kph[2], nh[2], wordsh[2], lo[2];
+  uint64_t blkh[2];
+  const uint4* ssrc4[2];
+  uint4 rv[2][QH];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint64_t u = u0 + h;
+    acth[h] = u < U;
+    kph[h] = acth[h] ? (uint32_t)(u % P) : 0u;
+    blkh[h] = acth[h] ? u / P : 0u;
+    const uint8_t* sec = B.in + B.section[kph[h]];
+    uint32_t nn = 0, words = 0, y = 0;
+    if (acth[h]) {
+      const uint2 bw = *(const uint2*)(sec + kSectionFixed + 4ull * L * B.nb + 8ull * blkh[h]);
+      nn = bw.x >> 16;
+      words = bw.x & 0xffffu;
+      y = bw.y;
+      const uint64_t start = blkh[h] * BLOCK;
+      const uint64_t expect = (B.n - start) < BLOCK ? (B.n - start) : BLOCK;
+      if (nn != expect || words > UNIT_CAP || (uint64_t)y + round_up(words, 8) > data_words ||
+          B.data_off + 2ull * ((uint64_t)y + round_up(words, 8)) > B.in_bytes) {
+        if (lane == 0) {
+          atomicOr(a.err, kErrBlockWords);
+          atomicOr(a.bad + b, kErrBlockWords);
+        }
+        acth[h] = false;
+        nn = 0;
+        words = 0;
+      }
+    }
+    nh[h] = wave_u32(nn);
+    wordsh[h] = wave_u32(words);
+    y = wave_u32(y);
+    ssrc4[h] = (const uint4*)(B.in + B.data_off + 2ull * y);
+    uint16_t* reg = wstage + h * kDecRing;
+    lo[h] = 0;
+    pend[h] = false;
+#pragma unroll
+    for (int q = 0; q < QH; ++q) rv[h][q] = make_uint4(0, 0, 0, 0);
+    if (wordsh[h] > 0) {
+      const uint32_t jt = (wordsh[h] - 1) / kH;
+      const uint32_t j0 = jt > 0 ? jt - 1 : 0;
+      lo[h] = j0 * kH;
+      for (uint32_t j = j0; j <= jt; ++j) {
+        const uint32_t left = wordsh[h] - j * kH;
+        const uint32_t n4 = (uint32_t)div_up(left < kH ? left : kH, 8);
+        for (uint32_t i = lane; i < n4; i += 64) {
+          const uint4 q = ssrc4[h][j * H4 + i];
+          ((uint4*)(reg + (j & 1) * kH))[i] = q;
+          if ((j & 1) == 0 && i < 8) ((uint4*)(reg + kR))[i] = q;
+        }
+      }
+    }
+  }
+  // this lane's unit
+  const bool act = half ? acth[1] : acth[0];
+  const uint32_t kp = half ? kph[1] : kph[0];
+  const uint64_t blk = half ? blkh[1] : blkh[0];
+  const uint32_t n = half ? nh[1] : nh[0];
+  uint32_t x[1], ptr[1], taken[1];
+  x[0] = act ? ((const uint32_t*)(B.in + B.section[kp] + kSectionFixed))[blk * L + lin] : kMinState;
+  ptr[0] = half ? wordsh[1] : wordsh[0];
+  taken[0] = 0;
+  const uint32_t* lut[1] = {s_lut + half * W};
+  const uint16_t* stage[1] = {wstage + half * kDecRing};
+  // the units' LUTs, built by the buffer's LUT item
+  const uint64_t key = ((uint64_t)b << 4) | (kph[0] << 2) | kph[1];
+  if (key != tkey) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const uint64_t* g = (const uint64_t*)(a.luts + ((uint64_t)b * 4u + kph[c]) * W);
+      uint64_t* d = (uint64_t*)(s_lut + c * W);
+      for (uint32_t j = lane; j < W / 2; j += 64) d[j] = atomic_ld64(g + j);
+    }
+    tkey = key;
+  }
+  __asm__ volatile("" : "+v"(x[0]));
+  drain();
+
+  auto fetch = [&](int h) {
+    const uint32_t j = lo[h] / kH - 1;
+#pragma unroll
+    for (int q = 0; q < QH; ++q) rv[h][q] = ssrc4[h][j * H4 + 64 * q + lane];
+    pend[h] = true;
+  };
+  auto commit = [&](int h) {
+    const uint32_t j = lo[h] / kH - 1;
+    uint16_t* reg = wstage + h * kDecRing;
+#pragma unroll
+    for (int q = 0; q < QH; ++q) ((uint4*)(reg + (j & 1) * kH))[64 * q + lane] = rv[h][q];
+    if ((j & 1) == 0 && lane < 8) ((uint4*)(reg + kR))[lane] = rv[h][0];
+    lo[h] -= kH;
+    pend[h] = false;
+  };
+  auto refill = [&]() {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t p = __builtin_amdgcn_readlane(ptr[0], (uint32_t)h * 32u);
+      if constexpr (KB == 1) {
+        if (!pend[h] && lo[h] > 0 && p <= lo[h] + kH) fetch(h);
+        if (pend[h] && p < lo[h] + 64u) commit(h);
+      } else {
+        if (!pend[h] && lo[h] > 0) fetch(h);
+        if (pend[h] && p <= lo[h] + kH) {
+          commit(h);
+          if (lo[h] > 0) fetch(h);
+        }
+      }
+    }
+  };
+
+  const bool act1 = acth[1];
+  const uint32_t n_min = act1 ? (nh[0] < nh[1] ? nh[0] : nh[1]) : 0u;
+  const uint32_t n_max = nh[0] > nh[1] ? nh[0] : nh[1];
+  const uint32_t full = n_min / L;
+  const uint32_t all = (uint32_t)div_up(n_max, L);
+  uint8_t* o = B.out + blk * BLOCK * P + kp;
+  auto store = [&](uint32_t s, uint32_t el, bool v) {
+    const uint64_t i = (uint64_t)s * L + lin;
+    if (v) *(__attribute__((address_space(1))) uint8_t*)(o + i * P) = (uint8_t)el;
+  };
+  const bool vall[1] = {true};
+  if constexpr (KB == 1) {
+    for (uint32_t s = all; s > full;) {
+      --s;
+      const bool v[1] = {s * L + lin < n};
+      store(s, dec_steps<L, PB, 1, kMask, uint32_t>(x, ptr, taken, lut, stage, v, half), v[0]);
+      refill();
+    }
+    for (uint32_t s = full; s > 0;) {
+      --s;
+      store(s, dec_steps<L, PB, 1, kMask, uint32_t>(x, ptr, taken, lut, stage, vall, half), true);
+      refill();
+    }
+  } else {
+    uint32_t s = all;
+    while (s > full || s % KB != 0) {
+      --s;
+      const bool v[1] = {s * L + lin < n};
+      store(s, dec_steps<L, PB, 1, kMask, uint32_t>(x, ptr, taken, lut, stage, v, half), v[0]);
+      if (s % KB == 0) refill();
+    }
+    while (s > 0) {
+#pragma unroll
+      for (int j = 0; j < KB; ++j) {
+        --s;
+        store(s, dec_steps<L, PB, 1, kMask, uint32_t>(x, ptr, taken, lut, stage, vall, half), true);
+      }
+      refill();
+    }
+  }
+  const bool bad = act && (taken[0] != (half ? wordsh[1] : wordsh[0]) || x[0] != kMinState);
+  if (__ballot(bad) && lane == 0) {
+    atomicOr(a.err, kErrFinalState);
+    atomicOr(a.bad + b, kErrFinalState);
+  }
+}
+
+template <int PB, int NW, int KB, int L>
 __global__ __launch_bounds__(64 * NW) void batch_decode_kernel(BatchDecodeArgs a) {
   constexpr uint32_t W = 1u << PB;
   constexpr uint32_t WAVE_WORDS = 2 * W + DecRing<KB>::WORDS;  // two LUTs | two rings (u16)
@@ -931,7 +1318,8 @@ __global__ __launch_bounds__(64 * NW) void batch_decode_kernel(BatchDecodeArgs a
         __builtin_amdgcn_s_sleep(2);
       if (!(r & kBatchBad)) {
         const uint32_t lt = it - a.nbufs - B.tick_first;
-        if (B.eb == 1) decode_ticket<PB, false, KB>(B, b, lt, a, wl, tkey);
+        if constexpr (L == 32) decode_ticket32<PB, KB>(B, b, lt, a, wl, tkey);
+        else if (B.eb == 1) decode_ticket<PB, false, KB>(B, b, lt, a, wl, tkey);
         else decode_ticket<PB, true, KB>(B, b, lt, a, wl, tkey);
       }
     }
@@ -961,14 +1349,17 @@ hipError_t launch_dec_shape(const BatchDecodeArgs& a, int cu, hipStream_t s) {
     return hipErrorInvalidConfiguration;
   } else {
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_decode_kernel<PB, NW, KB>, 64 * NW, lds);
+    hipError_t e = a.lanes == 32
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_decode_kernel<PB, NW, KB, 32>, 64 * NW, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_decode_kernel<PB, NW, KB, 64>, 64 * NW, lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     uint64_t grid = (uint64_t)cu * (uint64_t)per_cu;
     const uint64_t want = div_up(a.nitems, (uint64_t)NW * a.run);
     if (want < grid) grid = want;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL((batch_decode_kernel<PB, NW, KB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+    if (a.lanes == 32) hipLaunchKernelGGL((batch_decode_kernel<PB, NW, KB, 32>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+    else hipLaunchKernelGGL((batch_decode_kernel<PB, NW, KB, 64>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
     return hipGetLastError();
   }
 }

@@ -1070,6 +1070,7 @@ struct mans_batch {
   uint64_t total_bytes = 0;
   uint32_t width_mask = 0;
   BatchGrain grain = {kBatchHistChunkMax, 4, 4};  // work-item grain of the plan
+  uint32_t lanes = 64;       // interleave width of every frame of the batch (plan-time)
 };
 
 namespace {
@@ -1106,22 +1107,25 @@ BatchGrain batch_grain(uint64_t bytes, uint64_t units, int cu) {
 }
 
 // The grain of a batch of these buffers (elem_bytes 0 counts as 1).
-BatchGrain grain_of(const mans_buffer* h_bufs, int32_t count, int cu) {
+BatchGrain grain_of(const mans_buffer* h_bufs, int32_t count, int cu, int lanes) {
   uint64_t bytes = 0, units = 0;
   for (int32_t i = 0; i < count; ++i) {
     const uint64_t eb = (uint64_t)std::max<int32_t>(h_bufs[i].elem_bytes, 1);
     bytes += h_bufs[i].n_elems * eb;
     units += eb * div_up(h_bufs[i].n_elems, (uint64_t)format_for_lanes(64).block_bytes);
   }
-  return batch_grain(bytes, units, cu);
+  BatchGrain g = batch_grain(bytes, units, cu);
+  if (lanes == 32) g.rounds = 1;  // 32-lane tickets are one pair of units
+  return g;
 }
 
-// Lays out every buffer of a batch as the 64-lane byte-plane frame of
-// mans_frame_encode and counts its histogram items (g.hist_chunk bytes) and
-// tickets (decode: two units; encode: g.rounds rounds of enc_chains units).
+// Lays out every buffer of a batch as the byte-plane frame of
+// mans_frame_encode in p->lanes' format and counts its histogram items
+// (g.hist_chunk bytes) and tickets (decode and 32-lane encode: two units;
+// 64-lane encode: g.rounds rounds of enc_chains units).
 mans_status plan_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, bool decode,
                        const BatchGrain& g, std::vector<BatchBuf>& bufs, uint64_t& nhist, uint64_t& nticks) {
-  const Format f = format_for_lanes(64);
+  const Format f = format_for_lanes(p->lanes);
   bufs.assign((size_t)count, BatchBuf{});
   nhist = nticks = 0;
   for (int32_t i = 0; i < count; ++i) {
@@ -1141,7 +1145,7 @@ mans_status plan_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t 
     CodecDesc d;
     memset(&d, 0, sizeof(d));
     d.prob_bits = p->prob_bits;
-    d.lanes = 64;
+    d.lanes = (int)f.lanes;
     d.num_planes = (int)eb;
     d.order = 0;
     const uint64_t nb = div_up(m.n_elems, f.block_bytes);
@@ -1165,7 +1169,8 @@ mans_status plan_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t 
     B.tick_first = (uint32_t)nticks;
     // decode tickets: two units; encode tickets: g.rounds rounds of four
     // chains (two for uint32), as enc_ticket_units in batch_kernels.hpp
-    B.nticks = (uint32_t)div_up(eb * nb, decode ? 2u : g.rounds * (eb == 4 ? 2u : 4u));
+    B.nticks = (uint32_t)div_up(eb * nb, decode || f.lanes == 32 ? 2u : g.rounds * (eb == 4 ? 2u : 4u));
+    B.lanes = f.lanes;
     B.in_bytes = decode ? m.in_bytes : 0;
     frame_header(B.frame_hdr, m.n_elems, eb, MANS_MODE_PLANES, f, p->prob_bits, d, 0, 0);
     put64(B.frame_hdr, 56, off);  // total size of an empty frame
@@ -1229,7 +1234,7 @@ mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, ui
 mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h_bufs, int32_t count, hipStream_t s) {
   std::vector<BatchBuf> bufs;
   uint64_t nhist = 0, nticks = 0;
-  const BatchGrain grain = grain_of(h_bufs, count, b->ws->cu);
+  const BatchGrain grain = grain_of(h_bufs, count, b->ws->cu, p->lanes);
   mans_status st = plan_batch(p, h_bufs, count, b->decode, grain, bufs, nhist, nticks);
   if (st != MANS_OK) return st;
   const uint64_t nitems = (b->decode ? (uint64_t)count : nhist) + nticks;
@@ -1284,7 +1289,7 @@ mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h
     b->total_bytes += h_bufs[i].n_elems * (uint64_t)h_bufs[i].elem_bytes;
     b->width_mask |= (uint32_t)h_bufs[i].elem_bytes;
     mans_params q = *p;
-    q.lanes = 64;
+    q.lanes = p->lanes;
     q.elem_bytes = h_bufs[i].elem_bytes;
     q.mode = MANS_MODE_PLANES;
     const uint64_t fm = mans_frame_max_size(h_bufs[i].n_elems, &q);
@@ -1293,6 +1298,7 @@ mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h
   }
   b->nbufs = (uint32_t)count;
   b->grain = grain;
+  b->lanes = (uint32_t)p->lanes;
   b->nhist = b->decode ? 0u : (uint32_t)nhist;
   b->nticks = (uint32_t)nticks;
   b->nitems = (uint32_t)nitems;
@@ -1321,6 +1327,7 @@ BatchEncodeArgs batch_args(const mans_batch* b, uint64_t* d_sizes) {
   a.spill_bytes = b->spill_cap;
   a.hist_chunk = b->grain.hist_chunk;
   a.rounds = b->grain.rounds;
+  a.lanes = b->lanes;
   return a;
 }
 
@@ -1342,8 +1349,8 @@ uint32_t log2_bucket(uint64_t v) {
 
 std::string tune_key(const mans_batch* b) {
   char buf[96];
-  snprintf(buf, sizeof(buf), "%s-p%d-n%u-b%u-w%u-cu%d", b->decode ? "dec" : "enc", b->pb, log2_bucket(b->nbufs),
-           log2_bucket(b->total_bytes), b->width_mask, b->ws->cu);
+  snprintf(buf, sizeof(buf), "%s-p%d-l%u-n%u-b%u-w%u-cu%d", b->decode ? "dec" : "enc", b->pb, b->lanes,
+           log2_bucket(b->nbufs), log2_bucket(b->total_bytes), b->width_mask, b->ws->cu);
   return buf;
 }
 
@@ -1452,7 +1459,7 @@ mans_status read_frame_dims(const mans_params* p, std::vector<mans_buffer>& v, m
     memcpy(&total, hb + 56, 8);
     const uint32_t eb = hb[6], mode = hb[7], lanes = hb[8], pb = hb[9], P = hb[10], order = hb[11];
     mans_buffer& m = v[idx[k]];
-    if (magic != kFrameMagic || hb[4] != kFrameVersion || mode != MANS_MODE_PLANES || lanes != 64 ||
+    if (magic != kFrameMagic || hb[4] != kFrameVersion || mode != MANS_MODE_PLANES || (int)lanes != p->lanes ||
         (int)pb != p->prob_bits || P != eb || order != 0 || (eb != 1 && eb != 2 && eb != 4) || total > m.in_bytes)
       return MANS_ERR_CORRUPT;
     m.n_elems = ne;
@@ -1467,7 +1474,8 @@ mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buff
   *out = nullptr;
   mans_status st = check_params(p);
   if (st != MANS_OK) return st;
-  if (!ws || count < 0 || (count > 0 && !h_bufs_in) || p->lanes != 64) return MANS_ERR_INVALID_ARG;
+  if (!ws || count < 0 || (count > 0 && !h_bufs_in) || (p->lanes != 64 && p->lanes != 32))
+    return MANS_ERR_INVALID_ARG;
   std::vector<mans_buffer> given(h_bufs_in, h_bufs_in + count);
   if (decode) {
     st = read_frame_dims(p, given, ws);
@@ -1476,7 +1484,7 @@ mans_status batch_create(mans_batch** out, const mans_params* p, const mans_buff
   const mans_buffer* h_bufs = given.data();
   std::vector<BatchBuf> bufs;
   uint64_t nhist = 0, nticks = 0;
-  st = plan_batch(p, h_bufs, count, decode, grain_of(h_bufs, count, ws->cu), bufs, nhist, nticks);
+  st = plan_batch(p, h_bufs, count, decode, grain_of(h_bufs, count, ws->cu, p->lanes), bufs, nhist, nticks);
   if (st != MANS_OK) return st;
   DeviceGuard g(ws->device);
   mans_batch* b = new mans_batch();
@@ -1592,6 +1600,7 @@ mans_status mans_batch_decode_run(mans_batch* b, void* stream) {
   a.err = ws->counters + 3;
   a.bad = b->bad;
   a.run = b->grain.run;
+  a.lanes = b->lanes;
   // per-buffer flags start at zero in every run (a kernel, so the run can
   // be captured)
   HIP_TRY(launch_fill_u32(b->bad, b->nbufs, 0, s));
@@ -1702,12 +1711,13 @@ mans_status mans_batch_decode_flags(const mans_batch* b, uint32_t* h_flags, int3
 
 mans_status mans_batch_get_shape(const mans_batch* b, mans_batch_shape* out) {
   if (!b || !out) return MANS_ERR_INVALID_ARG;
-  *out = mans_batch_shape{64 * b->shape.nw, b->shape.kb, 64, 0};
+  *out = mans_batch_shape{64 * b->shape.nw, b->shape.kb, (int32_t)b->lanes, 0};
   return MANS_OK;
 }
 
 mans_status mans_batch_set_shape(mans_batch* b, const mans_batch_shape* sh) {
-  if (!b || !sh || sh->lanes != 64 || sh->reserved != 0 || sh->wg_threads % 64 != 0) return MANS_ERR_INVALID_ARG;
+  if (!b || !sh || sh->lanes != (int32_t)b->lanes || sh->reserved != 0 || sh->wg_threads % 64 != 0)
+    return MANS_ERR_INVALID_ARG;
   const BatchShape c{sh->wg_threads / 64, sh->syms_per_lane};
   if (!batch_shape_ok(c, b->pb, b->decode)) return MANS_ERR_INVALID_ARG;
   DeviceGuard g(b->ws->device);
@@ -1762,7 +1772,7 @@ mans_status mans_batch_tune(mans_batch* b, void* stream, int32_t reps, mans_tune
         pick = c;
       }
     }
-    if (i < max_entries) h_entries[i] = mans_tune_entry{mans_batch_shape{64 * c.nw, c.kb, 64, 0}, ms};
+    if (i < max_entries) h_entries[i] = mans_tune_entry{mans_batch_shape{64 * c.nw, c.kb, (int32_t)b->lanes, 0}, ms};
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

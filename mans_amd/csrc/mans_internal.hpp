@@ -216,7 +216,7 @@ struct BatchBuf {
   uint32_t nhist;                // histogram items (at least 1)
   uint32_t tick_first;           // first encode ticket (ticket = 2 units)
   uint32_t nticks;               // encode (decode) tickets
-  uint32_t pad_;
+  uint32_t lanes;                // 64 (native frames) or 32 (reference-format streams)
   uint64_t in_bytes;             // decode: bytes readable at `in` (the frame)
   uint32_t frame_hdr[32];        // frame header (data_words / total_size set by the kernel)
 };
@@ -244,7 +244,8 @@ struct BatchEncodeArgs {
   uint16_t* spill;               // per-wave spill areas
   uint64_t spill_bytes;
   uint32_t hist_chunk;           // bytes per histogram item (BatchGrain)
-  uint32_t rounds;               // rounds per encode ticket (BatchGrain)
+  uint32_t rounds;               // rounds per encode ticket (BatchGrain; 64 lanes)
+  uint32_t lanes;                // every buffer's frame: 64 or 32 lanes
 };
 // Batched decode: BatchBuf.in is the frame, .out the elements; items
 // [0, nbufs) build buffer b's LUTs, then come the decode tickets.
@@ -259,6 +260,7 @@ struct BatchDecodeArgs {
   uint32_t* err;
   uint32_t* bad;                 // [nbufs] the error flags of each buffer (zeroed before the launch)
   uint32_t run;                  // items a wavefront takes per queue grab (BatchGrain)
+  uint32_t lanes;                // every buffer's frame: 64 or 32 lanes
 };
 constexpr uint32_t kBatchBad = 1u << 31;
 constexpr uint32_t kBatchGenMask = 0x3fffu;     // 14-bit generations

@@ -86,6 +86,54 @@ def test_batch_frames_match_single_frames(ws, p):
     b.close()
 
 
+@pytest.mark.parametrize("p", [9, 10])
+def test_batch_32_lane_frames_match_single_frames(ws, p):
+    """The reference-format (32-lane, 4096-symbol blocks) batched form: every
+    frame equals the single-buffer 32-lane frame encode (and the oracle's),
+    and the batched decode restores every buffer."""
+    rng = np.random.default_rng(70 + p)
+    sizes = EDGE_SIZES + [4095, 4096, 4097, 12_289] + [int(v) for v in np.exp(rng.uniform(np.log(1 << 10),
+                                                                                          np.log(1 << 21), 30))]
+    xs, ts = make_batch(70 + p, sizes)
+    prm = mans_amd.params(p, 32)
+    print(f"batch 32 lanes p={p} buffers={len(xs)}", flush=True)
+    b = mans_amd.EncodeBatch(ts, prm, ws)
+    got_sizes = b.run().cpu().tolist()
+    ws.check()
+    assert b.shape[2] == 32
+    for i, (x, t) in enumerate(zip(xs, ts)):
+        q = mans_amd.params(p, 32, x.itemsize, mans_amd.MODE_PLANES)
+        ref, n = mans_amd.frame_encode(t, q, ws)
+        assert got_sizes[i] == n, (i, x.size, x.itemsize)
+        assert torch.equal(b.outs[i][:n], ref[:n]), (i, x.size, x.itemsize)
+        if i % 5 == 0:
+            o = oracle.frame_encode(x, oracle.MODE_PLANES, p, oracle.FORMAT_REF)
+            assert o.size == n and (o == b.outs[i][:n].cpu().numpy()).all(), (i, x.size)
+    dec = mans_amd.DecodeBatch(b.outs, got_sizes, ts, prm, ws)
+    outs = dec.run()
+    torch.cuda.synchronize()
+    ws.check()
+    for i, x in enumerate(xs):
+        assert (outs[i].cpu().numpy().view(x.dtype) == x).all(), (i, x.size, x.itemsize)
+    # a 64-lane frame handed to a 32-lane decode batch is rejected, alone
+    if len(xs) > 3 and xs[3].size > 0:
+        q64 = mans_amd.params(p, 64, xs[3].itemsize, mans_amd.MODE_PLANES)
+        f64, n64 = mans_amd.frame_encode(ts[3], q64, ws)
+        frames = list(b.outs)
+        fsz = list(got_sizes)
+        frames[3], fsz[3] = f64, int(n64)
+        bad = mans_amd.DecodeBatch(frames, fsz, ts, prm, ws)
+        bad.run()
+        torch.cuda.synchronize()
+        with pytest.raises(mans_amd.MansError):
+            ws.check()
+        flags = bad.flags()
+        assert flags[3] != 0 and all(f == 0 for k, f in enumerate(flags) if k != 3)
+        bad.close()
+    dec.close()
+    b.close()
+
+
 def test_batch_graph_replay(ws):
     rng = np.random.default_rng(7)
     sizes = [int(v) for v in np.exp(rng.uniform(np.log(4096), np.log(1 << 20), 64))]
@@ -354,17 +402,20 @@ def test_batch_every_shape_is_bit_identical(ws, p):
     enc.close()
 
 
-GRAINS = [(1 << 16, 1, 1), (1 << 17, 2, 4), (1 << 18, 4, 1), (1 << 19, 4, 4), (1 << 19, 1, 4), (1 << 16, 2, 1)]
+GRAINS = [(1 << 16, 1, 1, 64), (1 << 17, 2, 4, 64), (1 << 18, 4, 1, 64), (1 << 19, 4, 4, 64), (1 << 19, 1, 4, 64),
+          (1 << 16, 2, 1, 64), (1 << 16, 1, 4, 32), (1 << 19, 4, 1, 32)]
 
 
-@pytest.mark.parametrize("grain", GRAINS, ids=lambda g: f"chunk{g[0] >> 10}k-r{g[1]}-run{g[2]}")
+@pytest.mark.parametrize("grain", GRAINS, ids=lambda g: f"chunk{g[0] >> 10}k-r{g[1]}-run{g[2]}-l{g[3]}")
 def test_batch_every_grain_is_bit_identical(ws, grain):
     """Every work-item grain (histogram chunk, encode ticket rounds, decode
     run) gives the frames of the single-buffer encode, and decodes back."""
     rng = np.random.default_rng(50)
     sizes = EDGE_SIZES + [int(v) for v in np.exp(rng.uniform(np.log(1 << 10), np.log(3 << 20), 24))]
     xs, ts = make_batch(50, sizes)
-    prm = mans_amd.params(10, 64)
+    lanes = grain[3]
+    grain = grain[:3]
+    prm = mans_amd.params(10, lanes)
     print(f"grain {grain} buffers={len(xs)}", flush=True)
     mans_amd.set_batch_grain(*grain)
     try:
@@ -372,7 +423,7 @@ def test_batch_every_grain_is_bit_identical(ws, grain):
         got_sizes = enc.run().cpu().tolist()
         ws.check()
         for i, (x, t) in enumerate(zip(xs, ts)):
-            q = mans_amd.params(10, 64, x.itemsize, mans_amd.MODE_PLANES)
+            q = mans_amd.params(10, lanes, x.itemsize, mans_amd.MODE_PLANES)
             ref, n = mans_amd.frame_encode(t, q, ws)
             assert got_sizes[i] == n, (grain, i, x.size, x.itemsize)
             assert torch.equal(enc.outs[i][:n], ref[:n]), (grain, i, x.size, x.itemsize)
