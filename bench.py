@@ -288,7 +288,9 @@ def batched_bench(args):
     are compared byte for byte.  Before the batched launches are timed the
     autotuner (mans_batch_tune) picks their shape from workgroup size
     {256, 512, 1024} x symbols per lane (encode {4, 8, 16}, decode
-    {1, 4, 8, 16}) at 64 lanes; its table is reported."""
+    {1, 4, 8, 16}); the same is done for a 32-lane (reference-format) plan of
+    the same buffers (width32_leg), and the interleave width with the faster
+    tuned encode + decode is reported as picked, with both tables."""
     import torch
 
     import mans_amd
@@ -413,7 +415,14 @@ def batched_bench(args):
     enc_picked, dec_picked = b.shape, d.shape
     d.close()
     b.close()
+    del outs_plain, outs_batch, dec_plain, outs
+    torch.cuda.empty_cache()
     ws.close()
+    w32 = width32_leg(xs, args, s)
+    # the interleave width the tuner picks: the faster plan (encode + decode,
+    # each at its tuned shape); the frame format follows the width
+    t64 = graph_ms + graph_dec_ms
+    t32 = w32["graph_ms"] + w32["decode_graph_ms"]
     return {
         "workload": f"{len(xs)} buffers, sizes log-uniform in [4 KiB, 16 MiB], uint8/16/32 at 20/60/20 %, "
                     f"Laplace or Zipf values 50/50, seeds 1000+i; byte-plane frames, 64 lanes, p={args.prob_bits}",
@@ -438,15 +447,110 @@ def batched_bench(args):
         "decode_speedup_vs_plain_kernel_time": round(plain_dec_kernel_ms / graph_dec_ms, 2),
         "decode_round_trip_bit_exact": dec_ok,
         "autotune": {
-            "candidates": "wg_threads {256, 512, 1024} x syms_per_lane (encode {4, 8, 16}, decode {1, 4, 8, 16}), "
-                          "64 lanes; ms < 0: does not fit the 160 KiB LDS",
+            "candidates": "interleave width {64, 32} (the plan's frame format) x wg_threads {256, 512, 1024} x "
+                          "syms_per_lane (encode {4, 8, 16}, decode {1, 4, 8, 16}); ms < 0: does not fit the "
+                          "160 KiB LDS",
+            "picked_width": 64 if t64 <= t32 else 32,
+            "width_ms": {"64": round(t64, 3), "32": round(t32, 3)},
             "encode": {"default": list(enc_default), "picked": list(enc_picked),
-                       "ms": [[r["wg_threads"], r["syms_per_lane"], round(r["ms"], 3)] for r in enc_tune]},
+                       "ms": [[r["wg_threads"], r["syms_per_lane"], 64, round(r["ms"], 3)] for r in enc_tune] +
+                             w32["encode_ms"]},
             "decode": {"default": list(dec_default), "picked": list(dec_picked),
-                       "ms": [[r["wg_threads"], r["syms_per_lane"], round(r["ms"], 3)] for r in dec_tune]},
+                       "ms": [[r["wg_threads"], r["syms_per_lane"], 64, round(r["ms"], 3)] for r in dec_tune] +
+                             w32["decode_ms"]},
         },
+        "width32": {k: v for k, v in w32.items() if k not in ("encode_ms", "decode_ms")},
         "gen_s": round(gen_s, 2),
     }
+
+
+def width32_leg(xs, args, s):
+    """The same buffers as 32-lane (reference-format) frames: the batched
+    launch tuned and timed (eager and from a graph), its frames compared with
+    one plain 32-lane frame encode per buffer, and the batched decode tuned,
+    timed from a graph and checked against the input."""
+    import torch
+
+    import mans_amd
+
+    p32 = mans_amd.params(args.prob_bits, 32)
+    stream = torch.cuda.current_stream()
+    # a workspace of its own (the 64-lane leg's was used under graph capture,
+    # which freezes its sizes)
+    ws = mans_amd.Workspace(0)
+    for eb in (1, 2, 4):
+        ws.reserve(16 << 20, mans_amd.params(args.prob_bits, 32, eb, mans_amd.MODE_PLANES))
+    caps = [mans_amd.frame_max_size(x.numel(), mans_amd.params(args.prob_bits, 32, x.element_size(),
+                                                                 mans_amd.MODE_PLANES)) for x in xs]
+    plain = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    dsz = torch.zeros(len(xs), dtype=torch.int64, device="cuda")
+    for i, x in enumerate(xs):
+        mans_amd.frame_encode(x, p32, ws, out=plain[i], sync_size=False, d_size=dsz[i:i + 1])
+    torch.cuda.synchronize()
+    ws.check()
+    b = mans_amd.EncodeBatch(xs, p32, ws)
+    tune_e = b.tune(reps=args.batch_steps)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.batch_steps):
+        b.run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    batch_ms = e0.elapsed_time(e1) / args.batch_steps
+    g = torch.cuda.CUDAGraph()
+    s.wait_stream(stream)
+    with torch.cuda.graph(g, stream=s):
+        b.run(stream=s)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.batch_steps):
+        g.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    graph_ms = e0.elapsed_time(e1) / args.batch_steps
+    ws.check()
+    ok = bool(torch.equal(b.sizes, dsz))
+    sz = dsz.cpu().tolist()
+    if ok:
+        ok = all(torch.equal(b.outs[i][:sz[i]], plain[i][:sz[i]]) for i in range(len(xs)))
+    d = mans_amd.DecodeBatch(b.outs, sz, xs, p32, ws)
+    tune_d = d.tune(reps=args.batch_steps)
+    for o in d.outs:
+        o.zero_()
+    gd = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(gd, stream=s):
+        d.run(stream=s)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.batch_steps):
+        gd.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    graph_dec_ms = e0.elapsed_time(e1) / args.batch_steps
+    ws.check()
+    dec_ok = all(torch.equal(d.outs[i][:xs[i].numel()], xs[i]) for i in range(len(xs)))
+    total = sum(x.numel() * x.element_size() for x in xs)
+    out = {
+        "compressed_bytes": int(dsz.sum().item()),
+        "batched_ms": round(batch_ms, 3),
+        "batched_GBps": round(total / (batch_ms * 1e-3) / 1e9, 2),
+        "graph_ms": round(graph_ms, 3),
+        "graph_GBps": round(total / (graph_ms * 1e-3) / 1e9, 2),
+        "bit_exact_vs_plain": ok,
+        "decode_graph_ms": round(graph_dec_ms, 3),
+        "decode_graph_GBps": round(total / (graph_dec_ms * 1e-3) / 1e9, 2),
+        "decode_round_trip_bit_exact": dec_ok,
+        "encode_picked": list(b.shape),
+        "decode_picked": list(d.shape),
+        "encode_ms": [[r["wg_threads"], r["syms_per_lane"], 32, round(r["ms"], 3)] for r in tune_e],
+        "decode_ms": [[r["wg_threads"], r["syms_per_lane"], 32, round(r["ms"], 3)] for r in tune_d],
+    }
+    d.close()
+    b.close()
+    ws.close()
+    del plain
+    torch.cuda.empty_cache()
+    return out
 
 
 def served_bench(args):

@@ -213,7 +213,7 @@ typedef struct mans_buffer {
   const void* d_in;      /* encode: elements; decode: frame (16-byte aligned) */
   uint64_t in_bytes;     /* decode: bytes readable at d_in (the frame); encode: ignored */
   void* d_out;           /* encode: frame; decode: elements (16-byte aligned) */
-  uint64_t out_capacity; /* encode: >= mans_frame_max_size(n_elems, {p, 64, elem_bytes});
+  uint64_t out_capacity; /* encode: >= mans_frame_max_size(n_elems, {p, lanes, elem_bytes});
                             decode: >= n_elems * elem_bytes */
   uint64_t n_elems;
   int32_t elem_bytes;    /* 1, 2 or 4 */
@@ -222,7 +222,10 @@ typedef struct mans_buffer {
 
 typedef struct mans_batch mans_batch;
 
-/* p->lanes must be 64; p->elem_bytes and p->mode are ignored. */
+/* p->lanes is the interleave width of every frame of the batch: 64 (the
+ * native format) or 32 (reference-format streams, byte-identical to
+ * mans_frame_encode with lanes 32); it is fixed for the batch's plan.  The
+ * request server takes 64.  p->elem_bytes and p->mode are ignored. */
 mans_status mans_batch_encode_create(mans_batch** b, const mans_params* p, const mans_buffer* h_bufs,
                                      int32_t count, mans_workspace* ws);
 /* d_sizes: device array of count uint64 frame sizes, written by the launch. */
@@ -276,7 +279,8 @@ typedef struct mans_batch_shape {
   int32_t syms_per_lane; /* symbols each lane codes per batch.  Encode: 4, 8 or 16
                             elements loaded ahead; decode: 1, 4, 8 or 16 steps
                             between two upkeeps of the word ring */
-  int32_t lanes;         /* interleave width: 64 (the batched frames' format) */
+  int32_t lanes;         /* interleave width: the batch's frame format, 64 (native) or 32
+                            (reference-format streams), fixed when the batch is planned */
   int32_t reserved;      /* 0 */
 } mans_batch_shape;
 
