@@ -4,22 +4,23 @@
 // instantiating the kernels of every launch shape (BatchShape) for it.
 //
 // Every buffer's frame is byte-identical to what mans_frame_encode writes for
-// it alone (MANS_MODE_PLANES, 64-lane format): the same histogram and
-// normalisation (GpuANSStatistics.h:23-151, :186-406), the same rANS
-// arithmetic and container (GpuANSEncode.h:24-164, :180-284).  What changes
-// is the schedule.  One queue of work items (an atomic counter) holds first
-// the histogram chunks of every buffer, then the encode tickets of every
-// buffer in order; every wavefront takes items one at a time:
+// it alone (MANS_MODE_PLANES, in the batch's 64- or 32-lane format): the same
+// histogram and normalisation (GpuANSStatistics.h:23-151, :186-406), the
+// same rANS arithmetic and container (GpuANSEncode.h:24-164, :180-284).  What
+// changes is the schedule.  One queue of work items (an atomic counter) holds
+// the histogram chunks of every buffer and its encode tickets, the tickets of
+// a buffer queued a few hundred items after its last chunk; every wavefront
+// takes items one at a time:
 //  * a histogram item counts one chunk of one buffer (all its byte planes)
 //    into LDS and adds the counts to the buffer's accumulators; the wavefront
 //    that finishes a buffer's last chunk normalises its planes (one wavefront
 //    per buffer), writes the frame and plane headers and publishes the
 //    encoder tables, then marks the buffer ready;
-//  * an encode ticket covers two consecutive units of one buffer (both byte
-//    planes of a uint16 block, two planes of a uint32 block, or two blocks of
-//    a uint8 buffer).  It waits until its buffer is ready, codes the two
-//    chains with the single-buffer coder's step and LDS ring, and resolves
-//    its output offset by decoupled look-back within its buffer.
+//  * a 64-lane encode ticket is 1, 2 or 4 rounds of four chains per lane (two
+//    for uint32) with the single-buffer coder's step; a 32-lane ticket is two
+//    units, one per half-wave.  A ticket waits until its buffer is ready,
+//    codes its units and resolves its output offset by decoupled look-back
+//    within its buffer.
 // A ticket only waits on items taken before it (its buffer's histogram
 // chunks and earlier tickets of its buffer), all held by running
 // wavefronts, so the launch always drains.  Data published inside the launch
