@@ -113,13 +113,16 @@ __device__ uint64_t look_back_gen(uint64_t* status, uint64_t idx, uint64_t first
 // flight while the previous eight are counted.
 static_assert(kBatchHistChunkMax / 16u < 65536u, "16-bit pair counters");
 static_assert(4u * 128u * (16u / 4u) <= kTabWords, "counters fit the wave's table area");
-constexpr int kHistLoads = 8;
+constexpr int kHistLoads = 4;
 
-__device__ void hist_chunk(const BatchBuf& B, uint32_t b, uint64_t c0, uint64_t c1, uint32_t* cnt,
-                           uint32_t* ghist) {
+// (EB: the buffer's element bytes = planes, a template argument so the
+// plane of each byte and the replica stride are constants)
+template <uint32_t EB>
+__device__ __forceinline__ void hist_chunk(const BatchBuf& B, uint32_t b, uint64_t c0, uint64_t c1, uint32_t* cnt,
+                                           uint32_t* ghist) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t P = B.eb;
-  const uint32_t rep = 16u / P;
+  constexpr uint32_t P = EB;
+  constexpr uint32_t rep = 16u / P;
   uint32_t* c = cnt + lane % rep;
   const uint4* p4 = (const uint4*)(B.in + c0);
   const uint64_t n16 = (c1 - c0) / 16u;
@@ -727,7 +730,11 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
       const uint64_t bytes = B.n * B.eb;
       const uint64_t c0 = (uint64_t)li * a.hist_chunk;
       const uint64_t c1 = c0 + a.hist_chunk < bytes ? c0 + a.hist_chunk : bytes;
-      if (c0 < c1) hist_chunk(B, b, c0, c1, wl, a.hist);
+      if (c0 < c1) {
+        if (B.eb == 1) hist_chunk<1>(B, b, c0, c1, wl, a.hist);
+        else if (B.eb == 2) hist_chunk<2>(B, b, c0, c1, wl, a.hist);
+        else hist_chunk<4>(B, b, c0, c1, wl, a.hist);
+      }
       publish();
       uint32_t d = 0;
       if (lane == 0) d = atomicAdd(a.done + b, 1u);
