@@ -82,22 +82,28 @@ __device__ __forceinline__ void drain() { __builtin_amdgcn_fence(__ATOMIC_RELEAS
 // this data and costs a writeback per item.)
 __device__ __forceinline__ void publish() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// Look-back within one buffer's tickets [first, ...): like look_back, with
-// each status word tagged by the launch generation (an entry of another
-// generation reads as not yet published).
-__device__ uint64_t look_back_gen(uint64_t* status, uint64_t idx, uint64_t first, uint64_t agg, uint32_t gen,
-                                  int lane) {
+// Look-back within one buffer's tickets [first, ...), each status word
+// tagged by the launch generation (an entry of another generation reads as
+// not yet published).  A ticket publishes its aggregate as soon as it is
+// coded (publish_agg_gen: the buffer's first ticket publishes it as its
+// inclusive prefix) and resolves its offset when its predecessors have
+// published theirs (resolve_gen; with TRY, ~0 if one of them has not yet).
+__device__ __forceinline__ void publish_agg_gen(uint64_t* status, uint64_t idx, uint64_t first, uint64_t agg,
+                                                uint32_t gen, int lane) {
   const uint64_t G = (uint64_t)gen << kGenShift;
-  if (idx == first) {
-    if (lane == 0) atomic_st64(status + idx, kFlagPrefix | G | agg);
-    return 0;
-  }
-  if (lane == 0) atomic_st64(status + idx, kFlagAgg | G | agg);
-  const uint64_t excl = look_back_window<8>(status, (int64_t)idx - 1, (int64_t)first, kFlagPrefix | G, lane,
+  if (lane == 0) atomic_st64(status + idx, (idx == first ? kFlagPrefix : kFlagAgg) | G | agg);
+}
+template <bool TRY>
+__device__ uint64_t resolve_gen(uint64_t* status, uint64_t idx, uint64_t first, uint64_t agg, uint32_t gen,
+                                int lane) {
+  if (idx == first) return 0;
+  const uint64_t G = (uint64_t)gen << kGenShift;
+  const uint64_t excl = look_back_window<8, TRY>(status, (int64_t)idx - 1, (int64_t)first, kFlagPrefix | G, lane,
                                             [gen](uint64_t v, uint32_t& f, uint64_t& val) {
                                               f = ((v >> kGenShift) & kBatchGenMask) == gen ? (uint32_t)(v >> 62) : 0u;
                                               val = v & kValue48;
                                             });
+  if (TRY && excl == ~0ull) return excl;
   if (lane == 0) atomic_st64(status + idx, kFlagPrefix | G | (excl + agg));
   return excl;
 }
@@ -245,6 +251,14 @@ __device__ void finish_buffer(const BatchBuf& B, uint32_t b, const BatchEncodeAr
 // interleaved chains per round.  One queue item, one table load and one
 // look-back serve the whole ticket.
 constexpr uint32_t kMaxRounds = 4;
+constexpr uint32_t kMaxTicketUnits = 16;  // kMaxRounds x four chains
+// A wavefront's spill slot: an area of chain_stage_words per unit of a
+// ticket, then the units' padded word counts (kMaxTicketUnits uint32, in a
+// 128-byte tail).
+template <int PB>
+__host__ __device__ constexpr uint64_t batch_slot_words() {
+  return (uint64_t)kMaxTicketUnits * chain_stage_words<PB>() + 64u;
+}
 __host__ __device__ constexpr uint32_t enc_chains(uint32_t eb) { return eb == 4 ? 2u : 4u; }
 __host__ __device__ constexpr uint32_t enc_ticket_units(uint32_t eb, uint32_t rounds) {
   return rounds * enc_chains(eb);
@@ -257,12 +271,13 @@ __host__ __device__ constexpr uint32_t enc_ticket_units(uint32_t eb, uint32_t ro
 // uint32 tickets of four rounds code planes 0-1 of both blocks, then planes
 // 2-3, so the tables change once; of two rounds, planes 0-1 then 2-3 of one
 // block; of one round, two planes of a block.
-// The ticket's offset comes from a look-back within its buffer once all its
-// rounds are coded; then block word offsets and words go to their place, in
-// unit order.  tkey: which tables the wave's LDS holds ((buffer << 4) |
-// first plane, ~0 for none).  KBB: elements loaded ahead per block.
+// Once all its rounds are coded the ticket publishes its aggregate and
+// leaves its units' padded sizes at the tail of its slot; its offset is
+// resolved by the caller (resolve_gen, place_ticket).  Returns the aggregate.  tkey: which
+// tables the wave's LDS holds ((buffer << 4) | first plane, ~0 for none).
+// KBB: elements loaded ahead per block.
 template <int PB, int EB, int KBB>
-__device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
+__device__ uint64_t encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
                               uint32_t gen, uint32_t* wl, uint16_t* spill, uint64_t& tkey) {
   constexpr int CH = (int)enc_chains(EB);                    // chains per round
   constexpr int UT = (int)enc_ticket_units(EB, kMaxRounds);  // units per ticket, at most
@@ -281,9 +296,10 @@ __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64
   uint4* s_tab = (uint4*)wl;  // [SPB <= 2][256]
   uint16_t* wstage = (uint16_t*)(wl + kTabWords);
   const uint32_t dummy_addr = lds_addr(wstage + CH * REG + 2u * lane);
-  uint32_t padj[UT], cntj[UT];  // per unit of the ticket (wave-uniform)
+  static_assert(UT <= kMaxTicketUnits, "a ticket's units fit its slot");
+  uint32_t padj[UT];  // per unit of the ticket (wave-uniform)
 #pragma unroll
-  for (int j = 0; j < UT; ++j) padj[j] = 0, cntj[j] = 0;
+  for (int j = 0; j < UT; ++j) padj[j] = 0;
 
 #pragma unroll 1
   for (uint32_t r = 0; r < R; ++r) {
@@ -434,74 +450,22 @@ __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64
       }
 #pragma unroll
       for (int j = 0; j < UT; ++j)
-        if ((uint32_t)j == j0 + (uint32_t)c) padj[j] = padded, cntj[j] = cnt;
+        if ((uint32_t)j == j0 + (uint32_t)c) padj[j] = padded;
     }
   }
 
-  // the ticket's offset, then block word offsets and words in unit order
+  // the units' padded sizes to the slot's tail, the aggregate to the
+  // ticket's status word
   uint64_t agg = 0;
-  uint32_t qb[UT + 1];  // first uint4 of each unit in the ticket's output
-  qb[0] = 0;
+  uint32_t pv = 0;
 #pragma unroll
   for (int j = 0; j < UT; ++j) {
     agg += padj[j];
-    qb[j + 1] = qb[j] + padj[j] / 8;
+    pv = lane == (uint32_t)j ? padj[j] : pv;
   }
-  const uint64_t excl = look_back_gen(a.status, tix, tix - lt, agg, gen, (int)lane);
-  drain();  // the slot was written by other lanes of this wave
-  if (lane == 0) {
-    uint64_t y = excl;
-#pragma unroll
-    for (int j = 0; j < UT; ++j) {
-      const uint64_t u = u0 + j;
-      if ((uint32_t)j < UTR && u < U) {
-        const uint32_t kp = (uint32_t)(u % P);
-        ((uint32_t*)(B.out + B.section[kp] + kSectionFixed + 4ull * kL * B.nb + 8ull * (u / P)))[1] = (uint32_t)y;
-      }
-      y += padj[j];
-    }
-  }
-  {
-    const uint32_t nq = qb[UT];
-    const uint4* s4 = (const uint4*)spill;
-    uint4* dst = (uint4*)(B.out + B.data_off + 2ull * excl);
-    constexpr int CBQ = 8;
-    for (uint32_t j0 = 0; j0 < nq; j0 += CBQ * kL) {
-      uint4 q[CBQ];
-#pragma unroll
-      for (int u = 0; u < CBQ; ++u) {
-        const uint32_t j = j0 + u * kL + lane;
-        const uint32_t jj = j < nq ? j : 0u;
-        uint32_t dd = 0;
-#pragma unroll
-        for (int k = 1; k < UT; ++k) dd = jj >= qb[k] ? k * (CSTAGE / 8) - qb[k] : dd;
-        q[u] = s4[jj + dd];
-      }
-#pragma unroll
-      for (int u = 0; u < CBQ; ++u) {
-        const uint32_t j = j0 + u * kL + lane;
-        if (j < nq) dst[j] = q[u];
-      }
-    }
-  }
-  if (lt + 1 == B.nticks && lane == 0) {
-    const uint64_t total_words = excl + agg;
-    const uint64_t size = B.data_off + 2ull * total_words;
-    a.sizes[b] = size;
-    ((uint64_t*)B.out)[6] = total_words;  // data_words at byte 48
-    ((uint64_t*)B.out)[7] = size;         // total_size at byte 56
-  }
-  // the last ticket of the buffer to finish writes each plane's
-  // totalCompressedWords (section header field 3)
-  publish();
-  uint32_t f = 0;
-  if (lane == 0) f = atomicAdd(a.fin + b, 1u);
-  if (wave_u32(f) + 1 == B.nticks && lane < B.eb) {
-    const uint32_t v = atomic_ld32(a.ptot + 4ull * b + lane);
-    atomic_st32(a.ptot + 4ull * b + lane, 0u);
-    ((uint32_t*)(B.out + B.section[lane]))[3] = v;
-    if (lane == 0) atomic_st32(a.fin + b, 0u);
-  }
+  if (lane < kMaxTicketUnits) ((uint32_t*)(spill + kMaxTicketUnits * CSTAGE))[lane] = pv;
+  publish_agg_gen(a.status, tix, tix - lt, agg, gen, (int)lane);
+  return agg;
 }
 
 // ---- 32-lane (reference-format) encode ticket ------------------------------
@@ -512,10 +476,10 @@ __device__ void encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64
 // words (checked every 4 steps, at most 32 words per step) and a slot area of
 // CSTAGE / 2 words in the wave's spill area.  The pair's planes (both planes
 // of a uint16 block, two of a uint32 block, or two blocks of uint8 plane 0)
-// have their tables at s_tab[kp - p0].  One look-back per ticket, as for the
-// 64-lane tickets.
+// have their tables at s_tab[kp - p0].  Published and resolved as the
+// 64-lane tickets are.
 template <int PB, int EB, int KBB>
-__device__ void encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
+__device__ uint64_t encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
                                 uint32_t gen, uint32_t* wl, uint16_t* spill, uint64_t& tkey) {
   constexpr uint32_t L = 32, BLOCK = 4096;
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
@@ -648,31 +612,74 @@ __device__ void encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, uint
     }
   }
 
-  // the ticket's offset, then block word offsets and words in unit order
+  // the units' padded sizes to the slot's tail, the aggregate to the
+  // ticket's status word
   const uint64_t agg = (uint64_t)pad[0] + pad[1];
-  const uint64_t excl = look_back_gen(a.status, tix, tix - lt, agg, gen, (int)lane);
-  drain();  // the slot areas were written by other lanes of this wave
-  if (act && lin == 0) {
-    const uint8_t* sec = B.out + B.section[kp] + kSectionFixed;
-    ((uint32_t*)(sec + 4ull * L * B.nb + 8ull * blk))[1] = (uint32_t)(excl + (half ? pad[0] : 0u));
+  if (lane < kMaxTicketUnits)
+    ((uint32_t*)(spill + kMaxTicketUnits * CSTAGE))[lane] = lane == 0 ? pad[0] : (lane == 1 ? pad[1] : 0u);
+  publish_agg_gen(a.status, tix, tix - lt, agg, gen, (int)lane);
+  return agg;
+}
+
+// ---- placing a resolved ticket ---------------------------------------------
+// Ticket lt of buffer B, coded by this wavefront into `slot` with aggregate
+// agg and exclusive offset excl (in words): each unit's block word offset,
+// its words from the slot areas (unit j at j x AREA words) to their place in
+// unit order; the buffer's last ticket writes the frame's sizes, the last to
+// finish each plane's totalCompressedWords (section header field 3).
+template <int PB, int L>
+__device__ void place_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t agg, uint64_t excl,
+                             const BatchEncodeArgs& a, const uint16_t* slot) {
+  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+  constexpr uint32_t AREA = L == 32 ? CSTAGE / 2 : CSTAGE;  // a unit's slot area
+  constexpr int UT = L == 32 ? 2 : (int)kMaxTicketUnits;     // units per ticket, at most
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t P = B.eb;
+  const uint64_t U = (uint64_t)P * B.nb;
+  const uint32_t ut = L == 32 ? 2u : enc_ticket_units(P, a.rounds);
+  const uint64_t u0 = (uint64_t)ut * lt;
+  // lane j < UT: unit j's padded words and the first uint4 of its words in
+  // the ticket's output (exclusive scan over the units)
+  publish();  // the slot's stores (by every lane of this wave) are done
+  const uint32_t pq = lane < (uint32_t)UT ? ((const uint32_t*)(slot + kMaxTicketUnits * CSTAGE))[lane] / 8u : 0u;
+  uint32_t inc = pq;
+#pragma unroll
+  for (int o = 1; o < UT; o <<= 1) {
+    const uint32_t v = __shfl_up(inc, o, 64);
+    if (lane >= (uint32_t)o) inc += v;
+  }
+  const uint32_t exq = inc - pq;
+  const uint32_t nq = __builtin_amdgcn_readlane(inc, UT - 1);  // the ticket's uint4s
+  uint32_t qb[UT];  // first uint4 of each unit (wave-uniform)
+#pragma unroll
+  for (int k = 1; k < UT; ++k) qb[k] = __builtin_amdgcn_readlane(exq, k);
+  {
+    const uint64_t u = u0 + lane;
+    if (lane < ut && u < U) {
+      const uint32_t kp = (uint32_t)(u % P);
+      ((uint32_t*)(B.out + B.section[kp] + kSectionFixed + 4ull * L * B.nb + 8ull * (u / P)))[1] =
+          (uint32_t)(excl + 8ull * exq);
+    }
   }
   {
-    const uint32_t q0 = pad[0] / 8, nq = q0 + pad[1] / 8;
-    const uint4* s4 = (const uint4*)spill;
+    const uint4* s4 = (const uint4*)slot;
     uint4* dst = (uint4*)(B.out + B.data_off + 2ull * excl);
     constexpr int CBQ = 8;
     for (uint32_t j0 = 0; j0 < nq; j0 += CBQ * 64) {
       uint4 q[CBQ];
 #pragma unroll
-      for (int k = 0; k < CBQ; ++k) {
-        const uint32_t j = j0 + k * 64 + lane;
+      for (int u = 0; u < CBQ; ++u) {
+        const uint32_t j = j0 + u * 64 + lane;
         const uint32_t jj = j < nq ? j : 0u;
-        q[k] = s4[jj >= q0 ? jj - q0 + SSTAGE / 8 : jj];
+        uint32_t dd = 0;
+#pragma unroll
+        for (int k = 1; k < UT; ++k) dd = jj >= qb[k] ? k * (AREA / 8) - qb[k] : dd;
+        q[u] = s4[jj + dd];
       }
 #pragma unroll
-      for (int k = 0; k < CBQ; ++k) {
-        const uint32_t j = j0 + k * 64 + lane;
-        if (j < nq) dst[j] = q[k];
+      for (int u = 0; u < CBQ; ++u) {
+        const uint32_t j = j0 + u * 64 + lane;
+        if (j < nq) dst[j] = q[u];
       }
     }
   }
@@ -683,8 +690,6 @@ __device__ void encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, uint
     ((uint64_t*)B.out)[6] = total_words;  // data_words at byte 48
     ((uint64_t*)B.out)[7] = size;         // total_size at byte 56
   }
-  // the last ticket of the buffer to finish writes each plane's
-  // totalCompressedWords (section header field 3)
   publish();
   uint32_t f = 0;
   if (lane == 0) f = atomicAdd(a.fin + b, 1u);
@@ -698,20 +703,36 @@ __device__ void encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, uint
 
 // L: every buffer's frame format, 64 or 32 lanes (a kernel of its own for
 // each, so neither ticket coder shares a kernel body with the other).
+// (At most 168 VGPRs: the wave's LDS allows three four-wave workgroups per
+// CU, three wavefronts per SIMD, and a larger register budget would halve
+// that to two.)
 template <int PB, int NW, int KB, int L>
-__global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a) {
-  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) void batch_encode_kernel(
+    BatchEncodeArgs a) {
   extern __shared__ uint32_t lds[];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = wave_u32(threadIdx.x >> 6);
   uint32_t* wl = lds + wave * kWaveWords;
-  uint16_t* const spill = a.spill + ((uint64_t)blockIdx.x * NW + wave) * (16ull * CSTAGE);
+  // two slots, used in turn: the ticket being coded and the one whose offset
+  // is still to be resolved
+  uint16_t* const spill = a.spill + ((uint64_t)blockIdx.x * NW + wave) * (2ull * batch_slot_words<PB>());
   for (uint32_t i = lane; i < kTabWords; i += 64) wl[i] = 0;  // histogram counters start at zero
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   // this launch's generation (every wavefront reads it before the last one
   // out advances it)
   const uint32_t gen = wave_u32(lane == 0 ? atomic_ld32(a.queue + 2) : 0u);
   uint64_t tkey = ~0ull;  // tables held in this wave's LDS
+  // the ticket coded last if not resolved yet (wave-uniform), in the slot
+  // other than `slot`
+  bool pend = false;
+  uint32_t p_b = 0, p_lt = 0, slot = 0;
+  uint64_t p_agg = 0;
+  auto resolve_pending = [&]() {
+    const BatchBuf& PBuf = a.bufs[p_b];
+    const uint64_t ptix = PBuf.tick_first + p_lt;
+    const uint64_t excl = resolve_gen<false>(a.status, ptix, ptix - p_lt, p_agg, gen, (int)lane);
+    place_ticket<PB, L>(PBuf, p_b, p_lt, p_agg, excl, a, spill + (slot ^ 1u) * batch_slot_words<PB>());
+  };
 
   uint32_t it = 0;
   if (lane == 0) it = atomicAdd(a.queue, 1u);
@@ -745,14 +766,33 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
       while (wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u) != gen) __builtin_amdgcn_s_sleep(2);
       // (the tables share LDS with the histogram counters; items start in
       // queue order, so no histogram item follows a ticket in a wave)
+      uint16_t* const sl = spill + slot * batch_slot_words<PB>();
+      uint64_t agg;
       if constexpr (L == 32) {
-        if (B.eb == 1) encode_ticket32<PB, 1, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
-        else if (B.eb == 2) encode_ticket32<PB, 2, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
-        else encode_ticket32<PB, 4, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
+        if (B.eb == 1) agg = encode_ticket32<PB, 1, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
+        else if (B.eb == 2) agg = encode_ticket32<PB, 2, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
+        else agg = encode_ticket32<PB, 4, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
       } else {
-        if (B.eb == 1) encode_ticket<PB, 1, (KB / 4 < 4 ? 4 : KB / 4)>(B, b, lt, tix, a, gen, wl, spill, tkey);
-        else if (B.eb == 2) encode_ticket<PB, 2, (KB / 2 < 4 ? 4 : KB / 2)>(B, b, lt, tix, a, gen, wl, spill, tkey);
-        else encode_ticket<PB, 4, KB>(B, b, lt, tix, a, gen, wl, spill, tkey);
+        if (B.eb == 1) agg = encode_ticket<PB, 1, (KB / 4 < 4 ? 4 : KB / 4)>(B, b, lt, tix, a, gen, wl, sl, tkey);
+        else if (B.eb == 2) agg = encode_ticket<PB, 2, (KB / 2 < 4 ? 4 : KB / 2)>(B, b, lt, tix, a, gen, wl, sl, tkey);
+        else agg = encode_ticket<PB, 4, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
+      }
+      // a ticket left unresolved last time (its predecessors have had this
+      // ticket's coding time to publish), then this one if its predecessors
+      // have all published, else it waits in its slot for the next ticket
+      if (pend) {
+        resolve_pending();
+        pend = false;
+      }
+      const uint64_t excl = resolve_gen<true>(a.status, tix, tix - lt, agg, gen, (int)lane);
+      if (excl != ~0ull) {
+        place_ticket<PB, L>(B, b, lt, agg, excl, a, sl);
+      } else {
+        pend = true;
+        p_b = b;
+        p_lt = lt;
+        p_agg = agg;
+        slot ^= 1u;
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -761,6 +801,7 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
     if (lane == 0) nxt = atomicAdd(a.queue, 1u);
     it = wave_u32(nxt);
   }
+  if (pend) resolve_pending();
   // the last wavefront out resets the queue and advances the generation for
   // the next launch (a replayed graph passes the same arguments every time)
   if (lane == 0) {
@@ -775,7 +816,6 @@ __global__ __launch_bounds__(64 * NW) void batch_encode_kernel(BatchEncodeArgs a
 
 template <int PB, int NW, int KB>
 hipError_t launch_enc_shape(const BatchEncodeArgs& a, int cu, hipStream_t s, uint64_t* need_spill) {
-  constexpr uint32_t CSTAGE = chain_stage_words<PB>();
   constexpr size_t lds = (size_t)NW * kWaveWords * 4u;
   static_assert(lds == batch_encode_lds(BatchShape{NW, KB}), "encode LDS layout");
   if constexpr (lds > kLdsBytes) {
@@ -791,7 +831,7 @@ hipError_t launch_enc_shape(const BatchEncodeArgs& a, int cu, hipStream_t s, uin
     const uint64_t want = div_up(a.nitems, NW);
     if (want < grid) grid = want;
     if (grid < 1) grid = 1;
-    const uint64_t spill_bytes = grid * NW * 16ull * CSTAGE * 2ull;  // a slot area per unit of a ticket
+    const uint64_t spill_bytes = grid * NW * 2ull * batch_slot_words<PB>() * 2ull;  // two slots per wave
     if (need_spill) {
       *need_spill = spill_bytes;
       return hipSuccess;

@@ -257,9 +257,11 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
 // Lane l reads entries j - l - 64 k, k < K: 64 K entries per round trip.
 // Groups of 64 are taken in order; a group with an unpublished entry before
 // its nearest prefix ends the round (the groups before it are kept) and the
-// wave sleeps briefly before reading again.  dec(raw, flag, value) splits a
-// status word (flag 0: not yet published, 1: aggregate, 2: inclusive prefix).
-template <int K, typename Dec>
+// wave sleeps briefly before reading again; with TRY it returns ~0 instead
+// (an entry still unpublished: the caller resolves later).  dec(raw, flag,
+// value) splits a status word (flag 0: not yet published, 1: aggregate,
+// 2: inclusive prefix).
+template <int K, bool TRY = false, typename Dec>
 __device__ __forceinline__ uint64_t look_back_window(const uint64_t* status, int64_t j, int64_t first,
                                                      uint64_t zero_prefix, int lane, Dec dec) {
   uint64_t excl = 0;
@@ -281,7 +283,10 @@ __device__ __forceinline__ uint64_t look_back_window(const uint64_t* status, int
       const uint64_t pm = __ballot(f == 2), nm = __ballot(f == 0);
       // lanes up to and including the nearest prefix (all 64 if none)
       const uint64_t upto = pm ? (pm & (0 - pm)) * 2u - 1u : ~0ull;
-      if (nm & upto) break;
+      if (nm & upto) {
+        if constexpr (TRY) return ~0ull;
+        break;
+      }
       part += ((upto >> lane) & 1u) ? val : 0u;
       ++groups;
       if (pm) {

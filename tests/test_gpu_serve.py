@@ -110,7 +110,9 @@ def big_requests(count: int, n: int, ws):
 def test_server_admission_waits_for_memory(ws):
     # outputs of all requests together exceed the arena: later requests are
     # admitted only as earlier results are released
-    srv = mans_amd.Server(mans_amd.params(10, 64), 2 << 30)
+    # (the server's own state, two slots of batch workspaces, takes about
+    # 2.1 GB of the 4 GiB)
+    srv = mans_amd.Server(mans_amd.params(10, 64), 4 << 30)
     arena = srv.stats()["arena_bytes"]
     n = 16 << 20  # 32 MiB of uint16 per request
     count = int(1.5 * arena // (2 * n)) + 4
