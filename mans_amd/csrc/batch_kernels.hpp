@@ -108,58 +108,51 @@ __device__ uint64_t resolve_gen(uint64_t* status, uint64_t idx, uint64_t first, 
   return excl;
 }
 
-// ---- histogram item -------------------------------------------------------
-// Counts bytes [c0, c1) of buffer B (plane = byte offset % P) into this
-// wave's LDS counters and adds them to the buffer's accumulators.  The
-// counters are 16-bit pairs: [P][128 bin pairs][16 / P replicas] words (bins
-// 2i and 2i + 1 in the low and high half of word i), lane l adding into
-// replica l % (16 / P).  A replica counts at most 64 / (16 / P) lanes x
-// chunk / 64 / P symbols of a plane (16384 for 256 KiB chunks), so the halves
-// cannot carry into each other.  Loads: eight 16-byte loads per lane in
-// flight while the previous eight are counted.
-static_assert(kBatchHistChunkMax / 16u < 65536u, "16-bit pair counters");
-static_assert(4u * 128u * (16u / 4u) <= kTabWords, "counters fit the wave's table area");
+// ---- histogram chunks -----------------------------------------------------
+// A histogram item (bytes [c0, c1) of a buffer; plane = byte offset % P) is
+// counted by a whole workgroup into its LDS counters [P][256 bins][R]
+// (32-bit words, R = 32 / P replicas, thread t adding into replica t % R:
+// the 32 lanes of an LDS pass hit at most two words of one bank, and for
+// uint8 buffers none), which it then adds to the buffer's accumulators.
+// Loads: kHistLoads 16-byte loads per thread in flight while the previous
+// ones are counted.
+constexpr uint32_t kHistCntWords = 8192;  // P x 256 x 32 / P
 constexpr int kHistLoads = 4;
 
-// (EB: the buffer's element bytes = planes, a template argument so the
-// plane of each byte and the replica stride are constants)
-template <uint32_t EB>
-__device__ __forceinline__ void hist_chunk(const BatchBuf& B, uint32_t b, uint64_t c0, uint64_t c1, uint32_t* cnt,
-                                           uint32_t* ghist) {
-  const uint32_t lane = threadIdx.x & 63u;
-  constexpr uint32_t P = EB;
-  constexpr uint32_t rep = 16u / P;
-  uint32_t* c = cnt + lane % rep;
-  const uint4* p4 = (const uint4*)(B.in + c0);
-  const uint64_t n16 = (c1 - c0) / 16u;
-  auto add = [&](uint32_t plane, uint32_t byte) {
-    atomicAdd(c + (plane * 128u + (byte >> 1)) * rep, (byte & 1u) ? 0x10000u : 1u);
-  };
+template <uint32_t EB, int NW>
+__device__ __forceinline__ void wg_hist_chunk(const uint8_t* in, uint64_t c0, uint64_t c1, uint32_t* cnt) {
+  constexpr uint32_t P = EB, R = 32u / EB, T = 64u * NW;
+  const uint32_t t = threadIdx.x;
+  uint32_t* const c = cnt + t % R;
+  const uint4* p4 = (const uint4*)(in + c0);
+  const uint32_t n16 = (uint32_t)((c1 - c0) / 16u);  // (a chunk is at most 512 KiB)
   auto count16 = [&](const uint4& v) {
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int m = 0; m < 4; ++m) add((uint32_t)(4 * q + m) % P, __builtin_amdgcn_ubfe(w[q], 8u * (uint32_t)m, 8u));
+      for (int m = 0; m < 4; ++m)
+        atomicAdd(c + ((uint32_t)((4 * q + m) % (int)P) * 256u + __builtin_amdgcn_ubfe(w[q], 8u * (uint32_t)m, 8u)) * R,
+                  1u);
   };
-  // rows of kHistLoads uint4 per lane, two buffers; loads are unconditional
-  // (indices past the end re-read uint4 0 and are not counted)
-  constexpr uint64_t ROW = 64ull * kHistLoads;
-  auto load = [&](uint4 (&buf)[kHistLoads], uint64_t base) {
+  // rows of kHistLoads uint4 per thread, two buffers; loads are
+  // unconditional (indices past the end re-read uint4 0 and are not counted)
+  constexpr uint32_t ROW = T * kHistLoads;
+  auto load = [&](uint4 (&buf)[kHistLoads], uint32_t base) {
 #pragma unroll
     for (int k = 0; k < kHistLoads; ++k) {
-      const uint64_t i = base + 64ull * k + lane;
-      buf[k] = p4[i < n16 ? i : 0];
+      const uint32_t i = base + T * (uint32_t)k + t;
+      buf[k] = p4[i < n16 ? i : 0u];
     }
   };
-  auto count = [&](const uint4 (&buf)[kHistLoads], uint64_t base) {
+  auto count = [&](const uint4 (&buf)[kHistLoads], uint32_t base) {
 #pragma unroll
     for (int k = 0; k < kHistLoads; ++k)
-      if (base + 64ull * k + lane < n16) count16(buf[k]);
+      if (base + T * (uint32_t)k + t < n16) count16(buf[k]);
   };
   if (n16 > 0) {
     uint4 bufA[kHistLoads], bufB[kHistLoads];
-    uint64_t r = 0;
+    uint32_t r = 0;
     load(bufA, 0);
     for (;;) {
       load(bufB, r + ROW);
@@ -172,22 +165,73 @@ __device__ __forceinline__ void hist_chunk(const BatchBuf& B, uint32_t b, uint64
       if (r >= n16) break;
     }
   }
-  const uint64_t tail = c0 + 16u * n16;
-  if (tail + lane < c1) add((uint32_t)((tail - c0 + lane) % P), B.in[tail + lane]);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  // merge the replicas: each lane a bin pair of a plane per round
-  for (uint32_t pr = lane; pr < P * 128u; pr += 64) {
-    uint32_t lo = 0, hi = 0;
-    for (uint32_t q = 0; q < rep; ++q) {
-      const uint32_t v = cnt[pr * rep + q];
-      cnt[pr * rep + q] = 0;
-      lo += v & 0xffffu;
-      hi += v >> 16;
+  const uint64_t tail = c0 + 16ull * n16;  // (a multiple of P from c0: plane t % P)
+  if (tail + t < c1) atomicAdd(c + ((t % P) * 256u + in[tail + t]) * R, 1u);
+}
+
+// Adds the workgroup's counters to buffer b's accumulators and zeroes them:
+// thread t sums bin t (t + T, ...) of the P x 256, its replicas read from a
+// rotated start so the lanes of a pass spread over the banks.
+template <uint32_t EB, int NW>
+__device__ __forceinline__ void wg_hist_merge(uint32_t b, uint32_t* cnt, uint32_t* ghist) {
+  constexpr uint32_t P = EB, R = 32u / EB, T = 64u * NW;
+  for (uint32_t i = threadIdx.x; i < P * 256u; i += T) {
+    uint32_t* const row = cnt + i * R;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < R; ++q) {
+      const uint32_t qq = (q + i) % R;
+      sum += row[qq];
+      row[qq] = 0;
     }
-    uint32_t* g = ghist + (uint64_t)b * 1024u + (pr / 128u) * 256u + 2u * (pr % 128u);
-    if (lo) atomicAdd(g, lo);
-    if (hi) atomicAdd(g + 1, hi);
+    if (sum) atomicAdd(ghist + (uint64_t)b * 1024u + i, sum);
   }
+}
+
+// The histogram phase of a workgroup that starts on histograms: it takes
+// chunk items from their own counter (queue[3]) until none is left, one
+// workgroup barrier round per chunk (the next grab is taken while the
+// current chunk is counted); the workgroup that counts a buffer's last chunk
+// runs its finisher on wavefront 0.  LDS: the counters, the finisher's 256
+// keys, the two grabbed items and the finish flag.
+template <int PB>
+__device__ void finish_buffer(const BatchBuf& B, uint32_t b, const BatchEncodeArgs& a, uint32_t gen, uint32_t* keys);
+
+template <int PB, int NW>
+__device__ void hist_phase(const BatchEncodeArgs& a, uint32_t gen, uint32_t* lds) {
+  static_assert(kHistCntWords + 256u + 4u <= NW * kWaveWords, "the histogram phase fits the workgroup's LDS");
+  uint32_t* const cnt = lds;
+  uint32_t* const keys = lds + kHistCntWords;
+  uint32_t* const s_it = keys + 256;  // [0..1] grabbed items, [2] finish flag
+  for (uint32_t i = threadIdx.x; i < kHistCntWords; i += 64u * NW) cnt[i] = 0;
+  if (threadIdx.x == 0) s_it[0] = atomicAdd(a.queue + 3, 1u);
+  __syncthreads();
+  for (uint32_t k = 0;; ++k) {
+    const uint32_t it = wave_u32(s_it[k & 1u]);
+    if (it >= a.nhist_items) break;
+    if (threadIdx.x == 0) s_it[(k + 1) & 1u] = atomicAdd(a.queue + 3, 1u);
+    const uint2 itm = a.items[it];
+    const uint32_t b = wave_u32(itm.x), li = wave_u32(itm.y);
+    const BatchBuf& B = a.bufs[b];
+    const uint64_t bytes = B.n * B.eb;
+    const uint64_t c0 = (uint64_t)li * a.hist_chunk;
+    const uint64_t c1 = c0 + a.hist_chunk < bytes ? c0 + a.hist_chunk : bytes;
+    if (c0 < c1) {
+      if (B.eb == 1) wg_hist_chunk<1, NW>(B.in, c0, c1, cnt);
+      else if (B.eb == 2) wg_hist_chunk<2, NW>(B.in, c0, c1, cnt);
+      else wg_hist_chunk<4, NW>(B.in, c0, c1, cnt);
+    }
+    __syncthreads();
+    if (B.eb == 1) wg_hist_merge<1, NW>(b, cnt, a.hist);
+    else if (B.eb == 2) wg_hist_merge<2, NW>(b, cnt, a.hist);
+    else wg_hist_merge<4, NW>(b, cnt, a.hist);
+    publish();
+    __syncthreads();
+    if (threadIdx.x == 0) s_it[2] = atomicAdd(a.done + b, 1u) + 1u == B.nhist ? 1u : 0u;
+    __syncthreads();
+    if (s_it[2] && threadIdx.x < 64) finish_buffer<PB>(B, b, a, gen, keys);
+  }
+  __syncthreads();  // (the LDS becomes the wavefronts' ticket areas)
 }
 
 // ---- finisher: one wavefront normalises every plane of buffer b -----------
@@ -716,11 +760,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
   // two slots, used in turn: the ticket being coded and the one whose offset
   // is still to be resolved
   uint16_t* const spill = a.spill + ((uint64_t)blockIdx.x * NW + wave) * (2ull * batch_slot_words<PB>());
-  for (uint32_t i = lane; i < kTabWords; i += 64) wl[i] = 0;  // histogram counters start at zero
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   // this launch's generation (every wavefront reads it before the last one
   // out advances it)
   const uint32_t gen = wave_u32(lane == 0 ? atomic_ld32(a.queue + 2) : 0u);
+  // the first hist_wgs workgroups count histogram chunks until none is left
+  // (the rest start on tickets, whose buffers become ready as their chunks
+  // are counted)
+  if (blockIdx.x < a.hist_wgs) hist_phase<PB, NW>(a, gen, lds);
   uint64_t tkey = ~0ull;  // tables held in this wave's LDS
   // the ticket coded last if not resolved yet (wave-uniform), in the slot
   // other than `slot`
@@ -734,66 +780,45 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
     place_ticket<PB, L>(PBuf, p_b, p_lt, p_agg, excl, a, spill + (slot ^ 1u) * batch_slot_words<PB>());
   };
 
+  // tickets, a wavefront at a time, from their own counter (queue[0])
+  const uint32_t nt = a.nitems - a.nhist_items;
   uint32_t it = 0;
   if (lane == 0) it = atomicAdd(a.queue, 1u);
   it = wave_u32(it);
-  while (it < a.nitems) {
-    const uint2 itm = a.items[it];
+  while (it < nt) {
+    const uint2 itm = a.items[a.nhist_items + it];
     const uint32_t b = wave_u32(itm.x), li = wave_u32(itm.y);
     const BatchBuf& B = a.bufs[b];
-    if (!(li & kItemTicket)) {
-      if (tkey != ~0ull) {
-        // the area held a ticket's tables: the counters start at zero
-        for (uint32_t i = lane; i < kTabWords / 4; i += 64) ((uint4*)wl)[i] = make_uint4(0, 0, 0, 0);
-        drain();
-      }
-      tkey = ~0ull;
-      const uint64_t bytes = B.n * B.eb;
-      const uint64_t c0 = (uint64_t)li * a.hist_chunk;
-      const uint64_t c1 = c0 + a.hist_chunk < bytes ? c0 + a.hist_chunk : bytes;
-      if (c0 < c1) {
-        if (B.eb == 1) hist_chunk<1>(B, b, c0, c1, wl, a.hist);
-        else if (B.eb == 2) hist_chunk<2>(B, b, c0, c1, wl, a.hist);
-        else hist_chunk<4>(B, b, c0, c1, wl, a.hist);
-      }
-      publish();
-      uint32_t d = 0;
-      if (lane == 0) d = atomicAdd(a.done + b, 1u);
-      if (wave_u32(d) + 1 == B.nhist) finish_buffer<PB>(B, b, a, gen, wl + kTabWords);
+    const uint32_t lt = li & ~kItemTicket;
+    const uint32_t tix = B.tick_first + lt;
+    while (wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u) != gen) __builtin_amdgcn_s_sleep(2);
+    uint16_t* const sl = spill + slot * batch_slot_words<PB>();
+    uint64_t agg;
+    if constexpr (L == 32) {
+      if (B.eb == 1) agg = encode_ticket32<PB, 1, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
+      else if (B.eb == 2) agg = encode_ticket32<PB, 2, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
+      else agg = encode_ticket32<PB, 4, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
     } else {
-      const uint32_t lt = li & ~kItemTicket;
-      const uint32_t tix = B.tick_first + lt;
-      while (wave_u32(lane == 0 ? atomic_ld32(a.ready + b) : 0u) != gen) __builtin_amdgcn_s_sleep(2);
-      // (the tables share LDS with the histogram counters; items start in
-      // queue order, so no histogram item follows a ticket in a wave)
-      uint16_t* const sl = spill + slot * batch_slot_words<PB>();
-      uint64_t agg;
-      if constexpr (L == 32) {
-        if (B.eb == 1) agg = encode_ticket32<PB, 1, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
-        else if (B.eb == 2) agg = encode_ticket32<PB, 2, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
-        else agg = encode_ticket32<PB, 4, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
-      } else {
-        if (B.eb == 1) agg = encode_ticket<PB, 1, (KB / 4 < 4 ? 4 : KB / 4)>(B, b, lt, tix, a, gen, wl, sl, tkey);
-        else if (B.eb == 2) agg = encode_ticket<PB, 2, (KB / 2 < 4 ? 4 : KB / 2)>(B, b, lt, tix, a, gen, wl, sl, tkey);
-        else agg = encode_ticket<PB, 4, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
-      }
-      // a ticket left unresolved last time (its predecessors have had this
-      // ticket's coding time to publish), then this one if its predecessors
-      // have all published, else it waits in its slot for the next ticket
-      if (pend) {
-        resolve_pending();
-        pend = false;
-      }
-      const uint64_t excl = resolve_gen<true>(a.status, tix, tix - lt, agg, gen, (int)lane);
-      if (excl != ~0ull) {
-        place_ticket<PB, L>(B, b, lt, agg, excl, a, sl);
-      } else {
-        pend = true;
-        p_b = b;
-        p_lt = lt;
-        p_agg = agg;
-        slot ^= 1u;
-      }
+      if (B.eb == 1) agg = encode_ticket<PB, 1, (KB / 4 < 4 ? 4 : KB / 4)>(B, b, lt, tix, a, gen, wl, sl, tkey);
+      else if (B.eb == 2) agg = encode_ticket<PB, 2, (KB / 2 < 4 ? 4 : KB / 2)>(B, b, lt, tix, a, gen, wl, sl, tkey);
+      else agg = encode_ticket<PB, 4, KB>(B, b, lt, tix, a, gen, wl, sl, tkey);
+    }
+    // a ticket left unresolved last time (its predecessors have had this
+    // ticket's coding time to publish), then this one if its predecessors
+    // have all published, else it waits in its slot for the next ticket
+    if (pend) {
+      resolve_pending();
+      pend = false;
+    }
+    const uint64_t excl = resolve_gen<true>(a.status, tix, tix - lt, agg, gen, (int)lane);
+    if (excl != ~0ull) {
+      place_ticket<PB, L>(B, b, lt, agg, excl, a, sl);
+    } else {
+      pend = true;
+      p_b = b;
+      p_lt = lt;
+      p_agg = agg;
+      slot ^= 1u;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     // the next item only once this one is done (items start in order)
@@ -809,6 +834,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
     if (atomicAdd(a.queue + 1, 1u) + 1 == total) {
       atomic_st32(a.queue, 0u);
       atomic_st32(a.queue + 1, 0u);
+      atomic_st32(a.queue + 3, 0u);
       atomic_st32(a.queue + 2, gen % kBatchGenMask + 1u);
     }
   }
@@ -837,8 +863,12 @@ hipError_t launch_enc_shape(const BatchEncodeArgs& a, int cu, hipStream_t s, uin
       return hipSuccess;
     }
     if (spill_bytes > a.spill_bytes) return hipErrorInvalidValue;
-    if (a.lanes == 32) hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB, 32>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
-    else hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB, 64>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+    // half the workgroups start on histogram chunks (at least one when there
+    // are any: the tickets wait for them)
+    BatchEncodeArgs g = a;
+    g.hist_wgs = a.nhist_items == 0 ? 0u : (uint32_t)(grid > 1 ? grid / 2 : 1);
+    if (a.lanes == 32) hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB, 32>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, g);
+    else hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB, 64>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, g);
     return hipGetLastError();
   }
 }

@@ -1050,7 +1050,7 @@ struct mans_batch {
   uint64_t* status = nullptr; // [cap_ticks]
   uint32_t* ptot = nullptr;   // [cap_bufs][4]
   uint32_t* fin = nullptr;    // [cap_bufs]
-  uint32_t* queue = nullptr;  // [3] next item, exited waves, generation
+  uint32_t* queue = nullptr;  // [4] next item, exited waves, generation, next chunk
   uint32_t* luts = nullptr;   // decode: [cap_bufs][4][2^p]
   BatchBuf* h_bufs = nullptr; // pinned staging for asynchronous plans (or null):
                               // descriptors then items, one upload
@@ -1200,10 +1200,10 @@ mans_status batch_alloc(mans_batch* b, uint32_t cap_bufs, uint64_t cap_items, ui
     b->items_dev = b->item_buf;
   }
   HIP_TRY(hipMalloc(&b->ready, nb1 * 4));
-  HIP_TRY(hipMalloc(&b->queue, 12));
+  HIP_TRY(hipMalloc(&b->queue, 16));
   HIP_TRY(hipMemset(b->ready, 0, nb1 * 4));
-  const uint32_t q0[3] = {0, 0, 1};  // next item, exited waves, first generation
-  HIP_TRY(hipMemcpy(b->queue, q0, 12, hipMemcpyHostToDevice));
+  const uint32_t q0[4] = {0, 0, 1, 0};  // next item, exited waves, first generation, next chunk
+  HIP_TRY(hipMemcpy(b->queue, q0, 16, hipMemcpyHostToDevice));
   if (b->decode) {
     HIP_TRY(hipMalloc(&b->luts, nb1 * 4 * (4ull << b->pb)));
     HIP_TRY(hipMalloc(&b->bad, nb1 * 4));
@@ -1253,23 +1253,12 @@ mans_status batch_plan(mans_batch* b, const mans_params* p, const mans_buffer* h
     for (int32_t i = 0; i < count; ++i)
       for (uint32_t r = 0; r < bufs[(size_t)i].nticks; ++r) dst[k++] = make_uint2((uint32_t)i, 0u);
   } else {
-    // histogram chunks in buffer order; a buffer's tickets follow once
-    // `lag` more items have been queued after its last chunk (by then its
-    // chunks are counted, normally), so counting and coding overlap
-    const uint64_t lag = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)b->ws->cu * 16u, nhist / 4));
-    std::vector<std::pair<uint64_t, int32_t>> pending;  // (item after the buffer's chunks, buffer)
-    size_t head = 0;
-    auto emit_tickets = [&](int32_t i) {
+    // histogram chunks in buffer order, then the tickets in buffer order
+    // (the kernel takes each kind from its own counter)
+    for (int32_t i = 0; i < count; ++i)
+      for (uint32_t r = 0; r < bufs[(size_t)i].nhist; ++r) dst[k++] = make_uint2((uint32_t)i, r);
+    for (int32_t i = 0; i < count; ++i)
       for (uint32_t r = 0; r < bufs[(size_t)i].nticks; ++r) dst[k++] = make_uint2((uint32_t)i, kItemTicket | r);
-    };
-    for (int32_t i = 0; i < count; ++i) {
-      for (uint32_t r = 0; r < bufs[(size_t)i].nhist; ++r) {
-        dst[k++] = make_uint2((uint32_t)i, r);
-        while (head < pending.size() && k >= pending[head].first + lag) emit_tickets(pending[head++].second);
-      }
-      pending.emplace_back(k, i);
-    }
-    while (head < pending.size()) emit_tickets(pending[head++].second);
   }
   if (b->h_bufs) {
     if (count > 0) memcpy(b->h_bufs, bufs.data(), bufs.size() * sizeof(BatchBuf));

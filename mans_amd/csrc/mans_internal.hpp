@@ -227,8 +227,8 @@ struct BatchEncodeArgs {
   const BatchBuf* bufs;
   const uint2* items;            // [nitems] histogram chunks and encode tickets
   uint32_t nbufs;
-  uint32_t nhist_items;          // histogram chunks among the items
-  uint32_t nitems;               // items: a buffer's chunks come before its tickets
+  uint32_t nhist_items;          // items [0, nhist_items): histogram chunks in buffer order
+  uint32_t nitems;               // then the encode tickets in buffer order
   uint32_t* hist;                // [nbufs][4][256] accumulators (left zeroed)
   uint32_t* done;                // [nbufs] histogram items finished (left zeroed)
   uint32_t* ready;               // [nbufs] generation whose tables are ready
@@ -236,9 +236,10 @@ struct BatchEncodeArgs {
   uint64_t* status;              // [tickets] look-back words tagged with gen
   uint32_t* ptot;                // [nbufs][4] words per plane (left zeroed)
   uint32_t* fin;                 // [nbufs] encode tickets finished (left zeroed)
-  uint32_t* queue;               // [3] next item, exited wavefronts (both left
+  uint32_t* queue;               // [4] next ticket, exited wavefronts (both left
                                  // zeroed), launch generation 1 .. kBatchGenMask
-                                 // (advanced by the last wavefront out)
+                                 // (advanced by the last wavefront out), next
+                                 // histogram chunk (left zeroed)
   uint32_t* err;
   uint64_t* sizes;               // [nbufs] frame sizes out
   uint16_t* spill;               // per-wave spill areas
@@ -246,6 +247,7 @@ struct BatchEncodeArgs {
   uint32_t hist_chunk;           // bytes per histogram item (BatchGrain)
   uint32_t rounds;               // rounds per encode ticket (BatchGrain; 64 lanes)
   uint32_t lanes;                // every buffer's frame: 64 or 32 lanes
+  uint32_t hist_wgs;             // workgroups that start on histogram chunks (set at launch)
 };
 // Batched decode: BatchBuf.in is the frame, .out the elements; items
 // [0, nbufs) build buffer b's LUTs, then come the decode tickets.
@@ -256,7 +258,7 @@ struct BatchDecodeArgs {
   uint32_t nitems;
   uint32_t* ready;               // [nbufs] generation whose LUTs are built (| kBatchBad: frame rejected)
   uint32_t* luts;                // [nbufs][4][2^p] packed LUT entries
-  uint32_t* queue;               // [3] as BatchEncodeArgs::queue
+  uint32_t* queue;               // [4] as BatchEncodeArgs::queue (next item, exited, generation)
   uint32_t* err;
   uint32_t* bad;                 // [nbufs] the error flags of each buffer (zeroed before the launch)
   uint32_t run;                  // items a wavefront takes per queue grab (BatchGrain)
