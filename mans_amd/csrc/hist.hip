@@ -103,48 +103,53 @@ __device__ uint4 normalize_symbol(uint32_t count, uint32_t total, int p, uint32_
   return make_uint4(pdf, cdf, magic, shift);
 }
 
-// Conflict-free counting.  Each wavefront owns one plane (wave w counts
-// plane w % P of its group's stripe of the input) and kRep lane-private
-// replicas laid out [bin][replica]: lane l adds into replica l % kRep, so the
-// 32 lanes of an LDS pass hit 32 distinct (bin * kRep + l % kRep) words and a
-// skewed plane (one or two byte values) never serialises on a bin.
-constexpr int kRep = 16;
-constexpr int kCountWords = kNumSymbols * kRep;   // per wave: 16 KiB
-constexpr int kLoads = 8;                         // 16-byte loads in flight per lane per buffer
+// Counting.  The workgroup shares one set of counters, [P][256 bins][R]
+// 32-bit words with R = 32 replicas for 1 or 2 planes and 16 for 3 or 4
+// (64 KiB), thread t adding into replica t % R: with R = 32 the 32 lanes of
+// an LDS pass hit 32 different banks whatever the bytes, so a skewed plane
+// (one or two byte values) never serialises on a word.  Every thread loads
+// bytes no other thread loads and counts all of their planes (byte b of an
+// element-aligned uint4 is plane plane0 + b % S).  (Measured on 1 GiB of
+// uint16 codes: 0.30 ms; a wave per plane with 16 private replicas each,
+// both waves loading the same bytes, 0.33 ms; 32 KiB of 16 / 8 replicas at
+// five workgroups per CU, 0.43 ms.)
+constexpr int kCountWords = 16384;
+constexpr int kLoads = 8;                                  // 16-byte loads in flight per thread per buffer
+constexpr uint32_t kRowU4 = (uint32_t)kThreads * kLoads;  // uint4s per row (32 KiB)
+static_assert(4 * 256 * 16 <= kCountWords && 2 * 256 * 32 <= kCountWords, "counters fit");
 
-template <int S>
-__device__ __forceinline__ void count_u4(uint32_t* c, const uint4& v, uint32_t j0) {
-  // bytes j0, j0 + S, ... of each dword belong to this wave's plane
+// c: this thread's replica of plane plane0 (plane plane0 + j at c + 256 R j)
+template <int S, int R>
+__device__ __forceinline__ void count_u4(uint32_t* c, const uint4& v) {
   const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
-    for (int q = 0; q < 4 / S; ++q)
-      atomicAdd(c + __builtin_amdgcn_ubfe(w[k], 8u * (j0 + (uint32_t)(q * S)), 8u) * kRep, 1u);
+    for (int m = 0; m < 4; ++m)
+      atomicAdd(c + (uint32_t)((4 * k + m) % S) * 256u * R + __builtin_amdgcn_ubfe(w[k], 8u * (uint32_t)m, 8u) * R, 1u);
 }
 
-// Whole rows [r0, r1) of a source, counted by one wavefront: row r is the
-// 64 kLoads uint4s from r * 64 kLoads, lane l loading uint4s l, l + 64, ...
+// Whole rows [r0, r1) of a source, counted by the workgroup: row r is the
+// kRowU4 uint4s from r * kRowU4, thread t loading uint4s t, t + kThreads, ...
 // of it.  Two register buffers, so one row loads while the other is
-// counted; row indices are wave-uniform (scalar row base, one lane offset
-// for every load).
-template <int S>
-__device__ void count_rows(const uint4* body, uint32_t r0, uint32_t r1, uint32_t* c, uint32_t j0) {
-  constexpr uint64_t row = 64ull * kLoads;
-  const uint32_t lane = threadIdx.x & 63;
+// counted; row indices are uniform (scalar row base, one thread offset for
+// every load).
+template <int S, int R>
+__device__ __forceinline__ void count_rows(const uint4* body, uint32_t r0, uint32_t r1, uint32_t* c) {
+  const uint32_t t = threadIdx.x;
   uint4 bufA[kLoads], bufB[kLoads];
   // (32-bit row indices: their compares stay scalar)
   auto load = [&](uint4 (&buf)[kLoads], uint32_t r) {
-    const uint4* rp = body + (uint64_t)r * row;
+    const uint4* rp = body + (uint64_t)r * kRowU4;
 #pragma unroll
-    for (int k = 0; k < kLoads; ++k) buf[k] = rp[lane + 64u * (uint32_t)k];
+    for (int k = 0; k < kLoads; ++k) buf[k] = rp[t + (uint32_t)kThreads * (uint32_t)k];
     // issue the row's loads here, ahead of the counting that follows (the
     // scheduler would otherwise move counting of the row in flight above them)
     __builtin_amdgcn_sched_barrier(0);
   };
   auto count = [&](const uint4 (&buf)[kLoads]) {
 #pragma unroll
-    for (int k = 0; k < kLoads; ++k) count_u4<S>(c, buf[k], j0);
+    for (int k = 0; k < kLoads; ++k) count_u4<S, R>(c, buf[k]);
   };
   // (the prefetch past the last row re-reads that row: straight-line code,
   // so the compiler keeps it in flight while the current row is counted)
@@ -160,20 +165,20 @@ __device__ void count_rows(const uint4* body, uint32_t r0, uint32_t r1, uint32_t
   }
 }
 
-// Rows per group and grab.  Workgroups take their rows in chunks from a
-// device-wide counter instead of a fixed share: workgroups on different CUs
-// stream at different rates (a fixed share ends 1.6x apart from the fastest
-// to the slowest), so a static split leaves the launch waiting on its
+// Rows per grab.  Workgroups take their rows in chunks from a device-wide
+// counter instead of a fixed share: workgroups on different CUs stream at
+// different rates, so a static split leaves the launch waiting on its
 // slowest workgroup.
-constexpr uint32_t kChunkRows = 8;
+constexpr uint32_t kChunkRows = 4;
 
-// One source, every wavefront of the workgroup taking part: the workgroup
-// grabs one chunk per group, each wave of the group counts its plane's bytes
-// of the group's chunk.  The next grab is taken while the current chunk is
-// counted (s_grab double-buffered, one barrier per chunk).
-template <int S>
-__device__ void count_source(const HistSource& src, uint32_t* c, uint32_t j0, bool active, uint32_t group,
-                             uint32_t groups, bool edges, uint32_t* grab, uint32_t* s_grab) {
+// One source: the workgroup grabs a chunk, counts it, and takes the next
+// grab while the current chunk is counted (s_grab double-buffered, one
+// barrier per chunk).  The edges (the uint4s after the whole rows, the
+// unaligned head and tail bytes) are counted by workgroup 0.  c: this
+// thread's replica of plane plane0.
+template <int S, int R>
+__device__ __forceinline__ void count_source(const HistSource& src, uint32_t* c, bool edges, uint32_t* grab,
+                             uint32_t* s_grab) {
   const uint8_t* p = src.ptr;
   const uint64_t n = src.nbytes;
   const uint32_t mis = (uint32_t)((uintptr_t)p & 15u);
@@ -181,45 +186,64 @@ __device__ void count_source(const HistSource& src, uint32_t* c, uint32_t j0, bo
   const uint64_t n4 = (n - head) / 16u;
   const uint64_t tail0 = head + 16u * n4;
   const uint4* body = (const uint4*)(p + head);
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t nrows = n4 / (64ull * kLoads);  // whole rows; the rest with the edges
+  const uint32_t t = threadIdx.x;
+  const uint64_t nrows = n4 / kRowU4;  // whole rows; the rest with the edges
   const uint64_t nchunks = div_up(nrows, (uint64_t)kChunkRows);
-  if (threadIdx.x == 0) s_grab[0] = atomicAdd(grab, groups);
+  if (t == 0) s_grab[0] = atomicAdd(grab, 1u);
   __syncthreads();
   for (uint32_t k = 0;; ++k) {
     const uint32_t cb = s_grab[k & 1u];
     if (cb >= nchunks) break;
-    if (threadIdx.x == 0) s_grab[(k + 1) & 1u] = atomicAdd(grab, groups);
-    const uint64_t ch = (uint64_t)cb + group;
-    if (active && ch < nchunks) {
-      const uint64_t r1 = (ch + 1) * kChunkRows;
-      count_rows<S>(body, (uint32_t)(ch * kChunkRows), (uint32_t)(r1 < nrows ? r1 : nrows), c, j0);
-    }
+    if (t == 0) s_grab[(k + 1) & 1u] = atomicAdd(grab, 1u);
+    const uint64_t r1 = ((uint64_t)cb + 1) * kChunkRows;
+    count_rows<S, R>(body, cb * kChunkRows, (uint32_t)(r1 < nrows ? r1 : nrows), c);
     __syncthreads();
   }
-  if (edges && active) {
-    // the uint4s after the whole rows (fewer than a row)
-    for (uint64_t i = nrows * 64ull * kLoads + lane; i < n4; i += 64) count_u4<S>(c, body[i], j0);
-    // unaligned head and tail (< 16 bytes each); offsets from the
-    // element-aligned source start give the plane
-    if (lane < head && lane % S == j0) atomicAdd(c + p[lane] * kRep, 1u);
-    const uint64_t j = tail0 + lane;
-    if (lane < 16 && j < n && j % S == j0) atomicAdd(c + p[j] * kRep, 1u);
+  // (every thread has read the last grab before s_grab is reused)
+  __syncthreads();
+  if (edges) {
+    for (uint64_t i = nrows * kRowU4 + t; i < n4; i += kThreads) count_u4<S, R>(c, body[i]);
+    // head and tail bytes: plane plane0 + (offset from the element-aligned
+    // source start) % S
+    if (t < head) atomicAdd(c + (t % S) * 256u * R + p[t] * R, 1u);
+    const uint64_t j = tail0 + t;
+    if (t < 16 && j < n) atomicAdd(c + (uint32_t)(j % S) * 256u * R + p[j] * R, 1u);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void count_all(const HistArgs& a, uint32_t* cnt, uint32_t* s_grab) {
+  const uint32_t t = threadIdx.x;
+  for (int s = 0; s < a.nsrc; ++s) {
+    const HistSource& src = a.src[s];
+    const bool edges = blockIdx.x == 0;
+    uint32_t* const c = cnt + (uint32_t)src.plane0 * 256u * R + t % R;
+    if (src.stride == 1) count_source<1, R>(src, c, edges, a.grab + s, s_grab);
+    else if (src.stride == 2) count_source<2, R>(src, c, edges, a.grab + s, s_grab);
+    else count_source<4, R>(src, c, edges, a.grab + s, s_grab);
+  }
+  __syncthreads();
+  // merge: thread t sums bin t (t + kThreads, ...) of the P x 256, its
+  // replicas read from a rotated start so a pass spreads over the banks
+  for (uint32_t i = t; i < (uint32_t)a.num_planes * kNumSymbols; i += kThreads) {
+    const uint32_t* row = cnt + i * R;
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < (uint32_t)R; ++q) v += row[(q + i) % R];
+    if (v) atomicAdd(a.hist + i, v);
   }
 }
 
 __global__ __launch_bounds__(kThreads) void hist_kernel(HistArgs a) {
-  // 64 KiB: kWaves x [256][kRep] counters; the finisher reuses the front
+  // 64 KiB of counters; the finisher reuses the front
   extern __shared__ uint32_t smem[];
   uint32_t* s_keys = smem;                 // [256]  (finisher)
   uint32_t* s_tmp = smem + kNumSymbols;    // [kWaves]
-  int* s_last = (int*)(smem + kNumSymbols + kWaves);
-  // (the wave index through readfirstlane: plane, group and every row index
-  // derived from them stay in scalar registers)
-  const int t = threadIdx.x, lane = t & 63, w = (int)__builtin_amdgcn_readfirstlane((uint32_t)t >> 6);
+  __shared__ int s_last_flag;
+  int* s_last = &s_last_flag;
+  __shared__ uint32_t s_grab[2];
+  const int t = threadIdx.x;
   const int P = a.num_planes;
-  const int groups = kWaves / P;           // P divides kWaves (1, 2 or 4)
-  const int plane = w % P, group = w / P;
 
   // Clear the encode launch's look-back words (the previous user of this
   // workspace has finished: kernels on one stream run in order).
@@ -227,32 +251,10 @@ __global__ __launch_bounds__(kThreads) void hist_kernel(HistArgs a) {
        i += (uint64_t)gridDim.x * kThreads)
     a.status[i] = 0;
 
-  uint32_t* wc = smem + w * kCountWords;
-  for (int i = lane; i < kCountWords / 4; i += 64) ((uint4*)wc)[i] = make_uint4(0, 0, 0, 0);
-  __threadfence_block();
-  uint32_t* c = wc + (lane % kRep);
-  __shared__ uint32_t s_grab[2];
-  for (int s = 0; s < a.nsrc; ++s) {
-    const HistSource& src = a.src[s];
-    const bool active = plane >= (int)src.plane0 && plane < (int)(src.plane0 + src.stride);
-    const uint32_t j0 = active ? (uint32_t)plane - src.plane0 : 0u;
-    const bool edges = blockIdx.x == 0 && group == 0;
-    if (src.stride == 1) count_source<1>(src, c, j0, active, group, groups, edges, a.grab + s, s_grab);
-    else if (src.stride == 2) count_source<2>(src, c, j0, active, group, groups, edges, a.grab + s, s_grab);
-    else count_source<4>(src, c, j0, active, group, groups, edges, a.grab + s, s_grab);
-  }
-  __threadfence_block();
-  // merge the wave's replicas and add to the device histogram of its plane
-  for (int b = lane; b < (int)kNumSymbols; b += 64) {
-    const uint4* r4 = (const uint4*)(wc + b * kRep);
-    uint32_t v = 0;
-#pragma unroll
-    for (int k = 0; k < kRep / 4; ++k) {
-      const uint4 q = r4[k];
-      v += q.x + q.y + q.z + q.w;
-    }
-    if (v) atomicAdd(a.hist + plane * kNumSymbols + b, v);
-  }
+  for (int i = t; i < kCountWords / 4; i += kThreads) ((uint4*)smem)[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  if (P <= 2) count_all<32>(a, smem, s_grab);
+  else count_all<16>(a, smem, s_grab);
   __threadfence();
   __syncthreads();
   if (t == 0) *s_last = atomicAdd(a.done, 1u) == gridDim.x - 1;
@@ -315,11 +317,14 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
 hipError_t launch_hist(const HistArgs& a, int cu, hipStream_t s) {
   uint64_t n4 = 0;
   for (int i = 0; i < a.nsrc; ++i) n4 = n4 > a.src[i].nbytes / 16u ? n4 : a.src[i].nbytes / 16u;
-  const uint64_t groups = (uint64_t)kWaves / (uint64_t)a.num_planes;
-  const uint64_t want = div_up(n4, 64ull * kLoads * groups);
-  const uint64_t cap = (uint64_t)cu * 2u;  // two 64 KiB workgroups per CU
+  const size_t lds = (size_t)kCountWords * sizeof(uint32_t);
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hist_kernel, kThreads, lds);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  const uint64_t want = div_up(n4, (uint64_t)kRowU4 * kChunkRows);
+  const uint64_t cap = (uint64_t)cu * (uint64_t)per_cu;
   const uint32_t grid = (uint32_t)(want < 1 ? 1 : (want > cap ? cap : want));
-  const size_t lds = (size_t)kWaves * kCountWords * sizeof(uint32_t);
   hipLaunchKernelGGL(hist_kernel, dim3(grid), dim3(kThreads), lds, s, a);
   return hipGetLastError();
 }

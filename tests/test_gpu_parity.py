@@ -50,10 +50,10 @@ def datasets(seed=0):
 @pytest.mark.parametrize("stride", [1, 2, 4])
 def test_histogram(ws, stride):
     rng = np.random.default_rng(stride)
-    # (8 KiB rows of 512 uint4s are counted in chunks of 8 rows: sizes on,
-    # around and between those boundaries)
-    for n in [0, 1, 15, 16, 17, 1000, 8192 // stride, 65_536 // stride, 65_536 + 7, 8192 * 43 // stride,
-              1_000_003]:
+    # (a workgroup counts 32 KiB rows of 2048 uint4s, taken in chunks of 4
+    # rows: sizes on, around and between those boundaries)
+    for n in [0, 1, 15, 16, 17, 1000, 8192 // stride, 32_768 // stride, 32_768 // stride + 9, 65_536 + 7,
+              131_072 // stride, 131_072 * 3 // stride + 5, 8192 * 43 // stride, 1_000_003]:
         raw = rng.integers(0, 256, n * stride + 64).astype(np.uint8)
         raw[: n * stride: 3] = 7  # a hot bin
         for off in ([0, 4, 8] if stride > 1 else [0, 1, 5, 15]):
