@@ -513,42 +513,56 @@ __device__ uint64_t encode_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, ui
 }
 
 // ---- 32-lane (reference-format) encode ticket ------------------------------
-// Units u0 = 2 lt and u0 + 1 of buffer B in the 32-lane format (4096-symbol
-// blocks): lanes 0-31 code the first and lanes 32-63 the second, one chain
-// per lane, with the single-buffer 32-lane step (enc_step32f: one ballot,
-// words compacted per half).  Each unit has a linear LDS stage of H + 256
-// words (checked every 4 steps, at most 32 words per step) and a slot area of
-// CSTAGE / 2 words in the wave's spill area.  The pair's planes (both planes
-// of a uint16 block, two of a uint32 block, or two blocks of uint8 plane 0)
-// have their tables at s_tab[kp - p0].  Published and resolved as the
-// 64-lane tickets are.
+// The 32-lane format (4096-symbol blocks): a ticket is enc32_units(EB) =
+// 2 NC consecutive units u0 = 2 NC lt, ..., lanes 0-31 coding units u0 ..
+// u0 + NC - 1 and lanes 32-63 the next NC, NC chains per lane (unit
+// u0 + NC h + c is chain c of half h), with the single-buffer 32-lane step
+// (enc_step32f: one ballot per chain, words compacted per half).  NC = 2
+// for uint8 (two blocks per half) and uint16 (both planes of one block per
+// half: one element load feeds both chains); uint32 tickets are the two
+// units u0, u0 + 1 (planes p0, p0 + 1 of one block), one chain per lane,
+// since four consecutive uint32 units would need four planes' tables.  Each
+// unit has a linear LDS stage of H + 256 words (checked every 4 steps, at
+// most 32 words per step) and a slot area of CSTAGE / 2 words.  Published
+// and resolved as the 64-lane tickets are.
+__host__ __device__ constexpr uint32_t enc32_chains(uint32_t eb) { return eb == 4 ? 1u : 2u; }
+__host__ __device__ constexpr uint32_t enc32_units(uint32_t eb) { return 2u * enc32_chains(eb); }
+
 template <int PB, int EB, int KBB>
 __device__ uint64_t encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_t tix, const BatchEncodeArgs& a,
-                                uint32_t gen, uint32_t* wl, uint16_t* spill, uint64_t& tkey) {
+                                    uint32_t gen, uint32_t* wl, uint16_t* spill, uint64_t& tkey) {
   constexpr uint32_t L = 32, BLOCK = 4096;
+  constexpr int NC = (int)enc32_chains(EB);    // chains per lane
+  constexpr int NU = 2 * NC;                   // units per ticket
+  constexpr int NBK = EB == 1 ? NC : 1;        // element blocks per lane
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
   constexpr uint32_t SSTAGE = CSTAGE / 2;  // a 32-lane unit's words at most, rounded
   constexpr uint32_t H = 256, H4 = H / 8, REG = H + 256;
-  constexpr uint32_t NT = EB == 1 ? 1u : 2u;  // tables of the pair
+  constexpr uint32_t NT = EB == 1 ? 1u : 2u;  // tables of the ticket
   static_assert(BLOCK / L % KBB == 0 && KBB % 4 == 0, "KBB steps per prefetch batch, checks every 4");
-  static_assert(2 * REG + 128 <= 2 * (kRingWords + 64), "stages and dummy slots fit the wave's stage area");
-  static_assert(SSTAGE % 8 == 0 && 2 * SSTAGE <= 16 * CSTAGE, "slot areas");
+  static_assert(NU * REG + 128 <= 2 * (kRingWords + 64), "stages and dummy slots fit the wave's stage area");
+  static_assert(SSTAGE % 8 == 0 && NU * SSTAGE <= kMaxTicketUnits * CSTAGE && NU <= (int)kMaxTicketUnits, "slot areas");
   using T = typename ElemOf<EB>::T;
   const uint32_t lane = threadIdx.x & 63u, half = lane >> 5, lin = lane & 31u;
   const uint32_t P = EB;
   const uint64_t U = (uint64_t)P * B.nb;
-  const uint64_t u0 = 2ull * lt;
-  // this lane's unit (its half's)
-  const uint64_t u = u0 + half;
-  const bool act = u < U;
-  const uint32_t kp = act ? (uint32_t)(u % P) : 0u;
-  const uint64_t blk = act ? u / P : 0u;
-  const uint32_t n = act ? (uint32_t)((B.n - blk * BLOCK) < BLOCK ? (B.n - blk * BLOCK) : BLOCK) : 0u;
+  const uint64_t u0 = (uint64_t)NU * lt;
   const uint32_t p0 = (uint32_t)(u0 % P);
-  const bool act1 = u0 + 1 < U;  // the second unit exists (wave-uniform)
-  uint4* s_tab = (uint4*)wl;     // [NT][256]
+  // this lane's chains: unit, plane, block, symbols
+  bool act[NC];
+  uint32_t kp[NC], n[NC];
+  uint64_t blk[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const uint64_t u = u0 + (uint64_t)NC * half + c;
+    act[c] = u < U;
+    kp[c] = act[c] ? (uint32_t)(u % P) : p0;
+    blk[c] = act[c] ? u / P : 0u;
+    n[c] = act[c] ? (uint32_t)((B.n - blk[c] * BLOCK) < BLOCK ? (B.n - blk[c] * BLOCK) : BLOCK) : 0u;
+  }
+  uint4* s_tab = (uint4*)wl;  // [NT][256]
   uint16_t* wstage = (uint16_t*)(wl + kTabWords);
-  const uint32_t dummy_addr = lds_addr(wstage + 2 * REG + 2u * lane);
+  const uint32_t dummy_addr = lds_addr(wstage + NU * REG + 2u * lane);
   const uint64_t key = ((uint64_t)b << 4) | p0;
   if (key != tkey) {
     drain();  // the previous ticket's table reads are done
@@ -561,20 +575,30 @@ __device__ uint64_t encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, 
     }
     tkey = key;
   }
-  const uint4* tab = s_tab + (kp - p0) * 256u;
-  uint32_t x = kMinState;
-  // per unit h: LDS byte address of the stage and of the next word, words
-  // moved to the slot area
-  uint32_t cb[2], sb[2], fl[2];
+  const uint4* tab[NC];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    cb[h] = wave_u32(lds_addr(wstage + h * REG));
-    sb[h] = cb[h];
-    fl[h] = 0;
+  for (int c = 0; c < NC; ++c) tab[c] = s_tab + (kp[c] - p0) * 256u;
+  uint32_t x[NC];
+  // per unit j = NC h + c: LDS byte address of the stage and of the next
+  // word, words moved to the slot area
+  uint32_t cb[NU], sb[NU], fl[NU];
+#pragma unroll
+  for (int j = 0; j < NU; ++j) {
+    cb[j] = wave_u32(lds_addr(wstage + j * REG));
+    sb[j] = cb[j];
+    fl[j] = 0;
   }
-  const uint32_t n0 = __builtin_amdgcn_readlane(n, 0), n1 = __builtin_amdgcn_readlane(n, 32);
-  const uint32_t n_min = act1 ? (n0 < n1 ? n0 : n1) : 0u;
-  const uint32_t n_max = n0 > n1 ? n0 : n1;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) x[c] = kMinState;
+  uint32_t n_min = BLOCK, n_max = 0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const uint32_t a0 = __builtin_amdgcn_readlane(n[c], 0), a1 = __builtin_amdgcn_readlane(n[c], 32);
+    n_min = a0 < n_min ? a0 : n_min;
+    n_min = a1 < n_min ? a1 : n_min;
+    n_max = a0 > n_max ? a0 : n_max;
+    n_max = a1 > n_max ? a1 : n_max;
+  }
   const uint32_t all = (uint32_t)div_up(n_max, L);
   drain();
 
@@ -582,36 +606,47 @@ __device__ uint64_t encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, 
   // area and the rest (fewer than 256) to the front
   auto spill_full = [&]() {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t un = wave_u32((sb[h] - cb[h]) >> 1);  // unspilled words
+    for (int j = 0; j < NU; ++j) {
+      const uint32_t un = wave_u32((sb[j] - cb[j]) >> 1);  // unspilled words
       if (un >= H) {
-        uint16_t* st = wstage + h * REG;
-        if (lane < H4) ((uint4*)(spill + h * SSTAGE + fl[h]))[lane] = ((const uint4*)st)[lane];
+        uint16_t* st = wstage + j * REG;
+        if (lane < H4) ((uint4*)(spill + j * SSTAGE + fl[j]))[lane] = ((const uint4*)st)[lane];
         if (4u * lane < un - H) ((uint2*)st)[lane] = ((const uint2*)(st + H))[lane];
-        fl[h] += H;
-        sb[h] = wave_u32(sb[h] - 2u * H);
+        fl[j] += H;
+        sb[j] = wave_u32(sb[j] - 2u * H);
       }
     }
   };
 
-  const T* ep = (const T*)B.in + blk * BLOCK;
+  // block d of this lane (uint8: chain d's block; otherwise the half's
+  // block, chain c taking byte kp[c] of each element)
+  const T* ep[NBK];
+#pragma unroll
+  for (int d = 0; d < NBK; ++d) ep[d] = (const T*)B.in + blk[d] * BLOCK;
+  auto shift_of = [&](int c) -> uint32_t { return EB == 1 ? 0u : 8u * kp[c]; };
   uint32_t s = 0;
   if (n_min == BLOCK) {
-    auto load = [&](uint32_t (&buf)[KBB], uint32_t s0) {
-      const gptr<T> bp = (gptr<T>)ep + lin + (uint64_t)s0 * L;
+    auto load = [&](uint32_t (&buf)[NBK][KBB], uint32_t s0) {
 #pragma unroll
-      for (int k = 0; k < KBB; ++k) buf[k] = bp[k * L];
+      for (int d = 0; d < NBK; ++d) {
+        const gptr<T> bp = (gptr<T>)ep[d] + lin + (uint64_t)s0 * L;
+#pragma unroll
+        for (int k = 0; k < KBB; ++k) buf[d][k] = bp[k * L];
+      }
     };
-    auto code = [&](const uint32_t (&buf)[KBB]) {
+    auto code = [&](const uint32_t (&buf)[NBK][KBB]) {
 #pragma unroll
       for (int k = 0; k < KBB; ++k) {
-        const uint4 e = tab[(buf[k] >> (8u * kp)) & 0xffu];
-        enc_step32f(x, sb[0], sb[1], half, e);
+        uint4 e[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) e[c] = tab[c][(buf[EB == 1 ? c : 0][k] >> shift_of(c)) & 0xffu];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) enc_step32f(x[c], sb[c], sb[NC + c], half, e[c]);
         if ((k & 3) == 3) spill_full();
       }
     };
     const uint32_t last = BLOCK / L - KBB;
-    uint32_t bA[KBB], bB[KBB];
+    uint32_t bA[NBK][KBB], bB[NBK][KBB];
     load(bA, 0);
     for (;;) {
       load(bB, s + KBB <= last ? s + KBB : last);
@@ -626,41 +661,53 @@ __device__ uint64_t encode_ticket32(const BatchBuf& B, uint32_t b, uint32_t lt, 
   }
   for (; s < all; ++s) {
     const uint32_t i = s * L + lin;
-    const bool v = i < n;
-    const uint32_t w = v ? (uint32_t)ep[i] : 0u;
-    enc_step32(x, sb[0], sb[1], half, dummy_addr, tab[(w >> (8u * kp)) & 0xffu], v);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const bool v = i < n[c];
+      const uint32_t w = v ? (uint32_t)ep[EB == 1 ? c : 0][i] : 0u;
+      enc_step32(x[c], sb[c], sb[NC + c], half, dummy_addr, tab[c][(w >> shift_of(c)) & 0xffu], v);
+    }
     spill_full();
   }
 
   // pad each unit, its stage remainder to its slot area, states, block sizes
-  uint32_t pad[2], cnt[2];
+  uint32_t pad[NU], cnt[NU];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const uint32_t rem = (sb[h] - cb[h]) >> 1;
-    const bool acth = h ? act1 : true;
-    cnt[h] = fl[h] + rem;
-    pad[h] = acth ? (uint32_t)round_up(cnt[h], 8) : 0u;
-    if (cnt[h] > SSTAGE && lane == 0) atomicOr(a.err, kErrStageOverflow);
-    uint16_t* st = wstage + h * REG;
-    if (lane < pad[h] - cnt[h]) st[rem + lane] = 0;
-    const uint32_t nr = pad[h] > fl[h] ? (pad[h] - fl[h]) / 8 : 0u;
-    for (uint32_t k = lane; k < nr; k += 64) ((uint4*)(spill + h * SSTAGE + fl[h]))[k] = ((const uint4*)st)[k];
+  for (int j = 0; j < NU; ++j) {
+    const uint32_t rem = (sb[j] - cb[j]) >> 1;
+    const bool actj = u0 + (uint64_t)j < U;
+    cnt[j] = fl[j] + rem;
+    pad[j] = actj ? (uint32_t)round_up(cnt[j], 8) : 0u;
+    if (cnt[j] > SSTAGE && lane == 0) atomicOr(a.err, kErrStageOverflow);
+    uint16_t* st = wstage + j * REG;
+    if (lane < pad[j] - cnt[j]) st[rem + lane] = 0;
+    const uint32_t nr = pad[j] > fl[j] ? (pad[j] - fl[j]) / 8 : 0u;
+    for (uint32_t k = lane; k < nr; k += 64) ((uint4*)(spill + j * SSTAGE + fl[j]))[k] = ((const uint4*)st)[k];
   }
-  if (act) {
-    uint8_t* sec = B.out + B.section[kp] + kSectionFixed;
-    ((uint32_t*)sec)[blk * L + lin] = x;
-    if (lin == 0) {
-      const uint32_t c = half ? cnt[1] : cnt[0];
-      ((uint32_t*)(sec + 4ull * L * B.nb + 8ull * blk))[0] = (n << 16) | c;
-      atomicAdd(a.ptot + 4ull * b + kp, half ? pad[1] : pad[0]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (act[c]) {
+      uint8_t* sec = B.out + B.section[kp[c]] + kSectionFixed;
+      ((uint32_t*)sec)[blk[c] * L + lin] = x[c];
+      if (lin == 0) {
+        const uint32_t cu = half ? cnt[NC + c] : cnt[c];
+        const uint32_t pu = half ? pad[NC + c] : pad[c];
+        ((uint32_t*)(sec + 4ull * L * B.nb + 8ull * blk[c]))[0] = (n[c] << 16) | cu;
+        atomicAdd(a.ptot + 4ull * b + kp[c], pu);
+      }
     }
   }
 
   // the units' padded sizes to the slot's tail, the aggregate to the
   // ticket's status word
-  const uint64_t agg = (uint64_t)pad[0] + pad[1];
-  if (lane < kMaxTicketUnits)
-    ((uint32_t*)(spill + kMaxTicketUnits * CSTAGE))[lane] = lane == 0 ? pad[0] : (lane == 1 ? pad[1] : 0u);
+  uint64_t agg = 0;
+  uint32_t pv = 0;
+#pragma unroll
+  for (int j = 0; j < NU; ++j) {
+    agg += pad[j];
+    pv = lane == (uint32_t)j ? pad[j] : pv;
+  }
+  if (lane < kMaxTicketUnits) ((uint32_t*)(spill + kMaxTicketUnits * CSTAGE))[lane] = pv;
   publish_agg_gen(a.status, tix, tix - lt, agg, gen, (int)lane);
   return agg;
 }
@@ -676,11 +723,11 @@ __device__ void place_ticket(const BatchBuf& B, uint32_t b, uint32_t lt, uint64_
                              const BatchEncodeArgs& a, const uint16_t* slot) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
   constexpr uint32_t AREA = L == 32 ? CSTAGE / 2 : CSTAGE;  // a unit's slot area
-  constexpr int UT = L == 32 ? 2 : (int)kMaxTicketUnits;     // units per ticket, at most
+  constexpr int UT = L == 32 ? 4 : (int)kMaxTicketUnits;     // units per ticket, at most
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t P = B.eb;
   const uint64_t U = (uint64_t)P * B.nb;
-  const uint32_t ut = L == 32 ? 2u : enc_ticket_units(P, a.rounds);
+  const uint32_t ut = L == 32 ? enc32_units(P) : enc_ticket_units(P, a.rounds);
   const uint64_t u0 = (uint64_t)ut * lt;
   // lane j < UT: unit j's padded words and the first uint4 of its words in
   // the ticket's output (exclusive scan over the units)
@@ -863,10 +910,11 @@ hipError_t launch_enc_shape(const BatchEncodeArgs& a, int cu, hipStream_t s, uin
       return hipSuccess;
     }
     if (spill_bytes > a.spill_bytes) return hipErrorInvalidValue;
-    // half the workgroups start on histogram chunks (at least one when there
-    // are any: the tickets wait for them)
+    // a quarter of the workgroups start on histogram chunks (at least one
+    // when there are any: the tickets wait for them).  (4096 buffers of
+    // 8.5 GB: 9.53 ms with a quarter, 9.65 with half or all of them.)
     BatchEncodeArgs g = a;
-    g.hist_wgs = a.nhist_items == 0 ? 0u : (uint32_t)(grid > 1 ? grid / 2 : 1);
+    g.hist_wgs = a.nhist_items == 0 ? 0u : (uint32_t)(grid > 3 ? grid / 4 : 1);
     if (a.lanes == 32) hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB, 32>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, g);
     else hipLaunchKernelGGL((batch_encode_kernel<PB, NW, KB, 64>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, g);
     return hipGetLastError();

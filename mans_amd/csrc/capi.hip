@@ -1121,8 +1121,9 @@ BatchGrain grain_of(const mans_buffer* h_bufs, int32_t count, int cu, int lanes)
 
 // Lays out every buffer of a batch as the byte-plane frame of
 // mans_frame_encode in p->lanes' format and counts its histogram items
-// (g.hist_chunk bytes) and tickets (decode and 32-lane encode: two units;
-// 64-lane encode: g.rounds rounds of enc_chains units).
+// (g.hist_chunk bytes) and tickets (decode: two units; 32-lane encode:
+// four units, two for uint32; 64-lane encode: g.rounds rounds of
+// enc_chains units).
 mans_status plan_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t count, bool decode,
                        const BatchGrain& g, std::vector<BatchBuf>& bufs, uint64_t& nhist, uint64_t& nticks) {
   const Format f = format_for_lanes(p->lanes);
@@ -1169,7 +1170,10 @@ mans_status plan_batch(const mans_params* p, const mans_buffer* h_bufs, int32_t 
     B.tick_first = (uint32_t)nticks;
     // decode tickets: two units; encode tickets: g.rounds rounds of four
     // chains (two for uint32), as enc_ticket_units in batch_kernels.hpp
-    B.nticks = (uint32_t)div_up(eb * nb, decode || f.lanes == 32 ? 2u : g.rounds * (eb == 4 ? 2u : 4u));
+    // units per ticket: decode 2; 32-lane encode 4 (uint32: 2); 64-lane
+    // encode rounds x enc_chains
+    const uint32_t per = decode ? 2u : (f.lanes == 32 ? (eb == 4 ? 2u : 4u) : g.rounds * (eb == 4 ? 2u : 4u));
+    B.nticks = (uint32_t)div_up(eb * nb, per);
     B.lanes = f.lanes;
     B.in_bytes = decode ? m.in_bytes : 0;
     frame_header(B.frame_hdr, m.n_elems, eb, MANS_MODE_PLANES, f, p->prob_bits, d, 0, 0);
