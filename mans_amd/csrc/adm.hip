@@ -11,9 +11,10 @@
 // lane l's bytes go to 32 O_g + l M_g with O = exclusive prefix of M.
 //
 // Structure: a wavefront takes 16 groups per ticket, two at a time (one
-// per half-wave).  Pass A writes codes and centers and finds each group's
-// byte length; after the look-back, pass B rebuilds the bits (lane bit
-// buffers in LDS, ds_or per element) and writes them at their offsets.
+// per half-wave).  Pass A writes codes and centers, finds each group's byte
+// length and publishes the ticket's sum; after pass A of the wave's next
+// ticket, the look-back gives the offset and pass B rebuilds the bits (lane
+// bit buffers in LDS, ds_or per element) and writes them at their offsets.
 // O_g is resolved by decoupled look-back over tickets (any number of groups;
 // the reference's fused path is limited to 1024 groups,
 // mappingV15_fused_uint16.cu:851), the whole wave reading 512 predecessors
@@ -102,68 +103,43 @@ __device__ __forceinline__ uint32_t map_lane(const uint32_t (&v)[kAdmChunk], uin
   return bo;
 }
 
+// A wavefront resolves a ticket's offset only after pass A of its next
+// ticket (the byte lengths of two tickets in LDS, used in turn): by then the
+// predecessors have had a whole pass A to publish their lengths, so the
+// look-back sums published entries instead of waiting for the slowest wave
+// still in pass A.  A wave waits only on lower tickets, each of which
+// publishes its aggregate right after its pass A, so the launch drains.
 __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
   __shared__ uint32_t s_bits[kWaves][64][kLaneWords];
-  __shared__ uint32_t s_m[kWaves][2 * kPairs];
+  __shared__ uint32_t s_m[kWaves][2][2 * kPairs];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t half = (uint32_t)lane >> 5, lin = (uint32_t)lane & 31u;
   uint32_t* mybits = s_bits[wave][lane];
   const uint8_t* wavebytes = (const uint8_t*)s_bits[wave];
-  uint32_t* wm = s_m[wave];
   constexpr uint32_t kTicketGroups = 2 * kPairs;
 
-  for (;;) {
-    uint32_t tk = 0;
-    if (lane == 0) tk = atomicAdd(a.ticket, kTicketGroups);
-    tk = __builtin_amdgcn_readfirstlane(tk);
-    if ((uint64_t)tk >= a.gsize) break;
+  // the ticket whose pass A is done and whose offset is not yet resolved
+  // (wave-uniform), its byte lengths in s_m[wave][slot ^ 1]; lane g < 16:
+  // group g's exclusive byte offset within it
+  bool pend = false;
+  uint32_t p_tk = 0, p_exlo = 0, slot = 0;
+  uint64_t p_agg = 0;
 
-    // ---- phase A: codes, centers and each group's byte length M_g --------
-    bool over = false;
-    for (uint32_t pr = 0; pr < kPairs; ++pr) {
-      const uint64_t g = (uint64_t)tk + 2 * pr + half;
-      uint32_t v[kAdmChunk], cnt, codes[4];
-      uint64_t base;
-      const uint32_t c = load_lane(a, g, lin, v, cnt, base);
-      const uint32_t bo = map_lane<false>(v, cnt, c, codes, nullptr);
-      if (bo > 8 * kAdmLaneBytes) over = true;
-      uint32_t m = half_max(cnt ? (bo + 7u) / 8u : 0u);
-      if (m > kAdmLaneBytes) m = kAdmLaneBytes;
-      if (cnt == kAdmChunk) {
-        *(uint4*)(a.codes + base) = make_uint4(codes[0], codes[1], codes[2], codes[3]);
-      } else {
-        for (uint32_t i = 0; i < cnt; ++i) a.codes[base + i] = (uint8_t)(codes[i >> 2] >> (8 * (i & 3)));
-      }
-      if (g < a.gsize && lin == 0) a.centers[g] = (uint16_t)c;
-      if (lin == 0) wm[2 * pr + half] = m;
-    }
-    if (__ballot(over) && lane == 0) atomicOr(a.err, kErrAdmDomain);
-    __threadfence_block();
-    // the ticket's groups: exclusive byte offsets within it, and their sum
-    uint32_t mine = (uint32_t)lane < kTicketGroups ? wm[lane] : 0u;
-    uint32_t incl = mine;
-#pragma unroll
-    for (int o = 1; o < (int)kTicketGroups; o <<= 1) {
-      const uint32_t u = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += u;
-    }
-    const uint64_t agg = __builtin_amdgcn_readlane(incl, kTicketGroups - 1);
-
-    // ---- look-back for the ticket's first offset (bytes per lane) ---------
-    const uint64_t chain = tk / kTicketGroups;
-    uint64_t excl = look_back(a.status, chain, agg, lane);
+  // offsets and pass B of the pending ticket: each pair's bit plane, lane
+  // l's m bytes at 32 og + l m
+  auto finish_pending = [&]() {
+    const uint32_t* wm = s_m[wave][slot ^ 1u];
+    const uint64_t chain = p_tk / kTicketGroups;
+    uint64_t excl = resolve_prefix(a.status, chain, p_agg, lane);
     excl = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(excl >> 32)) << 32) |
            (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)excl);
     {
-      const uint64_t g = (uint64_t)tk + lane;
-      if ((uint32_t)lane < kTicketGroups && g < a.gsize) a.offsets[g] = (int32_t)(excl + incl - mine);
+      const uint64_t g = (uint64_t)p_tk + lane;
+      if ((uint32_t)lane < kTicketGroups && g < a.gsize) a.offsets[g] = (int32_t)(excl + p_exlo);
     }
-    if ((uint64_t)tk + kTicketGroups >= a.gsize && lane == 0) a.offsets[a.gsize] = (int32_t)(excl + agg);
-    const uint32_t ex_lo = incl - mine;  // lane g's exclusive offset within the ticket
-
-    // ---- phase B: each pair's bit plane, lane l's m bytes at 32 og + l m ---
+    if ((uint64_t)p_tk + kTicketGroups >= a.gsize && lane == 0) a.offsets[a.gsize] = (int32_t)(excl + p_agg);
     for (uint32_t pr = 0; pr < kPairs; ++pr) {
-      const uint64_t g = (uint64_t)tk + 2 * pr + half;
+      const uint64_t g = (uint64_t)p_tk + 2 * pr + half;
       uint32_t v[kAdmChunk], cnt, codes[4];
       uint64_t base;
       const uint32_t c = load_lane(a, g, lin, v, cnt, base);
@@ -177,7 +153,7 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
         atomicOr(mybits + (bo >> 5), pad << (((bo >> 3) & 3u) * 8u));
       }
       __threadfence_block();
-      const uint32_t og_lo = __shfl(ex_lo, (int)(2 * pr), 64), og_hi = __shfl(ex_lo, (int)(2 * pr + 1), 64);
+      const uint32_t og_lo = __shfl(p_exlo, (int)(2 * pr), 64), og_hi = __shfl(p_exlo, (int)(2 * pr + 1), 64);
       const uint64_t og = excl + (half ? og_hi : og_lo);
       if (g < a.gsize && m > 0) {
         const uint64_t dst0 = 32ull * og;
@@ -196,6 +172,57 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
       }
       __threadfence_block();
     }
+  };
+
+  for (;;) {
+    uint32_t tk = 0;
+    if (lane == 0) tk = atomicAdd(a.ticket, kTicketGroups);
+    tk = __builtin_amdgcn_readfirstlane(tk);
+    const bool more = (uint64_t)tk < a.gsize;
+    uint32_t exlo = 0;
+    uint64_t agg = 0;
+    if (more) {
+      // ---- pass A: codes, centers and each group's byte length M_g -------
+      uint32_t* wm = s_m[wave][slot];
+      bool over = false;
+      for (uint32_t pr = 0; pr < kPairs; ++pr) {
+        const uint64_t g = (uint64_t)tk + 2 * pr + half;
+        uint32_t v[kAdmChunk], cnt, codes[4];
+        uint64_t base;
+        const uint32_t c = load_lane(a, g, lin, v, cnt, base);
+        const uint32_t bo = map_lane<false>(v, cnt, c, codes, nullptr);
+        if (bo > 8 * kAdmLaneBytes) over = true;
+        uint32_t m = half_max(cnt ? (bo + 7u) / 8u : 0u);
+        if (m > kAdmLaneBytes) m = kAdmLaneBytes;
+        if (cnt == kAdmChunk) {
+          *(uint4*)(a.codes + base) = make_uint4(codes[0], codes[1], codes[2], codes[3]);
+        } else {
+          for (uint32_t i = 0; i < cnt; ++i) a.codes[base + i] = (uint8_t)(codes[i >> 2] >> (8 * (i & 3)));
+        }
+        if (g < a.gsize && lin == 0) a.centers[g] = (uint16_t)c;
+        if (lin == 0) wm[2 * pr + half] = m;
+      }
+      if (__ballot(over) && lane == 0) atomicOr(a.err, kErrAdmDomain);
+      __threadfence_block();
+      // the ticket's groups: exclusive byte offsets within it, and their sum
+      const uint32_t mine = (uint32_t)lane < kTicketGroups ? wm[lane] : 0u;
+      uint32_t incl = mine;
+#pragma unroll
+      for (int o = 1; o < (int)kTicketGroups; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += u;
+      }
+      agg = __builtin_amdgcn_readlane(incl, kTicketGroups - 1);
+      exlo = incl - mine;
+      publish_aggregate(a.status, tk / kTicketGroups, agg, lane);
+    }
+    if (pend) finish_pending();
+    if (!more) break;
+    pend = true;
+    p_tk = tk;
+    p_exlo = exlo;
+    p_agg = agg;
+    slot ^= 1u;
   }
 }
 
