@@ -10,7 +10,7 @@
 // 0xFF >> (L % 8)), lanes without elements stay zero (mappingcpuV1.cpp:146);
 // lane l's bytes go to 32 O_g + l M_g with O = exclusive prefix of M.
 //
-// Structure: a wavefront takes 16 groups per ticket, two at a time (one
+// Structure: a wavefront takes 32 groups per ticket, two at a time (one
 // per half-wave).  Pass A writes codes and centers, finds each group's byte
 // length and publishes the ticket's sum; after pass A of the wave's next
 // ticket, the look-back gives the offset and pass B rebuilds the bits (lane
@@ -42,39 +42,58 @@ __device__ __forceinline__ uint32_t half_max(uint32_t v) {
   return v;
 }
 
-// Group pairs per ticket.  A ticket of 16 groups (8192 elements) keeps the
-// look-back chain short: the inclusive prefix moves back at most 512
-// tickets per round trip, so 2-group tickets (256K per 512 MiB) made the
-// scan, not the data, the bound.
+// Group pairs per ticket.  A ticket of 32 groups (16384 elements) keeps the
+// look-back chain short and the ticket counter's traffic low: 2-group
+// tickets (256K per 512 MiB) made the scan, not the data, the bound, and on
+// 512 MiB 16-group tickets took 0.55 ms, 32 0.50 ms, 64 0.53 ms.
 constexpr uint32_t kPairs = kAdmTicketGroups / 2;
 // Lane bit buffers are padded to 9 words so the 64 lanes' words of one
 // index fall in distinct LDS banks.
 constexpr uint32_t kLaneWords = kAdmLaneBytes / 4 + 1;
 
-// The 16 elements of one lane (half-wave = one group): loads them, and
-// returns the group's center (all lanes of the half-wave) and the count.
-__device__ __forceinline__ uint32_t load_lane(const AdmMapArgs& a, uint64_t g, uint32_t lin, uint32_t (&v)[kAdmChunk],
-                                              uint32_t& cnt, uint64_t& base) {
+// The 16 elements of one lane (half-wave = one group): raw_lane loads them
+// (two 16-byte loads for a whole lane, element loads for the last, partial
+// lane of the input); unpack_center spreads them and returns the group's
+// center (all lanes of the half-wave).  Split so a pair's loads can be in
+// flight while the previous pair is mapped.
+struct LaneRaw {
+  uint4 q0, q1;
+  uint32_t cnt;
+  uint64_t base;
+};
+__device__ __forceinline__ LaneRaw raw_lane(const AdmMapArgs& a, uint64_t g, uint32_t lin) {
+  LaneRaw r;
   const bool gactive = g < a.gsize;
-  base = g * kAdmGroup + (uint64_t)lin * kAdmChunk;
-  cnt = gactive && base < a.n ? (uint32_t)((a.n - base) < kAdmChunk ? (a.n - base) : kAdmChunk) : 0u;
-  if (cnt == kAdmChunk) {
-    const uint4* p = (const uint4*)(a.in + base);
-    const uint4 q0 = p[0], q1 = p[1];
-    const uint32_t w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+  r.base = g * kAdmGroup + (uint64_t)lin * kAdmChunk;
+  r.cnt = gactive && r.base < a.n ? (uint32_t)((a.n - r.base) < kAdmChunk ? (a.n - r.base) : kAdmChunk) : 0u;
+  if (r.cnt == kAdmChunk) {
+    const uint4* p = (const uint4*)(a.in + r.base);
+    r.q0 = p[0];
+    r.q1 = p[1];
+  } else {
+    uint32_t w[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      v[2 * i] = w[i] & 0xffffu;
-      v[2 * i + 1] = w[i] >> 16;
+      const uint32_t lo = (uint32_t)(2 * i) < r.cnt ? (uint32_t)a.in[r.base + 2 * i] : 0u;
+      const uint32_t hi = (uint32_t)(2 * i + 1) < r.cnt ? (uint32_t)a.in[r.base + 2 * i + 1] : 0u;
+      w[i] = lo | (hi << 16);
     }
-  } else {
+    r.q0 = make_uint4(w[0], w[1], w[2], w[3]);
+    r.q1 = make_uint4(w[4], w[5], w[6], w[7]);
+  }
+  return r;
+}
+__device__ __forceinline__ uint32_t unpack_center(const LaneRaw& r, uint32_t (&v)[kAdmChunk]) {
+  const uint32_t w[8] = {r.q0.x, r.q0.y, r.q0.z, r.q0.w, r.q1.x, r.q1.y, r.q1.z, r.q1.w};
 #pragma unroll
-    for (int i = 0; i < (int)kAdmChunk; ++i) v[i] = (uint32_t)i < cnt ? (uint32_t)a.in[base + i] : 0u;
+  for (int i = 0; i < 8; ++i) {
+    v[2 * i] = w[i] & 0xffffu;
+    v[2 * i + 1] = w[i] >> 16;
   }
   uint32_t lsum = 0;
 #pragma unroll
-  for (int i = 0; i < (int)kAdmChunk; ++i) lsum += (uint32_t)i < cnt ? v[i] : 0u;
-  const uint32_t gsum = half_sum(lsum), gcnt = half_sum(cnt);
+  for (int i = 0; i < (int)kAdmChunk; ++i) lsum += (uint32_t)i < r.cnt ? v[i] : 0u;
+  const uint32_t gsum = half_sum(lsum), gcnt = half_sum(r.cnt);
   return gcnt ? gsum / gcnt : 0u;  // mappingcpuV1.cpp:60-66
 }
 
@@ -88,17 +107,19 @@ __device__ __forceinline__ uint32_t map_lane(const uint32_t (&v)[kAdmChunk], uin
   uint32_t bo = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) codes[i] = 0;
+  // (branch-free over the elements: an element past cnt adds no code byte
+  // and no length)
 #pragma unroll
   for (int i = 0; i < (int)kAdmChunk; ++i) {
-    if ((uint32_t)i < cnt) {
-      const uint32_t x = v[i];
-      const uint32_t d = x > c ? x - c : c - x;
-      const uint32_t len = x == c ? 1u : (d + kAdmMod - 1) / kAdmMod;
-      const uint32_t code = x == c ? 1u : ((d - kAdmMod * (len - 1)) * 2u + (x > c ? 0u : 1u));
-      codes[i >> 2] |= (code & 0xffu) << (8 * (i & 3));
-      if (PACK && bo < 8 * kAdmLaneBytes) atomicOr(bits + (bo >> 5), 1u << (((bo >> 3) & 3u) * 8u + 7u - (bo & 7u)));
-      bo += len;
-    }
+    const bool ok = (uint32_t)i < cnt;
+    const uint32_t x = v[i];
+    const uint32_t d = x > c ? x - c : c - x;
+    const uint32_t len = x == c ? 1u : (d + kAdmMod - 1) / kAdmMod;
+    const uint32_t code = x == c ? 1u : ((d - kAdmMod * (len - 1)) * 2u + (x > c ? 0u : 1u));
+    codes[i >> 2] |= (ok ? code & 0xffu : 0u) << (8 * (i & 3));
+    if (PACK && ok && bo < 8 * kAdmLaneBytes)
+      atomicOr(bits + (bo >> 5), 1u << (((bo >> 3) & 3u) * 8u + 7u - (bo & 7u)));
+    bo += ok ? len : 0u;
   }
   return bo;
 }
@@ -138,11 +159,14 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
       if ((uint32_t)lane < kTicketGroups && g < a.gsize) a.offsets[g] = (int32_t)(excl + p_exlo);
     }
     if ((uint64_t)p_tk + kTicketGroups >= a.gsize && lane == 0) a.offsets[a.gsize] = (int32_t)(excl + p_agg);
+    LaneRaw nx = raw_lane(a, (uint64_t)p_tk + half, lin);
     for (uint32_t pr = 0; pr < kPairs; ++pr) {
       const uint64_t g = (uint64_t)p_tk + 2 * pr + half;
-      uint32_t v[kAdmChunk], cnt, codes[4];
-      uint64_t base;
-      const uint32_t c = load_lane(a, g, lin, v, cnt, base);
+      const LaneRaw cur = nx;
+      if (pr + 1 < kPairs) nx = raw_lane(a, g + 2, lin);  // the next pair's loads in flight
+      uint32_t v[kAdmChunk], codes[4];
+      const uint32_t cnt = cur.cnt;
+      const uint32_t c = unpack_center(cur, v);
 #pragma unroll
       for (int i = 0; i < (int)kLaneWords; ++i) mybits[i] = 0;
       __threadfence_block();
@@ -185,11 +209,15 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
       // ---- pass A: codes, centers and each group's byte length M_g -------
       uint32_t* wm = s_m[wave][slot];
       bool over = false;
+      LaneRaw nx = raw_lane(a, (uint64_t)tk + half, lin);
       for (uint32_t pr = 0; pr < kPairs; ++pr) {
         const uint64_t g = (uint64_t)tk + 2 * pr + half;
-        uint32_t v[kAdmChunk], cnt, codes[4];
-        uint64_t base;
-        const uint32_t c = load_lane(a, g, lin, v, cnt, base);
+        const LaneRaw cur = nx;
+        if (pr + 1 < kPairs) nx = raw_lane(a, g + 2, lin);  // the next pair's loads in flight
+        uint32_t v[kAdmChunk], codes[4];
+        const uint32_t cnt = cur.cnt;
+        const uint64_t base = cur.base;
+        const uint32_t c = unpack_center(cur, v);
         const uint32_t bo = map_lane<false>(v, cnt, c, codes, nullptr);
         if (bo > 8 * kAdmLaneBytes) over = true;
         uint32_t m = half_max(cnt ? (bo + 7u) / 8u : 0u);
