@@ -193,6 +193,19 @@ mans_status mans_frame_encode(const mans_params* p, const void* d_in, uint64_t n
  * asynchronous call, or NULL to have it read from d_in (synchronising). */
 mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header, void* d_out,
                               uint64_t out_capacity, uint64_t* h_n_elems, mans_workspace* ws, void* stream);
+/* mans_frame_decode with the ADM restore fused into the decode: for a 64-lane
+ * ADM frame the bit plane is decoded first, then one launch decodes the code
+ * plane and restores each block's elements from codes kept in LDS, so the
+ * codes never reach memory and no unmap launch runs: the reference's
+ * restore kernel (`decompress`, mapping/nv/mappingV15_fused_uint16.cu:24-121)
+ * moved inside the ANS decode.  Same output and errors as
+ * mans_frame_decode; other frames decode exactly as there.  On MI355X the
+ * two-launch path of mans_frame_decode measures faster (the restore work
+ * inside the latency-bound decode waves costs more than the code bytes it
+ * saves), so this entry point is for callers that want one launch per plane
+ * and no code scratch. */
+mans_status mans_frame_decode_fused(const void* d_in, uint64_t c, const void* h_header, void* d_out,
+                                    uint64_t out_capacity, uint64_t* h_n_elems, mans_workspace* ws, void* stream);
 /* Parses a frame header already on the host. */
 mans_status mans_frame_info(const void* h_header, uint64_t header_bytes, uint64_t* n_elems,
                             int32_t* elem_bytes, int32_t* mode, uint64_t* total_size);

@@ -87,6 +87,7 @@ SIGNATURES = {
     "mans_frame_max_size": (_u64, [_u64, _P]),
     "mans_frame_encode": (C.c_int, [_P, _vp, _u64, _vp, _u64, _vp, C.POINTER(_u64), _vp, _vp]),
     "mans_frame_decode": (C.c_int, [_vp, _u64, _vp, _vp, _u64, C.POINTER(_u64), _vp, _vp]),
+    "mans_frame_decode_fused": (C.c_int, [_vp, _u64, _vp, _vp, _u64, C.POINTER(_u64), _vp, _vp]),
     "mans_frame_info": (C.c_int, [_vp, _u64, C.POINTER(_u64), C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_u64)]),
     "mans_batch_encode_create": (C.c_int, [C.POINTER(_vp), _P, _B, _i32, _vp]),
     "mans_batch_encode_run": (C.c_int, [_vp, _vp, _vp]),
@@ -382,17 +383,20 @@ def frame_encode(x, p: Params, ws: Workspace, out=None, stream=None, sync_size: 
 
 
 def frame_decode(comp, size: int, dtype, n: int, ws: Workspace, out=None, stream=None,
-                 header: bytes | None = None):
+                 header: bytes | None = None, fused: bool = False):
     """Decode a MANS frame.  With `header` (the frame's first 128 bytes on the
-    host) the call does not synchronise."""
+    host) the call does not synchronise.  `fused`: mans_frame_decode_fused
+    (64-lane ADM frames restore their elements inside the code plane's decode
+    launch; other frames decode as usual)."""
     torch = _torch()
     if out is None:
         out = torch.empty(max(n, 8), dtype=dtype, device=comp.device)
     nout = C.c_uint64(0)
     hbuf = (C.c_uint8 * 128).from_buffer_copy(header[:128]) if header is not None else None
-    rc = lib().mans_frame_decode(comp.data_ptr(), size, hbuf, out.data_ptr(), out.numel() * out.element_size(),
-                                 C.byref(nout), ws.handle, _stream(stream))
-    _check(rc, "mans_frame_decode")
+    fn = lib().mans_frame_decode_fused if fused else lib().mans_frame_decode
+    rc = fn(comp.data_ptr(), size, hbuf, out.data_ptr(), out.numel() * out.element_size(), C.byref(nout), ws.handle,
+            _stream(stream))
+    _check(rc, "mans_frame_decode_fused" if fused else "mans_frame_decode")
     return out[:n]
 
 

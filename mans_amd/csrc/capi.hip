@@ -930,8 +930,9 @@ mans_status mans_frame_info(const void* h_header, uint64_t header_bytes, uint64_
   return MANS_OK;
 }
 
-mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header, void* d_out, uint64_t out_capacity,
-                              uint64_t* h_n_elems, mans_workspace* ws, void* stream) {
+static mans_status frame_decode_impl(const void* d_in, uint64_t c, const void* h_header, void* d_out,
+                                     uint64_t out_capacity, uint64_t* h_n_elems, mans_workspace* ws, void* stream,
+                                     bool fuse) {
   if (!ws || !d_in || c < kFrameHeaderBytes) return MANS_ERR_INVALID_ARG;
   if (!aligned(d_in, 16)) return MANS_ERR_ALIGNMENT;
   DeviceGuard g(ws->device);
@@ -981,6 +982,47 @@ mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header
   uint64_t units = 0;
   uint8_t* codes = nullptr;
   uint8_t* bits = nullptr;
+  if (mode == MANS_MODE_ADM && fuse && lanes == 64) {
+    // the bit plane alone, then the code plane with the restore fused in
+    // (no code scratch); the checks are those of the two-launch path
+    if (nsym[0] != n || nsym[1] > adm_bits_cap(n)) return MANS_ERR_CORRUPT;
+    if (meta + adm_meta_bytes(gsize) > c) return MANS_ERR_CORRUPT;
+    for (uint32_t k = 0; k < 2; ++k) {
+      if (sect[k] + kSectionFixed > c || nsym[k] > 0xffffffffull) return MANS_ERR_CORRUPT;
+      if (sect[k] + section_bytes(div_up(nsym[k], f.block_bytes), f) > data_off) return MANS_ERR_CORRUPT;
+    }
+    mans_status st = ensure_scratch(ws, round_up(nsym[1], 16) + 16);
+    if (st != MANS_OK) return st;
+    bits = ws->scratch;
+    a.d.num_planes = 1;
+    a.d.order = 0;
+    a.d.data_off = data_off;
+    a.in = (const uint8_t*)d_in;
+    a.in_bytes = c;
+    a.data_words = data_words;
+    a.err = ws->counters + 3;
+    a.from_header = 0;
+    const uint64_t nb1 = div_up(nsym[1], f.block_bytes), nb0 = div_up(nsym[0], f.block_bytes);
+    a.d.plane[0] = PlaneDesc{nullptr, bits, nsym[1], nb1, sect[1], 0, 1, 0};
+    a.d.total_units = nb1;
+    prof_mark(ws, s, "decode");
+    HIP_TRY(launch_decode(a, ws->cu, s));
+    a.d.plane[0] = PlaneDesc{nullptr, nullptr, nsym[0], nb0, sect[0], 0, 1, 0};
+    a.d.total_units = nb0;
+    a.adm.offsets = (const int32_t*)((const uint8_t*)d_in + meta);
+    a.adm.centers = (const uint16_t*)((const uint8_t*)d_in + meta + 4ull * (gsize + 1));
+    a.adm.codes = nullptr;
+    a.adm.bits = bits;
+    a.adm.n = n;
+    a.adm.gsize = gsize;
+    a.adm.bits_len = nsym[1];
+    a.adm.out = (uint16_t*)d_out;
+    a.adm.err = ws->counters + 3;
+    prof_mark(ws, s, "decode_restore");
+    HIP_TRY(launch_decode_fused(a, ws->cu, s));
+    prof_mark(ws, s, nullptr);
+    return MANS_OK;
+  }
   if (mode == MANS_MODE_ADM) {
     if (nsym[0] != n || nsym[1] > adm_bits_cap(n)) return MANS_ERR_CORRUPT;
     const uint64_t codes_bytes = round_up(n, 16);
@@ -1026,6 +1068,16 @@ mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header
   }
   prof_mark(ws, s, nullptr);
   return MANS_OK;
+}
+
+mans_status mans_frame_decode(const void* d_in, uint64_t c, const void* h_header, void* d_out, uint64_t out_capacity,
+                              uint64_t* h_n_elems, mans_workspace* ws, void* stream) {
+  return frame_decode_impl(d_in, c, h_header, d_out, out_capacity, h_n_elems, ws, stream, false);
+}
+
+mans_status mans_frame_decode_fused(const void* d_in, uint64_t c, const void* h_header, void* d_out,
+                                    uint64_t out_capacity, uint64_t* h_n_elems, mans_workspace* ws, void* stream) {
+  return frame_decode_impl(d_in, c, h_header, d_out, out_capacity, h_n_elems, ws, stream, true);
 }
 
 // ---- batches -----------------------------------------------------------------

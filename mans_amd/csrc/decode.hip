@@ -26,6 +26,7 @@
 // slot - cdf; x = pdf * (x >> p) + slot - cdf; lanes with x < 2^15 read a word from the
 // end of the block's remaining words, the highest such lane taking the last
 // word; x = (x << 16) + word.
+#include "adm_device.hpp"
 #include "codec_device.hpp"
 
 namespace mans {
@@ -57,7 +58,12 @@ struct DPlane {
 // (CH / S consecutive element blocks, 64-lane format only: chain d * S + b
 // decodes byte b of block d's elements), NW wavefronts per group, RB: log2 of
 // the ring in words (64-lane format), 0 = whole-unit staging.
-template <int L, int PB, int S, int CH, int NW, int RB>
+// FUSE (64-lane ADM frames, one plane, one chain): each block's codes go to
+// the wave's own BLOCK bytes of LDS instead of memory, and once the block is
+// decoded the wave restores its elements (ADM lanes of 16 codes) from the
+// codes, the already decoded bit plane, the offsets and the centers
+// (adm_restore_lane), so the codes never leave the CU.
+template <int L, int PB, int S, int CH, int NW, int RB, bool FUSE = false>
 __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
   using T = typename ElemOf<S>::T;
   constexpr int BPT = CH / S;  // element blocks per wave iteration
@@ -80,6 +86,8 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
   static_assert(CH % S == 0 && (L == 64 || RING || CH == S) && CH * 8 <= 32,
                 "chain d * S + b decodes byte b of block d");
   static_assert(!RING || (H >= 128 && H4 <= 64), "ring: 128 <= R/2 <= 512 words");
+  static_assert(!FUSE || (L == 64 && S == 1 && CH == 1 && RING), "fused ADM restore: 64-lane code plane");
+  constexpr uint32_t WAVE_U16 = (RING ? NSTR : CH) * REG + (RING ? 0u : 64u);  // stage u16 per wave
   extern __shared__ uint4 smem[];
   __shared__ DPlane s_pl[kMaxPlanes];
   __shared__ uint64_t s_ubase[kMaxPlanes + 1];
@@ -201,8 +209,11 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
   // with whole-unit staging 64 words of slack (a lane that takes no word may
   // read one slot past its unit's words)
   const uint32_t wave_u = __builtin_amdgcn_readfirstlane((uint32_t)wave);
-  uint16_t* wstage =
-      (uint16_t*)(smem + div_up((uint64_t)P * W * sizeof(LutEntry), 16u)) + wave_u * ((RING ? NSTR : CH) * REG + (RING ? 0u : 64u));
+  uint16_t* const stage0 = (uint16_t*)(smem + div_up((uint64_t)P * W * sizeof(LutEntry), 16u));
+  uint16_t* wstage = stage0 + wave_u * WAVE_U16;
+  // FUSE: this wave's block of codes and 16 bytes of signal scratch per lane
+  uint8_t* const wcodes = (uint8_t*)(stage0 + NW * WAVE_U16) + wave_u * (BLOCK + 64u * kAdmChunk);
+  uint8_t* const wsig = wcodes + BLOCK + (uint32_t)lane * kAdmChunk;
   const uint64_t total_units = s_total_units;
   const uint64_t data_off = s_data_off, data_words = s_data_words;
   const uint64_t nwaves = (uint64_t)gridDim.x * NW;
@@ -377,9 +388,13 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
       ep[d] = (__attribute__((address_space(1))) T*)dstp[d * S] + blk[d * S] * BLOCK + lin;
     // block d's element of step s: bytes d * S .. d * S + S - 1 of el
     auto put = [&](uint32_t s, uint32_t el, const bool (&v)[CH]) {
+      if constexpr (FUSE) {
+        if (v[0]) wcodes[s * L + lin] = (uint8_t)el;
+      } else {
 #pragma unroll
-      for (int d = 0; d < BPT; ++d)
-        if (v[d * S]) ep[d][(uint64_t)s * L] = (T)(el >> (8 * S * d));
+        for (int d = 0; d < BPT; ++d)
+          if (v[d * S]) ep[d][(uint64_t)s * L] = (T)(el >> (8 * S * d));
+      }
     };
     const uint32_t full = n_min / L;
     const uint32_t all = (uint32_t)div_up(n_max, L);
@@ -413,8 +428,12 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
 #pragma unroll
       for (int k = (int)RK - 1; k >= 0; --k) {
         const uint32_t el = dec_steps<L, PB, CH, MASK, LutEntry, !RING>(x, ptr, taken, lut, stage, vall, half);
+        if constexpr (FUSE) {
+          wcodes[(s + (uint32_t)k) * L + lin] = (uint8_t)el;
+        } else {
 #pragma unroll
-        for (int d = 0; d < BPT; ++d) gp[d][k * L] = (T)(el >> (8 * S * d));
+          for (int d = 0; d < BPT; ++d) gp[d][k * L] = (T)(el >> (8 * S * d));
+        }
       }
       refill();
     }
@@ -424,20 +443,40 @@ __global__ __launch_bounds__(64 * NW) void decode_kernel(DecodeArgs a) {
       const bool bad = act[c] && ((RING ? ptr[c] != 0u : taken[c] != wds[c]) || x[c] != kMinState);
       if (__ballot(bad) && lin == 0 && act[c]) atomicOr(a.err, kErrFinalState);
     }
+    if constexpr (FUSE) {
+      // the block's elements: ADM lanes li0 .. li0 + BLOCK / 16 - 1 (this
+      // lane: every 64th), codes from the wave's LDS (written by other lanes)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (act[0]) {
+        const uint64_t li0 = blk[0] * (BLOCK / kAdmChunk);
+        for (uint32_t k = lin; k < BLOCK / kAdmChunk; k += 64u) {
+          const uint64_t base = (li0 + k) * kAdmChunk;
+          if (base >= a.adm.n) break;
+          const uint32_t cnt = (uint32_t)((a.adm.n - base) < kAdmChunk ? (a.adm.n - base) : kAdmChunk);
+          const uint4 q = *(const uint4*)(wcodes + k * kAdmChunk);
+          const uint32_t cw[4] = {q.x, q.y, q.z, q.w};
+          adm_restore_lane(a.adm, li0 + k, cnt, cw, wsig);
+        }
+      }
+      // (the next block's codes overwrite these)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
     __threadfence_block();
   }
 }
 
-template <int L, int PB, int S, int CH, int NW, int RB>
+template <int L, int PB, int S, int CH, int NW, int RB, bool FUSE = false>
 hipError_t launch_t(const DecodeArgs& a, int cu, hipStream_t s) {
   constexpr uint32_t CSTAGE = chain_stage_words<PB>();
   constexpr uint32_t REG = RB > 0 ? (1u << RB) + 64u : CSTAGE;
   constexpr uint32_t NSTR = CH * (L == 32 ? 2u : 1u);
+  constexpr uint32_t BLOCK = L == 32 ? 4096u : 8192u;
   const size_t lds = (size_t)round_up((uint64_t)a.d.num_planes * (1u << PB) * sizeof(LutEntryT<L>), 16u) +
-                     (size_t)NW * ((RB > 0 ? NSTR : CH) * REG + (RB > 0 ? 0u : 64u)) * 2u;
+                     (size_t)NW * ((RB > 0 ? NSTR : CH) * REG + (RB > 0 ? 0u : 64u)) * 2u +
+                     (FUSE ? (size_t)NW * (BLOCK + 64u * kAdmChunk) : 0u);
   int per_cu = 0;
   hipError_t e =
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decode_kernel<L, PB, S, CH, NW, RB>, 64 * NW, lds);
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decode_kernel<L, PB, S, CH, NW, RB, FUSE>, 64 * NW, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   uint64_t grid = (uint64_t)cu * (uint64_t)per_cu;
@@ -447,7 +486,7 @@ hipError_t launch_t(const DecodeArgs& a, int cu, hipStream_t s) {
     if (want < grid) grid = want;
   }
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((decode_kernel<L, PB, S, CH, NW, RB>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
+  hipLaunchKernelGGL((decode_kernel<L, PB, S, CH, NW, RB, FUSE>), dim3((uint32_t)grid), dim3(64 * NW), lds, s, a);
   return hipGetLastError();
 }
 
@@ -489,6 +528,21 @@ hipError_t launch_p(const DecodeArgs& a, int cu, hipStream_t s) {
 hipError_t launch_decode(const DecodeArgs& a, int cu, hipStream_t s) {
   if (!a.from_header && a.d.total_units == 0) return hipSuccess;
   return a.d.lanes == 64 ? launch_p<64>(a, cu, s) : launch_p<32>(a, cu, s);
+}
+
+hipError_t launch_decode_fused(const DecodeArgs& a, int cu, hipStream_t s) {
+  if (a.d.total_units == 0) return hipSuccess;
+  if (a.d.lanes != 64 || a.d.num_planes != 1 || a.d.order != 0 || a.d.plane[0].stride != 1 || a.from_header)
+    return hipErrorInvalidValue;
+  // four waves per workgroup, each holding 9 KiB of codes and signal
+  // scratch and a 512-word ring (on 512 MiB of detector codes: 0.54 ms;
+  // 1024-word rings 0.71 ms; two waves per workgroup 0.99 ms)
+  switch (a.d.prob_bits) {
+    case 9: return launch_t<64, 9, 1, 1, 4, 9, true>(a, cu, s);
+    case 10: return launch_t<64, 10, 1, 1, 4, 9, true>(a, cu, s);
+    case 11: return launch_t<64, 11, 1, 1, 4, 9, true>(a, cu, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace mans

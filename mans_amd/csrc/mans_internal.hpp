@@ -163,6 +163,18 @@ struct EncodeArgs {
   uint64_t spill_bytes;          // their capacity
 };
 
+struct AdmUnmapArgs {
+  const int32_t* offsets;
+  const uint16_t* centers;
+  const uint8_t* codes;
+  const uint8_t* bits;
+  uint64_t n;
+  uint64_t gsize;
+  uint64_t bits_len;     // bytes in the bit plane
+  uint16_t* out;
+  uint32_t* err;         // kErrBlockWords on inconsistent offsets
+};
+
 struct DecodeArgs {
   CodecDesc d;                   // when from_header: plane 0 section/dst/stride only
   const uint8_t* in;
@@ -171,6 +183,7 @@ struct DecodeArgs {
   uint64_t data_words;           // frames: from the header; single stream: 0 -> header
   uint32_t* err;
   int32_t from_header;           // single stream: nb / nsym / version read on device
+  AdmUnmapArgs adm;              // fused ADM restore (launch_decode_fused): plane 0 is the code plane
 };
 
 struct AdmMapArgs {
@@ -187,17 +200,6 @@ struct AdmMapArgs {
   uint32_t* err;         // kErrAdmDomain / kErrCapacity
 };
 
-struct AdmUnmapArgs {
-  const int32_t* offsets;
-  const uint16_t* centers;
-  const uint8_t* codes;
-  const uint8_t* bits;
-  uint64_t n;
-  uint64_t gsize;
-  uint64_t bits_len;     // bytes in the bit plane
-  uint16_t* out;
-  uint32_t* err;         // kErrBlockWords on inconsistent offsets
-};
 
 // Launchers (one per translation unit).  Each returns hipSuccess or the
 // launch error.  `cu` is the device's compute-unit count.
@@ -367,6 +369,9 @@ hipError_t launch_encode(const EncodeArgs& a, int cu, hipStream_t s);
 // bytes of spill area launch_encode needs for a (0 for the 32-lane format)
 hipError_t encode_spill_bytes(const EncodeArgs& a, int cu, uint64_t* bytes);
 hipError_t launch_decode(const DecodeArgs& a, int cu, hipStream_t s);
+// 64-lane ADM frames: decodes plane 0 (the code plane) and restores each
+// block's elements from a.adm (bit plane already decoded) in the same wave.
+hipError_t launch_decode_fused(const DecodeArgs& a, int cu, hipStream_t s);
 hipError_t launch_adm_map(const AdmMapArgs& a, int cu, hipStream_t s);
 hipError_t launch_adm_unmap(const AdmUnmapArgs& a, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);

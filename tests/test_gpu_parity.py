@@ -352,6 +352,51 @@ def test_frame_adm_rejects_bad_offsets(ws):
         ws.check()
 
 
+@pytest.mark.parametrize("p", [9, 10, 11])
+def test_frame_adm_fused_decode(ws, p):
+    # mans_frame_decode_fused: the restore inside the code plane's decode
+    # launch gives the elements the oracle's frame holds, on ragged sizes
+    # (a partial last lane, group and block) and on wide signals
+    rng = np.random.default_rng(80 + p)
+    for n, spread in [(1, 4), (513, 4), (8191, 4), (8192, 4), (8193, 30), (5 * 8192 + 777, 90), (300_001, 4)]:
+        x = np.clip(32768 + np.round(rng.laplace(0, spread, n)), 0, 65535).astype(np.uint16)
+        print(f"frame adm fused decode n={n} p={p}", flush=True)
+        out, size = mans_amd.frame_encode(u16_dev(x), mans_amd.params(p, 64, 2, mans_amd.MODE_ADM), ws)
+        ref = oracle.frame_encode(x, oracle.MODE_ADM, p, FMTS[64])
+        assert size == ref.size and (host(out[:size]) == ref).all()
+        back = host(mans_amd.frame_decode(out, size, torch.int16, n, ws, fused=True)).view(np.uint16)
+        ws.check()
+        assert (back == x).all(), (n, np.flatnonzero(back != x)[:8])
+
+
+def test_frame_fused_decode_other_frames(ws):
+    # byte-plane frames and 32-lane ADM frames take the ordinary path
+    rng = np.random.default_rng(84)
+    x = np.clip(32768 + np.round(rng.laplace(0, 4, 70_001)), 0, 65535).astype(np.uint16)
+    for lanes, mode in ((64, mans_amd.MODE_PLANES), (32, mans_amd.MODE_ADM), (32, mans_amd.MODE_PLANES)):
+        print(f"fused decode entry, lanes={lanes} mode={mode}", flush=True)
+        out, size = mans_amd.frame_encode(u16_dev(x), mans_amd.params(10, lanes, 2, mode), ws)
+        back = host(mans_amd.frame_decode(out, size, torch.int16, x.size, ws, fused=True)).view(np.uint16)
+        ws.check()
+        assert (back == x).all()
+
+
+def test_frame_adm_fused_rejects_bad_offsets(ws):
+    rng = np.random.default_rng(63)
+    n = 3 * 8192
+    x = np.clip(32768 + np.round(rng.laplace(0, 4, n)), 0, 65535).astype(np.uint16)
+    out, size = mans_amd.frame_encode(u16_dev(x), mans_amd.params(10, 64, 2, mans_amd.MODE_ADM), ws)
+    frame = host(out[:size]).copy()
+    meta = int(frame[32:40].view(np.uint64)[0])
+    offs = frame[meta:meta + 8].view(np.int32)
+    offs[1] = offs[0] + 1000  # a group longer than 32 bytes per lane
+    print("fused decode of a frame with a corrupt offset table", flush=True)
+    bad = torch.from_numpy(frame).cuda()
+    mans_amd.frame_decode(bad, size, torch.int16, n, ws, fused=True)
+    with pytest.raises(mans_amd.MansError):
+        ws.check()
+
+
 def test_frame_auto_falls_back_outside_domain(ws):
     rng = np.random.default_rng(51)
     x = np.clip(32768 + np.round(rng.laplace(0, 4, 50_000)), 0, 65535).astype(np.uint16)
