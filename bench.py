@@ -845,8 +845,16 @@ def configs_bench(args) -> int:
 
                 def check():
                     return bool(torch.equal(back, x))
+            timed = [("encode", enc), ("decode", dec)]
+            if kind == "adm":
+                def dec_fused():
+                    mans_amd.frame_decode(out, csize, x.dtype, n, ws, out=back, header=header, fused=True)
+                timed.append(("decode_fused", dec_fused))
             times = {}
-            for fn_name, fn in (("encode", enc), ("decode", dec)):
+            oks = []
+            for fn_name, fn in timed:
+                if fn_name.startswith("decode"):
+                    back.zero_()  # a decode that wrote nothing fails the check
                 for _ in range(args.warmup):
                     fn()
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -857,7 +865,9 @@ def configs_bench(args) -> int:
                 e1.record(stream)
                 torch.cuda.synchronize()
                 times[fn_name] = e0.elapsed_time(e1) / args.steps
-            ok = check()
+                if fn_name.startswith("decode"):
+                    oks.append(check())
+            ok = all(oks)
             ws.check()
             row.update({
                 "compressed_bytes": int(csize),
@@ -868,6 +878,11 @@ def configs_bench(args) -> int:
                 "gpu_decode_ms": round(times["decode"], 4),
                 "bit_exact": ok,
             })
+            if "decode_fused" in times:
+                # mans_frame_decode_fused (the restore inside the code plane's
+                # decode launch), checked like the default decode
+                row["gpu_decode_fused_GBps"] = round(nbytes / (times["decode_fused"] * 1e-3) / 1e9, 1)
+                row["gpu_decode_fused_ms"] = round(times["decode_fused"], 4)
             # CPU port of the same form on a sample of the same data
             for threads, mib in ((cores, 32), (1, 8)):
                 m = min(n, (mib << 20) // eb)
