@@ -47,6 +47,7 @@ __device__ __forceinline__ uint32_t half_max(uint32_t v) {
 // tickets (256K per 512 MiB) made the scan, not the data, the bound, and on
 // 512 MiB 16-group tickets took 0.55 ms, 32 0.50 ms, 64 0.53 ms.
 constexpr uint32_t kPairs = kAdmTicketGroups / 2;
+static_assert(kPairs % 2 == 0, "pairs are mapped two at a time");
 // Lane bit buffers are padded to 9 words so the 64 lanes' words of one
 // index fall in distinct LDS banks.
 constexpr uint32_t kLaneWords = kAdmLaneBytes / 4 + 1;
@@ -61,30 +62,45 @@ struct LaneRaw {
   uint32_t cnt;
   uint64_t base;
 };
+// FULL: every lane of the ticket is whole (tickets before the input's end):
+// no bounds, no branch around the loads.
+template <bool FULL>
 __device__ __forceinline__ LaneRaw raw_lane(const AdmMapArgs& a, uint64_t g, uint32_t lin) {
   LaneRaw r;
-  const bool gactive = g < a.gsize;
   r.base = g * kAdmGroup + (uint64_t)lin * kAdmChunk;
+  if constexpr (FULL) {
+    r.cnt = kAdmChunk;
+    const uint4* p = (const uint4*)(a.in + r.base);
+    r.q0 = p[0];
+    r.q1 = p[1];
+    return r;
+  }
+  const bool gactive = g < a.gsize;
   r.cnt = gactive && r.base < a.n ? (uint32_t)((a.n - r.base) < kAdmChunk ? (a.n - r.base) : kAdmChunk) : 0u;
+  // (a partial lane, only at the end of the input, is loaded element by
+  // element in unpack_center: no element loads here, so nothing makes the
+  // wave wait for these loads before their pair is mapped)
   if (r.cnt == kAdmChunk) {
     const uint4* p = (const uint4*)(a.in + r.base);
     r.q0 = p[0];
     r.q1 = p[1];
   } else {
-    uint32_t w[8];
+    r.q0 = make_uint4(0, 0, 0, 0);
+    r.q1 = make_uint4(0, 0, 0, 0);
+  }
+  return r;
+}
+template <bool FULL>
+__device__ __forceinline__ uint32_t unpack_center(const AdmMapArgs& a, const LaneRaw& r, uint32_t (&v)[kAdmChunk]) {
+  uint32_t w[8] = {r.q0.x, r.q0.y, r.q0.z, r.q0.w, r.q1.x, r.q1.y, r.q1.z, r.q1.w};
+  if (!FULL && r.cnt > 0 && r.cnt < kAdmChunk) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const uint32_t lo = (uint32_t)(2 * i) < r.cnt ? (uint32_t)a.in[r.base + 2 * i] : 0u;
       const uint32_t hi = (uint32_t)(2 * i + 1) < r.cnt ? (uint32_t)a.in[r.base + 2 * i + 1] : 0u;
       w[i] = lo | (hi << 16);
     }
-    r.q0 = make_uint4(w[0], w[1], w[2], w[3]);
-    r.q1 = make_uint4(w[4], w[5], w[6], w[7]);
   }
-  return r;
-}
-__device__ __forceinline__ uint32_t unpack_center(const LaneRaw& r, uint32_t (&v)[kAdmChunk]) {
-  const uint32_t w[8] = {r.q0.x, r.q0.y, r.q0.z, r.q0.w, r.q1.x, r.q1.y, r.q1.z, r.q1.w};
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     v[2 * i] = w[i] & 0xffffu;
@@ -159,14 +175,12 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
       if ((uint32_t)lane < kTicketGroups && g < a.gsize) a.offsets[g] = (int32_t)(excl + p_exlo);
     }
     if ((uint64_t)p_tk + kTicketGroups >= a.gsize && lane == 0) a.offsets[a.gsize] = (int32_t)(excl + p_agg);
-    LaneRaw nx = raw_lane(a, (uint64_t)p_tk + half, lin);
-    for (uint32_t pr = 0; pr < kPairs; ++pr) {
+    auto pair_b = [&](uint32_t pr, const LaneRaw& cur, auto full) {
+      constexpr bool FULL = decltype(full)::value;
       const uint64_t g = (uint64_t)p_tk + 2 * pr + half;
-      const LaneRaw cur = nx;
-      if (pr + 1 < kPairs) nx = raw_lane(a, g + 2, lin);  // the next pair's loads in flight
       uint32_t v[kAdmChunk], codes[4];
       const uint32_t cnt = cur.cnt;
-      const uint32_t c = unpack_center(cur, v);
+      const uint32_t c = unpack_center<FULL>(a, cur, v);
 #pragma unroll
       for (int i = 0; i < (int)kLaneWords; ++i) mybits[i] = 0;
       __threadfence_block();
@@ -195,7 +209,21 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
         }
       }
       __threadfence_block();
-    }
+    };
+    // two buffers used in turn (no register copy between them, so the next
+    // pair's loads stay in flight while a pair is mapped)
+    auto pass_b = [&](auto full) {
+      constexpr bool FULL = decltype(full)::value;
+      LaneRaw ra = raw_lane<FULL>(a, (uint64_t)p_tk + half, lin), rb;
+      for (uint32_t pr = 0; pr < kPairs; pr += 2) {
+        rb = raw_lane<FULL>(a, (uint64_t)p_tk + 2 * (pr + 1) + half, lin);
+        pair_b(pr, ra, full);
+        if (pr + 2 < kPairs) ra = raw_lane<FULL>(a, (uint64_t)p_tk + 2 * (pr + 2) + half, lin);
+        pair_b(pr + 1, rb, full);
+      }
+    };
+    if ((uint64_t)(p_tk + kTicketGroups) * kAdmGroup <= a.n) pass_b(std::true_type{});
+    else pass_b(std::false_type{});
   };
 
   for (;;) {
@@ -209,15 +237,13 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
       // ---- pass A: codes, centers and each group's byte length M_g -------
       uint32_t* wm = s_m[wave][slot];
       bool over = false;
-      LaneRaw nx = raw_lane(a, (uint64_t)tk + half, lin);
-      for (uint32_t pr = 0; pr < kPairs; ++pr) {
+      auto pair_a = [&](uint32_t pr, const LaneRaw& cur, auto full) {
+        constexpr bool FULL = decltype(full)::value;
         const uint64_t g = (uint64_t)tk + 2 * pr + half;
-        const LaneRaw cur = nx;
-        if (pr + 1 < kPairs) nx = raw_lane(a, g + 2, lin);  // the next pair's loads in flight
         uint32_t v[kAdmChunk], codes[4];
         const uint32_t cnt = cur.cnt;
         const uint64_t base = cur.base;
-        const uint32_t c = unpack_center(cur, v);
+        const uint32_t c = unpack_center<FULL>(a, cur, v);
         const uint32_t bo = map_lane<false>(v, cnt, c, codes, nullptr);
         if (bo > 8 * kAdmLaneBytes) over = true;
         uint32_t m = half_max(cnt ? (bo + 7u) / 8u : 0u);
@@ -229,7 +255,20 @@ __global__ __launch_bounds__(kThreads) void adm_map_kernel(AdmMapArgs a) {
         }
         if (g < a.gsize && lin == 0) a.centers[g] = (uint16_t)c;
         if (lin == 0) wm[2 * pr + half] = m;
-      }
+      };
+      auto pass_a = [&](auto full) {
+        constexpr bool FULL = decltype(full)::value;
+        LaneRaw ra = raw_lane<FULL>(a, (uint64_t)tk + half, lin), rb;
+        for (uint32_t pr = 0; pr < kPairs; pr += 2) {
+          rb = raw_lane<FULL>(a, (uint64_t)tk + 2 * (pr + 1) + half, lin);
+          pair_a(pr, ra, full);
+          if (pr + 2 < kPairs) ra = raw_lane<FULL>(a, (uint64_t)tk + 2 * (pr + 2) + half, lin);
+          pair_a(pr + 1, rb, full);
+        }
+      };
+      // (tickets before the input's end: whole lanes only)
+      if ((uint64_t)(tk + kTicketGroups) * kAdmGroup <= a.n) pass_a(std::true_type{});
+      else pass_a(std::false_type{});
       if (__ballot(over) && lane == 0) atomicOr(a.err, kErrAdmDomain);
       __threadfence_block();
       // the ticket's groups: exclusive byte offsets within it, and their sum
