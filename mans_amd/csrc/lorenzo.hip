@@ -17,7 +17,8 @@
 // x 16 planes (tiles taken by an atomic ticket, in ticket order).  A thread
 // keeps its column's L2 of the previous plane for its 16 rows in registers,
 // so each element's f is read once (plus one halo row and the plane below
-// the tile); q(z, y, x - 1) comes from the neighbouring lane.  Outliers are
+// the tile), a plane's 17 rows loaded at once; q(z, y, x - 1) comes from the
+// neighbouring lane.  Outliers are
 // compacted in tile order (tile ticket, thread, plane, row) with a decoupled
 // look-back over tiles; a thread holds up to kHold of them in registers, and
 // a tile with more is rewritten by the fix-up launch.
@@ -61,52 +62,55 @@ __device__ void code_tile(const LorenzoArgs& a, uint32_t x0, uint32_t y0, uint32
   const bool xin = x < a.nx;
   const uint64_t plane = (uint64_t)a.ny * a.nx;
   const float inv = a.inv2eb;
-  // q at (z, y, x) and (z, y, x - 1); rows or planes outside the field are 0
-  auto qpair = [&](uint32_t z, int64_t y, int32_t& qc, int32_t& ql) {
-    qc = 0;
-    if (y >= 0 && xin) qc = quant(a.field[z * plane + (uint64_t)y * a.nx + x], inv, bad);
-    int32_t nb = __shfl_up(qc, 1, 64);
-    if (lane == 0) {
-      nb = 0;
-      if (y >= 0 && x > 0 && x - 1 < a.nx) nb = quant(a.field[z * plane + (uint64_t)y * a.nx + x - 1], inv, bad);
-    }
-    ql = nb;
-  };
   const uint32_t ny1 = a.ny - y0 < kTY ? a.ny - y0 : kTY;
   const uint32_t nz1 = a.nz - z0 < kTZ ? a.nz - z0 : kTZ;
+  // q of plane z at rows y0 - 1 .. y0 + kTY - 1 (row j - 1 of the tile in
+  // slot j; rows outside the field or the tile are 0) for this column and,
+  // on lane 0, the column to its left (the other lanes take it from their
+  // neighbour).  All of a plane's loads are issued before any is used.
+  auto plane_q = [&](uint32_t z, int32_t (&qc)[kTY + 1], int32_t (&ql)[kTY + 1]) {
+    float fc[kTY + 1], fl[kTY + 1];
+#pragma unroll
+    for (uint32_t j = 0; j <= kTY; ++j) {
+      const int64_t y = (int64_t)y0 - 1 + (int64_t)j;
+      const bool row = y >= 0 && (j == 0 || j - 1 < ny1);
+      const uint64_t r = row ? z * plane + (uint64_t)y * a.nx : 0u;
+      fc[j] = row && xin ? a.field[r + x] : 0.0f;
+      fl[j] = 0.0f;
+      if (lane == 0 && row && x > 0 && x - 1 < a.nx) fl[j] = a.field[r + x - 1];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j <= kTY; ++j) {
+      const int64_t y = (int64_t)y0 - 1 + (int64_t)j;
+      const bool row = y >= 0 && (j == 0 || j - 1 < ny1);
+      qc[j] = row && xin ? quant(fc[j], inv, bad) : 0;
+      int32_t nb = __shfl_up(qc[j], 1, 64);
+      if (lane == 0) nb = row && x > 0 && x - 1 < a.nx ? quant(fl[j], inv, bad) : 0;
+      ql[j] = nb;
+    }
+  };
   int32_t prev[kTY];
 #pragma unroll
   for (uint32_t j = 0; j < kTY; ++j) prev[j] = 0;
   // the plane below the tile: its L2 per row
   if (z0 > 0) {
-    int32_t qu, qul;
-    qpair(z0 - 1, (int64_t)y0 - 1, qu, qul);
+    int32_t qc[kTY + 1], ql[kTY + 1];
+    plane_q(z0 - 1, qc, ql);
 #pragma unroll
-    for (uint32_t j = 0; j < kTY; ++j) {
-      if (j < ny1) {
-        int32_t qc, ql;
-        qpair(z0 - 1, y0 + j, qc, ql);
-        prev[j] = qc - qu - ql + qul;
-        qu = qc;
-        qul = ql;
-      }
-    }
+    for (uint32_t j = 0; j < kTY; ++j)
+      if (j < ny1) prev[j] = qc[j + 1] - qc[j] - ql[j + 1] + ql[j];
   }
   for (uint32_t dz = 0; dz < nz1; ++dz) {
     const uint32_t z = z0 + dz;
-    int32_t qu, qul;
-    qpair(z, (int64_t)y0 - 1, qu, qul);
+    int32_t qc[kTY + 1], ql[kTY + 1];
+    plane_q(z, qc, ql);
 #pragma unroll
     for (uint32_t j = 0; j < kTY; ++j) {
       if (j < ny1) {
         const uint32_t y = y0 + j;
-        int32_t qc, ql;
-        qpair(z, y, qc, ql);
-        const int32_t l2 = qc - qu - ql + qul;
+        const int32_t l2 = qc[j + 1] - qc[j] - ql[j + 1] + ql[j];
         const int32_t r = l2 - prev[j];
         prev[j] = l2;
-        qu = qc;
-        qul = ql;
         if (xin) {
           const uint64_t i = z * plane + (uint64_t)y * a.nx + x;
           const bool in = r > -(int32_t)a.radius && r < (int32_t)a.radius;
