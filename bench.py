@@ -18,7 +18,9 @@ port of the same frame codec (oracle/, OpenMP) on a sample of the same codes.
 per buffer; `served` is the request server under open-loop Poisson arrivals
 (2000 req/s, 64 KiB - 256 MiB requests, 200 GB budget: p50 / p99 latency and
 GB/s) against the plain codec under the same arrivals, plus a burst for
-capacity.
+capacity.  `front_end` is the Lorenzo front end that makes the codes:
+forward (field -> codes) and inverse (codes -> field) launches on the same
+field.
 """
 from __future__ import annotations
 
@@ -57,6 +59,7 @@ def parse():
     ap.add_argument("--serve-pool", type=int, default=256, help="served workload: buffers requests draw data from")
     ap.add_argument("--serve-burst", type=int, default=2048, help="served workload: requests of the capacity burst")
     ap.add_argument("--serve-budget-gb", type=int, default=200, help="server device-memory budget (GB)")
+    ap.add_argument("--no-front-end", action="store_true", help="skip the Lorenzo front-end leg")
     ap.add_argument("--dump", default=None,
                     help="write the generated codes to this file and exit (input of mans_profile)")
     ap.add_argument("--configs", action="store_true",
@@ -178,6 +181,7 @@ def main() -> int:
         cpu = cpu_baseline(x, args)
     batched = batched_bench(args) if args.batch_count > 0 else None
     served = served_bench(args) if args.serve_seconds > 0 else None
+    front = None if args.no_front_end else front_end_bench(shape, x, args)
 
     line = {
         "metric": METRIC,
@@ -220,10 +224,11 @@ def main() -> int:
         "reference_api": ref_api,
         "batched": batched,
         "served": served,
+        "front_end": front,
         "gen_s": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)
-    return 0 if ok else 1
+    return 0 if ok and (front is None or front["codes_match_and_error_bounded"]) else 1
 
 
 def reference_api_bench(x, args, ws):
@@ -551,6 +556,70 @@ def width32_leg(xs, args, s):
     del plain
     torch.cuda.empty_cache()
     return out
+
+
+def front_end_bench(shape, codes, args):
+    """The Lorenzo front end on the headline field: `steps` forward passes
+    (mans_lorenzo_quantize: float32 field -> uint16 codes + outliers) and
+    `steps` inverse passes (mans_lorenzo_reconstruct), each launch timed with
+    HIP events.  Algorithmic bytes per element: 4 read + 2 written forward,
+    2 read + 4 written inverse (outliers, 8 B each, added).  Checks: the
+    forward codes equal the headline codes, and the rebuilt field is within
+    the error bound of the original."""
+    import torch
+
+    import mans_amd
+    from mans_amd import workloads
+
+    torch.cuda.empty_cache()
+    eb = 1e-3
+    f = workloads.sinusoid_field(shape)
+    n = f.numel()
+    ws = mans_amd.Workspace(0)
+    c, idx, val = mans_amd.lorenzo_quantize(f, eb, ws, radius=512)
+    rec = mans_amd.lorenzo_reconstruct(c, idx, val, shape, eb, ws)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(c, codes))
+    n_out = int(idx.numel())
+
+    def timed(fn, mark):
+        ws.set_profiling(True)
+        ws.kernel_times()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        kt = [ms for name, ms in ws.kernel_times() if name == mark]
+        ws.set_profiling(False)
+        return float(np.mean(kt)) if kt else float("nan")
+
+    c.fill_(0x5A5A)
+    fwd_ms = timed(lambda: mans_amd.lorenzo_quantize(f, eb, ws, radius=512, codes=c), "lorenzo")
+    ok = ok and bool(torch.equal(c, codes))
+    rec.fill_(float("nan"))
+    inv_ms = timed(lambda: mans_amd.lorenzo_reconstruct(c, idx, val, shape, eb, ws, out=rec), "lorenzo_inverse")
+    # |f - f'| <= eb up to the float32 rounding of q * 2eb (|q| < 2^28 here)
+    err = float((rec - f).abs().max())
+    ok = ok and err <= eb * (1 + 1e-4) + float(f.abs().max()) * 2 ** -23
+    ws.check()
+    ws.close()
+    fwd_bytes = 6 * n + 8 * n_out
+    inv_bytes = 6 * n + 8 * n_out
+    del f, rec, c
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"3-D dual-quant Lorenzo of the {shape[0]}x{shape[1]}x{shape[2]} float32 field "
+                    f"(error bound 1e-3, radius 512)",
+        "elements": n,
+        "outliers": n_out,
+        "forward_ms": round(fwd_ms, 4),
+        "forward_GBps_of_field": round(4 * n / (fwd_ms * 1e-3) / 1e9, 1),
+        "forward_hbm_frac": round(fwd_bytes / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        "inverse_ms": round(inv_ms, 4),
+        "inverse_GBps_of_field": round(4 * n / (inv_ms * 1e-3) / 1e9, 1),
+        "inverse_hbm_frac": round(inv_bytes / (inv_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        "max_abs_error": err,
+        "codes_match_and_error_bounded": ok,
+    }
 
 
 def served_bench(args):

@@ -21,7 +21,8 @@
 // neighbouring lane.  Outliers are
 // compacted in tile order (tile ticket, thread, plane, row) with a decoupled
 // look-back over tiles; a thread holds up to kHold of them in registers, and
-// a tile with more is rewritten by the fix-up launch.
+// a tile with more goes on a list, with each thread's offset inside the
+// tile, for the fix-up launch to write.
 #include "codec_device.hpp"
 
 namespace mans {
@@ -159,7 +160,7 @@ __device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t* s_w, uin
 }
 
 __global__ __launch_bounds__(kTX) void lorenzo_kernel(LorenzoArgs a) {
-  __shared__ uint32_t s_t, s_w[kTX / 64];
+  __shared__ uint32_t s_t, s_k, s_w[kTX / 64];
   __shared__ uint64_t s_off;
   __shared__ int s_bad;
   if (threadIdx.x == 0) {
@@ -195,32 +196,42 @@ __global__ __launch_bounds__(kTX) void lorenzo_kernel(LorenzoArgs a) {
       a.out_idx[pos + k] = hidx[k];
       a.out_val[pos + k] = hval[k];
     }
-  // a thread with more outliers than it holds: the fix-up launch rewrites
-  // the tile; outliers past the capacity: flagged
+  // a thread with more outliers than it holds: the tile goes on the fix-up
+  // list with every thread's offset inside the tile, and the fix-up launch
+  // writes the tile's outliers; outliers past the capacity: flagged
   const uint32_t over = __syncthreads_or(cnt > (uint32_t)kHold);
+  if (over) {  // uniform over the workgroup
+    if (threadIdx.x == 0) s_k = atomicAdd(a.fix_list, 1u);
+    __syncthreads();
+    const uint32_t k = s_k;
+    if (k < a.fix_cap) {
+      if (threadIdx.x == 0) a.fix_list[1 + k] = t;
+      a.fix_pos[(uint64_t)k * kTX + threadIdx.x] = mine;
+    } else if (threadIdx.x == 0) {
+      // more than fix_cap tiles over kHold: more than out_cap outliers
+      atomicOr(a.err, kErrCapacity);
+    }
+  }
   if (threadIdx.x == 0) {
-    a.tile_flag[t] = over ? 1u : 0u;
     if (s_off + total > a.out_cap) atomicOr(a.err, kErrCapacity);
     if (s_bad) atomicOr(a.err, kErrAdmDomain);
   }
 }
 
-// Rewrites every outlier of the tiles the main launch flagged.
+// Writes every outlier of the tiles on the fix-up list, each thread from the
+// offset the main launch left for it (one pass over each listed tile).
 __global__ __launch_bounds__(kTX) void lorenzo_fixup_kernel(LorenzoArgs a) {
-  __shared__ uint32_t s_w[kTX / 64];
-  const uint32_t t = blockIdx.x;
-  if (a.tile_flag[t] == 0) return;
-  uint32_t x0, y0, z0;
-  tile_of(a, t, x0, y0, z0);
-  uint32_t cnt = 0, dummy_i[kHold];
-  int32_t dummy_v[kHold];
-  bool bad = false;
-  // count first (same pass, nothing written), then write from the offsets
-  code_tile<false>(a, x0, y0, z0, cnt, dummy_i, dummy_v, 0, bad);
-  uint32_t total = 0;
-  const uint64_t pos = a.tile_off[t] + block_scan256(cnt, s_w, total);
-  cnt = 0;
-  code_tile<true>(a, x0, y0, z0, cnt, dummy_i, dummy_v, pos, bad);
+  const uint32_t nfix = a.fix_list[0] < a.fix_cap ? a.fix_list[0] : a.fix_cap;
+  for (uint32_t k = blockIdx.x; k < nfix; k += gridDim.x) {
+    const uint32_t t = a.fix_list[1 + k];
+    uint32_t x0, y0, z0;
+    tile_of(a, t, x0, y0, z0);
+    uint32_t cnt = 0, dummy_i[kHold];
+    int32_t dummy_v[kHold];
+    bool bad = false;
+    const uint64_t pos = a.tile_off[t] + a.fix_pos[(uint64_t)k * kTX + threadIdx.x];
+    code_tile<true>(a, x0, y0, z0, cnt, dummy_i, dummy_v, pos, bad);
+  }
 }
 
 // ---- inverse ---------------------------------------------------------------
@@ -236,56 +247,98 @@ __global__ void lorenzo_scatter_kernel(LorenzoArgs a) {
 }
 
 constexpr uint32_t kPlaneThreads = 1024;
+constexpr uint32_t kRowGroup = 16;  // rows of a plane loaded, scanned and synchronised together
+static_assert(kPlaneThreads / 64 == 16, "the wave totals are scanned over 16 lanes");
 
+// One workgroup per plane: per row, the x-prefix of r (wave scans, then the
+// waves' totals through LDS), summed down each column (the y-prefix).  Rows
+// go kRowGroup at a time: their loads are all issued before any is scanned,
+// and one barrier serves the group (the wave totals alternate between two
+// LDS buffers, so a buffer is rewritten only after the barrier that follows
+// its last read).
 __global__ __launch_bounds__(kPlaneThreads) void lorenzo_plane_kernel(LorenzoArgs a) {
-  __shared__ int32_t s_w[kPlaneThreads / 64];
+  constexpr uint32_t kMaxChunks = 8;  // nx <= 8192
+  __shared__ int32_t s_w[2][kRowGroup][kPlaneThreads / 64];
+  __shared__ int32_t s_col[kMaxChunks][kPlaneThreads];  // each thread's column sums, one per chunk
   const uint32_t z = blockIdx.x;
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
   const uint64_t plane = (uint64_t)a.ny * a.nx;
-  constexpr int kMaxChunks = 8;  // nx <= 8192
-  int32_t colsum[kMaxChunks];
+  const uint32_t nchunks = (uint32_t)div_up(a.nx, kPlaneThreads);
+  for (uint32_t c = 0; c < nchunks; ++c) s_col[c][threadIdx.x] = 0;
+  uint32_t it = 0;
+  for (uint32_t y0 = 0; y0 < a.ny; y0 += kRowGroup) {
+    const uint32_t nrow = a.ny - y0 < kRowGroup ? a.ny - y0 : kRowGroup;
+    int32_t carry[kRowGroup];
 #pragma unroll
-  for (int c = 0; c < kMaxChunks; ++c) colsum[c] = 0;
-  for (uint32_t y = 0; y < a.ny; ++y) {
-    int32_t carry = 0;
+    for (uint32_t g = 0; g < kRowGroup; ++g) carry[g] = 0;
+#pragma unroll 1
+    for (uint32_t c = 0; c < nchunks; ++c) {
+      const uint32_t x = c * kPlaneThreads + threadIdx.x;
+      const bool xin = x < a.nx;
+      const uint64_t base = z * plane + (uint64_t)y0 * a.nx + x;
+      // cells outside the field or the group read as code 32768 (r = 0)
+      uint32_t code[kRowGroup];
 #pragma unroll
-    for (int c = 0; c < kMaxChunks; ++c) {
-      const uint32_t x = (uint32_t)c * kPlaneThreads + threadIdx.x;
-      if ((uint32_t)c * kPlaneThreads >= a.nx) break;
-      const uint64_t i = z * plane + (uint64_t)y * a.nx + x;
-      int32_t r = 0;
-      if (x < a.nx) {
-        const uint16_t code = a.codes[i];
-        r = code ? (int32_t)code - 32768 : a.q[i];
-      }
-      int32_t inc = r;
+      for (uint32_t g = 0; g < kRowGroup; ++g) code[g] = xin && g < nrow ? a.codes[base + (uint64_t)g * a.nx] : 32768u;
+      int32_t inc[kRowGroup];
+#pragma unroll
+      for (uint32_t g = 0; g < kRowGroup; ++g)
+        inc[g] = code[g] ? (int32_t)code[g] - 32768 : a.q[base + (uint64_t)g * a.nx];
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
-        const int32_t u = __shfl_up(inc, o, 64);
-        if (lane >= (uint32_t)o) inc += u;
+#pragma unroll
+        for (uint32_t g = 0; g < kRowGroup; ++g) {
+          const int32_t u = __shfl_up(inc[g], o, 64);
+          if (lane >= (uint32_t)o) inc[g] += u;
+        }
       }
-      if (lane == 63) s_w[w] = inc;
-      __syncthreads();
-      int32_t before = carry, all = 0;
-      for (uint32_t k = 0; k < kPlaneThreads / 64; ++k) {
-        if (k < w) before += s_w[k];
-        all += s_w[k];
+      const uint32_t buf = it & 1u;
+      ++it;
+      if (lane == 63) {
+#pragma unroll
+        for (uint32_t g = 0; g < kRowGroup; ++g) s_w[buf][g][w] = inc[g];
       }
       __syncthreads();
-      carry += all;
-      colsum[c] += before + inc;  // x-prefix of this row, summed over rows
-      if (x < a.nx) a.q[i] = colsum[c];
+      int32_t colsum = s_col[c][threadIdx.x];
+      const uint32_t l16 = lane & 15u;
+#pragma unroll
+      for (uint32_t g = 0; g < kRowGroup; ++g) {
+        // scan of the 16 wave totals over each 16-lane group of the wave
+        int32_t v = s_w[buf][g][l16];
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+          const int32_t u = __shfl_up(v, o, 16);
+          if (l16 >= (uint32_t)o) v += u;
+        }
+        const int32_t all = __shfl(v, 15, 64);
+        const int32_t prior = __shfl(v, w ? w - 1 : 0, 64);
+        const int32_t before = carry[g] + (w ? prior : 0);
+        carry[g] += all;
+        colsum += before + inc[g];  // x-prefix of this row, summed over rows
+        if (xin && g < nrow) a.q[base + (uint64_t)g * a.nx] = colsum;
+      }
+      s_col[c][threadIdx.x] = colsum;
     }
   }
 }
 
-__global__ void lorenzo_column_kernel(LorenzoArgs a) {
+constexpr uint32_t kColumnPlanes = 16;  // planes of a column loaded before any is summed
+
+// Per (y, x) column: the z-prefix of the plane sums and f' = q * 2 eb.
+__global__ __launch_bounds__(256) void lorenzo_column_kernel(LorenzoArgs a) {
   const uint64_t plane = (uint64_t)a.ny * a.nx;
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < plane; j += (uint64_t)gridDim.x * blockDim.x) {
     int32_t s = 0;
-    for (uint32_t z = 0; z < a.nz; ++z) {
-      s += a.q[z * plane + j];
-      a.recon[z * plane + j] = (float)s * a.twoeb;
+    for (uint32_t z0 = 0; z0 < a.nz; z0 += kColumnPlanes) {
+      int32_t v[kColumnPlanes];
+#pragma unroll
+      for (uint32_t k = 0; k < kColumnPlanes; ++k)
+        v[k] = z0 + k < a.nz ? a.q[(uint64_t)(z0 + k) * plane + j] : 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kColumnPlanes; ++k) {
+        s += v[k];
+        if (z0 + k < a.nz) a.recon[(uint64_t)(z0 + k) * plane + j] = (float)s * a.twoeb;
+      }
     }
   }
 }
@@ -297,7 +350,10 @@ hipError_t launch_lorenzo(const LorenzoArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(lorenzo_kernel, dim3(a.ntiles), dim3(kTX), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(lorenzo_fixup_kernel, dim3(a.ntiles), dim3(kTX), 0, s, a);
+  // the fix-up list is short (a tile is on it only with more than kHold
+  // outliers in one thread): a small grid walks it
+  const uint32_t g = a.fix_cap < 1024u ? a.fix_cap : 1024u;
+  hipLaunchKernelGGL(lorenzo_fixup_kernel, dim3(g), dim3(kTX), 0, s, a);
   return hipGetLastError();
 }
 

@@ -315,9 +315,10 @@ struct LorenzoArgs {
   uint64_t* status;              // [ntiles] look-back words (zeroed before)
   uint32_t* ticket;              // tile ticket (zeroed before)
   uint64_t* tile_off;            // [ntiles] first outlier of each tile
-  uint32_t* tile_flag;           // [ntiles] 1: the fix-up launch rewrites the tile
+  uint32_t* fix_list;            // [1 + fix_cap]: count (zeroed before), then tiles for the fix-up launch
+  uint32_t* fix_pos;             // [fix_cap][256]: each thread's first outlier inside a listed tile
   uint32_t ntiles;
-  uint32_t pad_;
+  uint32_t fix_cap;              // >= 1
   uint32_t* err;                 // kErrCapacity: outliers beyond out_cap; kErrAdmDomain: |q| >= 2^28
   // inverse
   const uint32_t* in_idx;

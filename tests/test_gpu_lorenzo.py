@@ -49,7 +49,8 @@ def check(f, eb, radius, ws, cap=None):
     return ri.size
 
 
-@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 5, 7), (17, 33, 300), (20, 16, 256), (5, 70, 1000)])
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 5, 7), (17, 33, 300), (20, 16, 256), (5, 70, 1000),
+                                   (3, 19, 2500), (2, 4, 8192)])
 def test_lorenzo_matches_oracle(ws, shape):
     check(field(shape, sum(shape), spikes=5), 1e-3, 512, ws)
 
@@ -60,6 +61,18 @@ def test_lorenzo_fixup_path(ws):
     f = field((18, 40, 300), 7, spikes=3)
     n = check(f, 1e-4, 1, ws, cap=f.size)
     assert n > 18 * 40 * 300 // 2
+
+
+def test_lorenzo_fixup_list_over_capacity(ws):
+    # radius 1 with a small capacity: more tiles need the fix-up than the
+    # capacity can hold outliers for, which is a capacity error; the same
+    # field with room for every outlier still matches the oracle afterwards
+    f = field((18, 40, 300), 8, spikes=3)
+    d = torch.from_numpy(f).cuda()
+    with pytest.raises(mans_amd.MansError) as e:
+        mans_amd.lorenzo_quantize(d, 1e-4, ws, radius=1, outlier_capacity=40)
+    assert e.value.status == 5  # MANS_ERR_CAPACITY
+    check(f, 1e-4, 1, ws, cap=f.size)
 
 
 def test_lorenzo_capacity_and_domain(ws):
