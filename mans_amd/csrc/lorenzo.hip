@@ -246,25 +246,35 @@ __global__ void lorenzo_scatter_kernel(LorenzoArgs a) {
   }
 }
 
-constexpr uint32_t kPlaneThreads = 1024;
-constexpr uint32_t kRowGroup = 16;  // rows of a plane loaded, scanned and synchronised together
-static_assert(kPlaneThreads / 64 == 16, "the wave totals are scanned over 16 lanes");
+constexpr uint32_t kPlaneChunk = 1024;  // elements of a row a workgroup scans at a time
+constexpr uint32_t kRowGroup = 8;       // rows of a plane loaded, scanned and synchronised together
 
 // One workgroup per plane: per row, the x-prefix of r (wave scans, then the
-// waves' totals through LDS), summed down each column (the y-prefix).  Rows
-// go kRowGroup at a time: their loads are all issued before any is scanned,
-// and one barrier serves the group (the wave totals alternate between two
-// LDS buffers, so a buffer is rewritten only after the barrier that follows
-// its last read).
-__global__ __launch_bounds__(kPlaneThreads) void lorenzo_plane_kernel(LorenzoArgs a) {
-  constexpr uint32_t kMaxChunks = 8;  // nx <= 8192
-  __shared__ int32_t s_w[2][kRowGroup][kPlaneThreads / 64];
-  __shared__ int32_t s_col[kMaxChunks][kPlaneThreads];  // each thread's column sums, one per chunk
+// waves' totals through LDS), summed down each column (the y-prefix).  A
+// thread takes E adjacent elements of a row (E = 2: one 4-byte load of two
+// codes, one 8-byte store of two sums; needs an even row length and 4-byte
+// aligned codes).  Rows go kRowGroup at a time: their loads are all issued
+// before any is scanned, and one barrier serves the group (the wave totals
+// alternate between two LDS buffers, so a buffer is rewritten only after the
+// barrier that follows its last read).  Rows longer than kPlaneChunk go
+// chunk by chunk with the row's running total carried between chunks.
+template <uint32_t E>
+__global__ __launch_bounds__(kPlaneChunk / E) __attribute__((amdgpu_waves_per_eu(8)))
+void lorenzo_plane_kernel(LorenzoArgs a) {
+  constexpr uint32_t kT = kPlaneChunk / E, kW = kT / 64;  // threads, waves
+  constexpr uint32_t kMaxChunks = 8;                     // nx <= 8192
+  static_assert(E == 1 || E == 2, "one or two elements a thread");
+  __shared__ int32_t s_w[2][kRowGroup][kW];
+  __shared__ int32_t s_col[kMaxChunks][kPlaneChunk];  // column sums of each chunk's elements
   const uint32_t z = blockIdx.x;
   const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint32_t wu = __builtin_amdgcn_readfirstlane(w);
+  const uint32_t lw = lane & (kW - 1);
   const uint64_t plane = (uint64_t)a.ny * a.nx;
-  const uint32_t nchunks = (uint32_t)div_up(a.nx, kPlaneThreads);
-  for (uint32_t c = 0; c < nchunks; ++c) s_col[c][threadIdx.x] = 0;
+  const uint32_t nchunks = (uint32_t)div_up(a.nx, kPlaneChunk);
+  for (uint32_t c = 0; c < nchunks; ++c)
+#pragma unroll
+    for (uint32_t e = 0; e < E; ++e) s_col[c][threadIdx.x * E + e] = 0;
   uint32_t it = 0;
   for (uint32_t y0 = 0; y0 < a.ny; y0 += kRowGroup) {
     const uint32_t nrow = a.ny - y0 < kRowGroup ? a.ny - y0 : kRowGroup;
@@ -273,17 +283,32 @@ __global__ __launch_bounds__(kPlaneThreads) void lorenzo_plane_kernel(LorenzoArg
     for (uint32_t g = 0; g < kRowGroup; ++g) carry[g] = 0;
 #pragma unroll 1
     for (uint32_t c = 0; c < nchunks; ++c) {
-      const uint32_t x = c * kPlaneThreads + threadIdx.x;
+      // this thread's first element; with E = 2 the row length is even, so
+      // x < nx means x + 1 < nx
+      const uint32_t x = c * kPlaneChunk + threadIdx.x * E;
       const bool xin = x < a.nx;
-      const uint64_t base = z * plane + (uint64_t)y0 * a.nx + x;
+      // row g of the group starts at element row0 + g * nx (uniform; the
+      // lane adds x)
+      const uint64_t row0 = z * plane + (uint64_t)y0 * a.nx;
       // cells outside the field or the group read as code 32768 (r = 0)
       uint32_t code[kRowGroup];
 #pragma unroll
-      for (uint32_t g = 0; g < kRowGroup; ++g) code[g] = xin && g < nrow ? a.codes[base + (uint64_t)g * a.nx] : 32768u;
-      int32_t inc[kRowGroup];
+      for (uint32_t g = 0; g < kRowGroup; ++g) {
+        const uint16_t* p = a.codes + row0 + (uint64_t)g * a.nx + x;
+        if (E == 1) code[g] = xin && g < nrow ? *p : 32768u;
+        else code[g] = xin && g < nrow ? *(const uint32_t*)p : 0x80008000u;
+      }
+      int32_t v[kRowGroup][E], inc[kRowGroup];
 #pragma unroll
-      for (uint32_t g = 0; g < kRowGroup; ++g)
-        inc[g] = code[g] ? (int32_t)code[g] - 32768 : a.q[base + (uint64_t)g * a.nx];
+      for (uint32_t g = 0; g < kRowGroup; ++g) {
+        inc[g] = 0;
+#pragma unroll
+        for (uint32_t e = 0; e < E; ++e) {
+          const uint32_t ce = (code[g] >> (16 * e)) & 0xFFFFu;
+          v[g][e] = ce ? (int32_t)ce - 32768 : (a.q + row0 + (uint64_t)g * a.nx)[x + e];
+          inc[g] += v[g][e];
+        }
+      }
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
 #pragma unroll
@@ -299,25 +324,39 @@ __global__ __launch_bounds__(kPlaneThreads) void lorenzo_plane_kernel(LorenzoArg
         for (uint32_t g = 0; g < kRowGroup; ++g) s_w[buf][g][w] = inc[g];
       }
       __syncthreads();
-      int32_t colsum = s_col[c][threadIdx.x];
-      const uint32_t l16 = lane & 15u;
+      int32_t colsum[E];
+#pragma unroll
+      for (uint32_t e = 0; e < E; ++e) colsum[e] = s_col[c][threadIdx.x * E + e];
 #pragma unroll
       for (uint32_t g = 0; g < kRowGroup; ++g) {
-        // scan of the 16 wave totals over each 16-lane group of the wave
-        int32_t v = s_w[buf][g][l16];
+        // scan of the kW wave totals over each kW-lane group of the wave
+        int32_t t = s_w[buf][g][lw];
 #pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-          const int32_t u = __shfl_up(v, o, 16);
-          if (l16 >= (uint32_t)o) v += u;
+        for (uint32_t o = 1; o < kW; o <<= 1) {
+          const int32_t u = __shfl_up(t, o, kW);
+          if (lw >= o) t += u;
         }
-        const int32_t all = __shfl(v, 15, 64);
-        const int32_t prior = __shfl(v, w ? w - 1 : 0, 64);
-        const int32_t before = carry[g] + (w ? prior : 0);
+        // all waves' total and the total of the waves before this one
+        // (uniform: read from one lane)
+        const int32_t all = __builtin_amdgcn_readlane(t, kW - 1);
+        const int32_t prior = wu ? __builtin_amdgcn_readlane(t, wu - 1) : 0;
+        int32_t run = carry[g] + prior + inc[g];  // x-prefix through this thread's last element
+#pragma unroll
+        for (uint32_t e = 0; e < E; ++e) run -= v[g][e];
+#pragma unroll
+        for (uint32_t e = 0; e < E; ++e) {
+          run += v[g][e];
+          colsum[e] += run;  // x-prefix of this row, summed over rows
+        }
         carry[g] += all;
-        colsum += before + inc[g];  // x-prefix of this row, summed over rows
-        if (xin && g < nrow) a.q[base + (uint64_t)g * a.nx] = colsum;
+        if (xin && g < nrow) {
+          int32_t* q = a.q + row0 + (uint64_t)g * a.nx + x;
+          if (E == 1) *q = colsum[0];
+          else *(int2*)q = make_int2(colsum[0], colsum[E - 1]);
+        }
       }
-      s_col[c][threadIdx.x] = colsum;
+#pragma unroll
+      for (uint32_t e = 0; e < E; ++e) s_col[c][threadIdx.x * E + e] = colsum[e];
     }
   }
 }
@@ -370,7 +409,12 @@ hipError_t launch_lorenzo_inverse(const LorenzoArgs& a, int cu, hipStream_t s) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(lorenzo_plane_kernel, dim3(a.nz), dim3(kPlaneThreads), 0, s, a);
+  // two elements a thread where rows are even and the codes 4-byte aligned
+  // (the q scratch is a whole allocation)
+  if (a.nx % 2 == 0 && ((uintptr_t)a.codes & 3u) == 0)
+    hipLaunchKernelGGL(lorenzo_plane_kernel<2>, dim3(a.nz), dim3(kPlaneChunk / 2), 0, s, a);
+  else
+    hipLaunchKernelGGL(lorenzo_plane_kernel<1>, dim3(a.nz), dim3(kPlaneChunk), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const uint64_t plane = (uint64_t)a.ny * a.nx;

@@ -43,6 +43,12 @@ def check(f, eb, radius, ws, cap=None):
     rec = mans_amd.lorenzo_reconstruct(codes, idx, val, f.shape, eb, ws).cpu().numpy()
     ref = L.lorenzo_reconstruct(rc, ri, rv, f.shape, eb)
     assert (rec == ref).all()
+    # codes at an address that is 2 but not 4 bytes aligned take the
+    # one-element-a-thread path of the inverse: same field
+    odd = torch.empty(codes.numel() + 1, dtype=codes.dtype, device=codes.device)
+    odd[1:] = codes
+    rec2 = mans_amd.lorenzo_reconstruct(odd[1:], idx, val, f.shape, eb, ws).cpu().numpy()
+    assert (rec2 == ref).all()
     # the error bound, up to the float32 roundings of f / (2 eb) and q * 2 eb
     assert np.abs(rec.astype(np.float64) - f).max() <= eb + 4 * np.finfo(np.float32).eps * np.abs(f).max()
     ws.check()
@@ -50,7 +56,7 @@ def check(f, eb, radius, ws, cap=None):
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (3, 5, 7), (17, 33, 300), (20, 16, 256), (5, 70, 1000),
-                                   (3, 19, 2500), (2, 4, 8192)])
+                                   (3, 19, 2500), (2, 4, 8192), (2, 9, 2049)])
 def test_lorenzo_matches_oracle(ws, shape):
     check(field(shape, sum(shape), spikes=5), 1e-3, 512, ws)
 
