@@ -742,10 +742,11 @@ mans_status mans_lorenzo_quantize(const float* d_field, uint32_t nz, uint32_t ny
   const uint32_t ntiles = n ? lorenzo_tiles(nz, ny, nx) : 0;
   // scratch: tile offsets (8 B a tile), the fix-up list (4 B a tile) and
   // its thread offsets (1 KiB a listed tile; a tile is listed only with more
-  // than 4 outliers, so more than out_cap / 5 + 1 listed tiles means more
-  // than out_cap outliers, which is a capacity error anyway)
+  // than kLorenzoHold outliers, so more than out_cap / (kLorenzoHold + 1) + 1
+  // listed tiles means more than out_cap outliers, which is a capacity error
+  // anyway)
   const uint64_t nt = std::max<uint32_t>(ntiles, 1);
-  const uint32_t fix_cap = (uint32_t)std::min<uint64_t>(nt, outlier_capacity / 5 + 1);
+  const uint32_t fix_cap = (uint32_t)std::min<uint64_t>(nt, outlier_capacity / (kLorenzoHold + 1) + 1);
   const uint64_t list_off = 8 * nt, pos_off = list_off + 4 * (nt + 1);
   mans_status st = ensure_status(ws, nt);
   if (st == MANS_OK) st = ensure_scratch(ws, pos_off + 1024ull * fix_cap);

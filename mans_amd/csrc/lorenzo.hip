@@ -29,7 +29,7 @@ namespace mans {
 namespace {
 
 constexpr uint32_t kTX = 256, kTY = 16, kTZ = 16;
-constexpr int kHold = 4;
+constexpr int kHold = kLorenzoHold;
 constexpr int32_t kQLimit = 1 << 28;  // |q| bound: residuals stay inside int32
 
 __device__ __forceinline__ int32_t quant(float f, float inv, bool& bad) {

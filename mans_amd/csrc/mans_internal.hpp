@@ -300,6 +300,9 @@ hipError_t launch_archive_pack(const ArchiveArgs& a, uint64_t max_frame, hipStre
 hipError_t launch_gather_headers(const uint8_t* const* src, uint32_t count, uint4* out, hipStream_t s);
 
 // ---- dual-quantisation Lorenzo front end (lorenzo.hip) ----------------------
+// outliers a thread of the Lorenzo forward launch holds in registers; a tile
+// with a thread over it goes to the fix-up launch
+constexpr int kLorenzoHold = 6;
 struct LorenzoArgs {
   const float* field;            // forward: f[nz][ny][nx]
   uint16_t* codes;               // forward: out; inverse: in
