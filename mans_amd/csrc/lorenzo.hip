@@ -54,16 +54,18 @@ __device__ __forceinline__ void tile_of(const LorenzoArgs& a, uint32_t t, uint32
 
 // Codes of one tile; outliers go to `hold` (first kHold) and are counted in
 // `cnt`.  With WRITE_ALL, every outlier is written at a.out_idx / a.out_val
-// from position `pos` on instead (the fix-up pass).
-template <bool WRITE_ALL>
+// from position `pos` on instead (the fix-up pass).  FULL: nx is a multiple
+// of 256 and ny of 16, so every column and row of every tile is inside the
+// field (only the row above the first row of tiles is not).
+template <bool WRITE_ALL, bool FULL>
 __device__ void code_tile(const LorenzoArgs& a, uint32_t x0, uint32_t y0, uint32_t z0, uint32_t& cnt,
                           uint32_t (&hidx)[kHold], int32_t (&hval)[kHold], uint64_t pos, bool& bad) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t x = x0 + threadIdx.x;
-  const bool xin = x < a.nx;
+  const bool xin = FULL || x < a.nx;
   const uint64_t plane = (uint64_t)a.ny * a.nx;
   const float inv = a.inv2eb;
-  const uint32_t ny1 = a.ny - y0 < kTY ? a.ny - y0 : kTY;
+  const uint32_t ny1 = FULL ? kTY : (a.ny - y0 < kTY ? a.ny - y0 : kTY);
   const uint32_t nz1 = a.nz - z0 < kTZ ? a.nz - z0 : kTZ;
   // q of plane z at rows y0 - 1 .. y0 + kTY - 1 (row j - 1 of the tile in
   // slot j; rows outside the field or the tile are 0) for this column and,
@@ -74,19 +76,19 @@ __device__ void code_tile(const LorenzoArgs& a, uint32_t x0, uint32_t y0, uint32
 #pragma unroll
     for (uint32_t j = 0; j <= kTY; ++j) {
       const int64_t y = (int64_t)y0 - 1 + (int64_t)j;
-      const bool row = y >= 0 && (j == 0 || j - 1 < ny1);
+      const bool row = FULL ? (j > 0 || y0 > 0) : (y >= 0 && (j == 0 || j - 1 < ny1));
       const uint64_t r = row ? z * plane + (uint64_t)y * a.nx : 0u;
       fc[j] = row && xin ? a.field[r + x] : 0.0f;
       fl[j] = 0.0f;
-      if (lane == 0 && row && x > 0 && x - 1 < a.nx) fl[j] = a.field[r + x - 1];
+      if (lane == 0 && row && x > 0 && (FULL || x - 1 < a.nx)) fl[j] = a.field[r + x - 1];
     }
 #pragma unroll
     for (uint32_t j = 0; j <= kTY; ++j) {
       const int64_t y = (int64_t)y0 - 1 + (int64_t)j;
-      const bool row = y >= 0 && (j == 0 || j - 1 < ny1);
+      const bool row = FULL ? (j > 0 || y0 > 0) : (y >= 0 && (j == 0 || j - 1 < ny1));
       qc[j] = row && xin ? quant(fc[j], inv, bad) : 0;
       int32_t nb = __shfl_up(qc[j], 1, 64);
-      if (lane == 0) nb = row && x > 0 && x - 1 < a.nx ? quant(fl[j], inv, bad) : 0;
+      if (lane == 0) nb = row && x > 0 && (FULL || x - 1 < a.nx) ? quant(fl[j], inv, bad) : 0;
       ql[j] = nb;
     }
   };
@@ -159,6 +161,7 @@ __device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t* s_w, uin
   return before + inc - v;
 }
 
+template <bool FULL>
 __global__ __launch_bounds__(kTX) void lorenzo_kernel(LorenzoArgs a) {
   __shared__ uint32_t s_t, s_k, s_w[kTX / 64];
   __shared__ uint64_t s_off;
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(kTX) void lorenzo_kernel(LorenzoArgs a) {
   uint32_t hidx[kHold];
   int32_t hval[kHold];
   bool bad = false;
-  code_tile<false>(a, x0, y0, z0, cnt, hidx, hval, 0, bad);
+  code_tile<false, FULL>(a, x0, y0, z0, cnt, hidx, hval, 0, bad);
   if (bad) s_bad = 1;
   uint32_t total = 0;
   const uint32_t mine = block_scan256(cnt, s_w, total);
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(kTX) void lorenzo_fixup_kernel(LorenzoArgs a) {
     int32_t dummy_v[kHold];
     bool bad = false;
     const uint64_t pos = a.tile_off[t] + a.fix_pos[(uint64_t)k * kTX + threadIdx.x];
-    code_tile<true>(a, x0, y0, z0, cnt, dummy_i, dummy_v, pos, bad);
+    code_tile<true, false>(a, x0, y0, z0, cnt, dummy_i, dummy_v, pos, bad);
   }
 }
 
@@ -386,7 +389,10 @@ __global__ __launch_bounds__(256) void lorenzo_column_kernel(LorenzoArgs a) {
 
 hipError_t launch_lorenzo(const LorenzoArgs& a, hipStream_t s) {
   if (a.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(lorenzo_kernel, dim3(a.ntiles), dim3(kTX), 0, s, a);
+  if (a.nx % kTX == 0 && a.ny % kTY == 0)
+    hipLaunchKernelGGL(lorenzo_kernel<true>, dim3(a.ntiles), dim3(kTX), 0, s, a);
+  else
+    hipLaunchKernelGGL(lorenzo_kernel<false>, dim3(a.ntiles), dim3(kTX), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // the fix-up list is short (a tile is on it only with more than kHold
