@@ -61,12 +61,14 @@ def test_lorenzo_matches_oracle(ws, shape):
     check(field(shape, sum(shape), spikes=5), 1e-3, 512, ws)
 
 
-def test_lorenzo_fixup_path(ws):
+@pytest.mark.parametrize("shape", [(18, 40, 300), (17, 32, 512)])
+def test_lorenzo_fixup_path(ws, shape):
     # radius 1: every nonzero residual is an outlier, so threads overflow the
-    # outliers they hold and the fix-up launch writes their tiles
-    f = field((18, 40, 300), 7, spikes=3)
+    # outliers they hold and the fix-up launch writes their tiles (the second
+    # shape has whole tiles only: the forward launch without bounds)
+    f = field(shape, 7, spikes=3)
     n = check(f, 1e-4, 1, ws, cap=f.size)
-    assert n > 18 * 40 * 300 // 2
+    assert n > f.size // 2
 
 
 def test_lorenzo_fixup_list_over_capacity(ws):
