@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--no-front-end", action="store_true", help="skip the Lorenzo front-end leg")
     ap.add_argument("--dump", default=None,
                     help="write the generated codes to this file and exit (input of mans_profile)")
+    ap.add_argument("--dump-workload", choices=("lorenzo", "detector"), default="lorenzo",
+                    help="--dump: the headline Lorenzo codes, or the 64 x 2048^2 uint16 detector frames")
     ap.add_argument("--configs", action="store_true",
                     help="instead of the benchmark line, print one JSON line per input of the benchmark plan "
                          "(c0, c1 at 512 MiB, c2a, c2b) and coding form")
@@ -85,6 +87,11 @@ def main() -> int:
         return configs_bench(args)
     shape = tuple(int(v) for v in args.shape.split("x"))
     torch.cuda.set_device(0)
+    if args.dump and args.dump_workload == "detector":
+        d = workloads.detector_u16()
+        d.cpu().numpy().tofile(args.dump)
+        print(json.dumps({"dumped": args.dump, "bytes": d.numel() * 2}), flush=True)
+        return 0
     t0 = time.time()
     x = workloads.lorenzo_codes(shape)
     torch.cuda.synchronize()

@@ -2,7 +2,7 @@
 // in it, for hardware-counter runs (rocprofv3 --pmc) of the frame codec.
 //
 //   mans_profile <codes.u2> [--steps K] [--lanes 32|64] [--prob-bits P]
-//                [--elem-bytes 1|2|4] [--no-copy]
+//                [--elem-bytes 1|2|4] [--mode planes|adm] [--no-copy]
 //
 // Reads the element file into device memory, runs K frame encodes and K frame
 // decodes on one stream (checking the round trip once), then one plain device
@@ -35,11 +35,14 @@ __global__ void copy_u4(const uint4* __restrict__ in, uint4* __restrict__ out, s
 
 int main(int argc, char** argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: %s <codes> [--steps K] [--lanes L] [--prob-bits P] [--elem-bytes B] [--no-copy]\n",
+    fprintf(stderr,
+            "usage: %s <codes> [--steps K] [--lanes L] [--prob-bits P] [--elem-bytes B] [--mode planes|adm] "
+            "[--no-copy]\n",
             argv[0]);
     return 1;
   }
   int steps = 3, lanes = 64, pb = 10, eb = 2;
+  int32_t mode = MANS_MODE_PLANES;
   bool copy = true;
   for (int i = 2; i < argc; ++i) {
     std::string a = argv[i];
@@ -47,7 +50,14 @@ int main(int argc, char** argv) {
     else if (a == "--lanes" && i + 1 < argc) lanes = atoi(argv[++i]);
     else if (a == "--prob-bits" && i + 1 < argc) pb = atoi(argv[++i]);
     else if (a == "--elem-bytes" && i + 1 < argc) eb = atoi(argv[++i]);
-    else if (a == "--no-copy") copy = false;
+    else if (a == "--mode" && i + 1 < argc) {
+      const std::string m = argv[++i];
+      if (m == "adm") mode = MANS_MODE_ADM;
+      else if (m != "planes") {
+        fprintf(stderr, "unknown mode %s\n", m.c_str());
+        return 1;
+      }
+    } else if (a == "--no-copy") copy = false;
   }
   std::ifstream f(argv[1], std::ios::binary | std::ios::ate);
   if (!f) {
@@ -59,7 +69,7 @@ int main(int argc, char** argv) {
   std::vector<uint8_t> host(bytes);
   f.read((char*)host.data(), (std::streamsize)bytes);
   const uint64_t n = bytes / (size_t)eb;
-  mans_params p = {pb, lanes, eb, MANS_MODE_PLANES};
+  mans_params p = {pb, lanes, eb, mode};
   mans_workspace* ws = nullptr;
   if (mans_workspace_create(&ws, 0) != MANS_OK) return 2;
   mans_workspace_reserve(ws, bytes, &p);
