@@ -76,7 +76,8 @@ def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only (the codec path is single-GPU; "
-                                   "sharding is described in DESIGN.md)", "n_gpus": args.gpus}))
+                                   "across GPUs, whole buffers or frames would go to ranks, each coded "
+                                   "on its own GPU)", "n_gpus": args.gpus}))
         return 2
     import torch
 
