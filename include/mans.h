@@ -169,9 +169,9 @@ mans_status mans_adm_unmap(const int32_t* d_offsets, const uint16_t* d_centers, 
  * outlier_capacity (then only the first outlier_capacity are written).
  * With h_outlier_count non-NULL the call synchronises, writes the count
  * there and returns MANS_ERR_CAPACITY / MANS_ERR_DOMAIN when they apply.
- * Workspace: 12 bytes per tile of 256 x 16 x 16 cells, plus 1 KiB for each
- * of min(tiles, outlier_capacity / 7 + 1) tiles (the tiles where a thread
- * finds more than 6 outliers are coded again by a second launch). */
+ * Workspace: 12 bytes per tile of 256 x 16 x 16 cells, plus 2 KiB for each
+ * of min(tiles, outlier_capacity / 6 + 1) tiles (the tiles where a thread
+ * finds more than 5 outliers are coded again by a second launch). */
 mans_status mans_lorenzo_quantize(const float* d_field, uint32_t nz, uint32_t ny, uint32_t nx, double error_bound,
                                   uint32_t radius, uint16_t* d_codes, uint32_t* d_outlier_idx, int32_t* d_outlier_val,
                                   uint64_t outlier_capacity, uint64_t* d_outlier_count, uint64_t* h_outlier_count,

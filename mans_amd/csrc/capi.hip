@@ -740,16 +740,16 @@ mans_status mans_lorenzo_quantize(const float* d_field, uint32_t nz, uint32_t ny
   hipStream_t s = (hipStream_t)stream;
   note_capture(ws, s);
   const uint32_t ntiles = n ? lorenzo_tiles(nz, ny, nx) : 0;
-  // scratch: tile offsets (8 B a tile), the fix-up list (4 B a tile) and
-  // its thread offsets (1 KiB a listed tile; a tile is listed only with more
-  // than kLorenzoHold outliers, so more than out_cap / (kLorenzoHold + 1) + 1
-  // listed tiles means more than out_cap outliers, which is a capacity error
-  // anyway)
+  // scratch: tile offsets (8 B a tile), the fix-up list (4 B a tile), its
+  // thread offsets and per-plane-group counts (2 KiB a listed tile; a tile
+  // is listed only with more than kLorenzoHold outliers, so more than
+  // out_cap / (kLorenzoHold + 1) + 1 listed tiles means more than out_cap
+  // outliers, which is a capacity error anyway)
   const uint64_t nt = std::max<uint32_t>(ntiles, 1);
   const uint32_t fix_cap = (uint32_t)std::min<uint64_t>(nt, outlier_capacity / (kLorenzoHold + 1) + 1);
-  const uint64_t list_off = 8 * nt, pos_off = list_off + 4 * (nt + 1);
+  const uint64_t list_off = 8 * nt, pos_off = list_off + 4 * (nt + 1), grp_off = pos_off + 1024ull * fix_cap;
   mans_status st = ensure_status(ws, nt);
-  if (st == MANS_OK) st = ensure_scratch(ws, pos_off + 1024ull * fix_cap);
+  if (st == MANS_OK) st = ensure_scratch(ws, grp_off + 1024ull * fix_cap);
   if (st != MANS_OK) return st;
   // counters[5]: tile ticket; counters[4]: this call's flags
   HIP_TRY(launch_fill_u64(ws->status, ntiles, 0, s));
@@ -775,6 +775,7 @@ mans_status mans_lorenzo_quantize(const float* d_field, uint32_t nz, uint32_t ny
   a.tile_off = (uint64_t*)ws->scratch;
   a.fix_list = (uint32_t*)(ws->scratch + list_off);
   a.fix_pos = (uint32_t*)(ws->scratch + pos_off);
+  a.fix_grp = (uint32_t*)(ws->scratch + grp_off);
   a.fix_cap = fix_cap;
   a.ntiles = ntiles;
   a.err = ws->counters + 4;

@@ -22,7 +22,8 @@
 // compacted in tile order (tile ticket, thread, plane, row) with a decoupled
 // look-back over tiles; a thread holds up to kHold of them in registers, and
 // a tile with more goes on a list, with each thread's offset inside the
-// tile, for the fix-up launch to write.
+// tile and its counts before planes 4, 8 and 12, for the fix-up launch to
+// write (four plane groups of the tile at once).
 #include "codec_device.hpp"
 
 namespace mans {
@@ -52,21 +53,25 @@ __device__ __forceinline__ void tile_of(const LorenzoArgs& a, uint32_t t, uint32
   z0 = (t / (nxb * nyb)) * kTZ;
 }
 
-// Codes of one tile; outliers go to `hold` (first kHold) and are counted in
-// `cnt`.  With WRITE_ALL, every outlier is written at a.out_idx / a.out_val
-// from position `pos` on instead (the fix-up pass).  FULL: nx is a multiple
-// of 256 and ny of 16, so every column and row of every tile is inside the
-// field (only the row above the first row of tiles is not).
+// Codes of planes [dz0, dz1) of one tile (a thread per x-column: column
+// threadIdx.x % 256); outliers go to `hold` (first kHold) and are counted in
+// `cnt`, and `grp` gets the count before planes 4, 8 and 12 (a byte each;
+// at most 192).  With WRITE_ALL, every outlier is written at a.out_idx /
+// a.out_val from position `pos` on instead (the fix-up pass).  FULL: nx is a
+// multiple of 256 and ny of 16, so every column and row of every tile is
+// inside the field (only the row above the first row of tiles is not).
 template <bool WRITE_ALL, bool FULL>
-__device__ void code_tile(const LorenzoArgs& a, uint32_t x0, uint32_t y0, uint32_t z0, uint32_t& cnt,
-                          uint32_t (&hidx)[kHold], int32_t (&hval)[kHold], uint64_t pos, bool& bad) {
+__device__ void code_tile(const LorenzoArgs& a, uint32_t x0, uint32_t y0, uint32_t z0, uint32_t dz0, uint32_t dz1,
+                          uint32_t& cnt, uint32_t& grp, uint32_t (&hidx)[kHold], int32_t (&hval)[kHold], uint64_t pos,
+                          bool& bad) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t x = x0 + threadIdx.x;
+  const uint32_t x = x0 + threadIdx.x % kTX;
   const bool xin = FULL || x < a.nx;
   const uint64_t plane = (uint64_t)a.ny * a.nx;
   const float inv = a.inv2eb;
   const uint32_t ny1 = FULL ? kTY : (a.ny - y0 < kTY ? a.ny - y0 : kTY);
-  const uint32_t nz1 = a.nz - z0 < kTZ ? a.nz - z0 : kTZ;
+  const uint32_t nz0 = a.nz - z0 < kTZ ? a.nz - z0 : kTZ;
+  const uint32_t nz1 = dz1 < nz0 ? dz1 : nz0;
   // q of plane z at rows y0 - 1 .. y0 + kTY - 1 (row j - 1 of the tile in
   // slot j; rows outside the field or the tile are 0) for this column and,
   // on lane 0, the column to its left (the other lanes take it from their
@@ -95,16 +100,17 @@ __device__ void code_tile(const LorenzoArgs& a, uint32_t x0, uint32_t y0, uint32
   int32_t prev[kTY];
 #pragma unroll
   for (uint32_t j = 0; j < kTY; ++j) prev[j] = 0;
-  // the plane below the tile: its L2 per row
-  if (z0 > 0) {
+  // the plane below the first one: its L2 per row
+  if (z0 + dz0 > 0 && dz0 < nz1) {
     int32_t qc[kTY + 1], ql[kTY + 1];
-    plane_q(z0 - 1, qc, ql);
+    plane_q(z0 + dz0 - 1, qc, ql);
 #pragma unroll
     for (uint32_t j = 0; j < kTY; ++j)
       if (j < ny1) prev[j] = qc[j + 1] - qc[j] - ql[j + 1] + ql[j];
   }
-  for (uint32_t dz = 0; dz < nz1; ++dz) {
+  for (uint32_t dz = dz0; dz < nz1; ++dz) {
     const uint32_t z = z0 + dz;
+    if (!WRITE_ALL && dz > 0 && (dz & 3u) == 0) grp |= cnt << (8u * (dz / 4u - 1u));
     int32_t qc[kTY + 1], ql[kTY + 1];
     plane_q(z, qc, ql);
 #pragma unroll
@@ -174,11 +180,11 @@ __global__ __launch_bounds__(kTX) void lorenzo_kernel(LorenzoArgs a) {
   const uint32_t t = s_t;
   uint32_t x0, y0, z0;
   tile_of(a, t, x0, y0, z0);
-  uint32_t cnt = 0;
+  uint32_t cnt = 0, grp = 0;
   uint32_t hidx[kHold];
   int32_t hval[kHold];
   bool bad = false;
-  code_tile<false, FULL>(a, x0, y0, z0, cnt, hidx, hval, 0, bad);
+  code_tile<false, FULL>(a, x0, y0, z0, 0, kTZ, cnt, grp, hidx, hval, 0, bad);
   if (bad) s_bad = 1;
   uint32_t total = 0;
   const uint32_t mine = block_scan256(cnt, s_w, total);
@@ -210,6 +216,7 @@ __global__ __launch_bounds__(kTX) void lorenzo_kernel(LorenzoArgs a) {
     if (k < a.fix_cap) {
       if (threadIdx.x == 0) a.fix_list[1 + k] = t;
       a.fix_pos[(uint64_t)k * kTX + threadIdx.x] = mine;
+      a.fix_grp[(uint64_t)k * kTX + threadIdx.x] = grp;
     } else if (threadIdx.x == 0) {
       // more than fix_cap tiles over kHold: more than out_cap outliers
       atomicOr(a.err, kErrCapacity);
@@ -221,19 +228,25 @@ __global__ __launch_bounds__(kTX) void lorenzo_kernel(LorenzoArgs a) {
   }
 }
 
-// Writes every outlier of the tiles on the fix-up list, each thread from the
-// offset the main launch left for it (one pass over each listed tile).
-__global__ __launch_bounds__(kTX) void lorenzo_fixup_kernel(LorenzoArgs a) {
+// Writes every outlier of the tiles on the fix-up list.  A workgroup of
+// kFixGroups x 256 threads takes a listed tile, thread (g, column) codes
+// planes [4g, 4g + 4) of its column from the offsets the main launch left:
+// the column's first outlier in the tile plus its count before plane 4g.
+constexpr uint32_t kFixGroups = kTZ / 4;
+__global__ __launch_bounds__(kTX * kFixGroups) void lorenzo_fixup_kernel(LorenzoArgs a) {
   const uint32_t nfix = a.fix_list[0] < a.fix_cap ? a.fix_list[0] : a.fix_cap;
+  const uint32_t col = threadIdx.x % kTX, g = threadIdx.x / kTX;
   for (uint32_t k = blockIdx.x; k < nfix; k += gridDim.x) {
     const uint32_t t = a.fix_list[1 + k];
     uint32_t x0, y0, z0;
     tile_of(a, t, x0, y0, z0);
-    uint32_t cnt = 0, dummy_i[kHold];
+    uint32_t cnt = 0, grp = 0, dummy_i[kHold];
     int32_t dummy_v[kHold];
     bool bad = false;
-    const uint64_t pos = a.tile_off[t] + a.fix_pos[(uint64_t)k * kTX + threadIdx.x];
-    code_tile<true, false>(a, x0, y0, z0, cnt, dummy_i, dummy_v, pos, bad);
+    const uint64_t e = (uint64_t)k * kTX + col;
+    const uint32_t before = g ? (a.fix_grp[e] >> (8u * (g - 1u))) & 0xffu : 0u;
+    const uint64_t pos = a.tile_off[t] + a.fix_pos[e] + before;
+    code_tile<true, false>(a, x0, y0, z0, 4u * g, 4u * g + 4u, cnt, grp, dummy_i, dummy_v, pos, bad);
   }
 }
 
@@ -398,7 +411,7 @@ hipError_t launch_lorenzo(const LorenzoArgs& a, hipStream_t s) {
   // the fix-up list is short (a tile is on it only with more than kHold
   // outliers in one thread): a small grid walks it
   const uint32_t g = a.fix_cap < 1024u ? a.fix_cap : 1024u;
-  hipLaunchKernelGGL(lorenzo_fixup_kernel, dim3(g), dim3(kTX), 0, s, a);
+  hipLaunchKernelGGL(lorenzo_fixup_kernel, dim3(g), dim3(kTX * kFixGroups), 0, s, a);
   return hipGetLastError();
 }
 

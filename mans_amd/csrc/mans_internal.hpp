@@ -302,7 +302,7 @@ hipError_t launch_gather_headers(const uint8_t* const* src, uint32_t count, uint
 // ---- dual-quantisation Lorenzo front end (lorenzo.hip) ----------------------
 // outliers a thread of the Lorenzo forward launch holds in registers; a tile
 // with a thread over it goes to the fix-up launch
-constexpr int kLorenzoHold = 6;
+constexpr int kLorenzoHold = 5;
 struct LorenzoArgs {
   const float* field;            // forward: f[nz][ny][nx]
   uint16_t* codes;               // forward: out; inverse: in
@@ -320,6 +320,7 @@ struct LorenzoArgs {
   uint64_t* tile_off;            // [ntiles] first outlier of each tile
   uint32_t* fix_list;            // [1 + fix_cap]: count (zeroed before), then tiles for the fix-up launch
   uint32_t* fix_pos;             // [fix_cap][256]: each thread's first outlier inside a listed tile
+  uint32_t* fix_grp;             // [fix_cap][256]: its outliers before planes 4, 8, 12 (a byte each)
   uint32_t ntiles;
   uint32_t fix_cap;              // >= 1
   uint32_t* err;                 // kErrCapacity: outliers beyond out_cap; kErrAdmDomain: |q| >= 2^28
