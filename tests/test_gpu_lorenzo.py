@@ -49,6 +49,11 @@ def check(f, eb, radius, ws, cap=None):
     odd[1:] = codes
     rec2 = mans_amd.lorenzo_reconstruct(odd[1:], idx, val, f.shape, eb, ws).cpu().numpy()
     assert (rec2 == ref).all()
+    # the field written at an address that is 4 but not 8 bytes aligned:
+    # same field
+    buf = torch.empty(codes.numel() + 1, dtype=torch.float32, device=codes.device)
+    rec3 = mans_amd.lorenzo_reconstruct(codes, idx, val, f.shape, eb, ws, out=buf[1:].view(f.shape))
+    assert (rec3.cpu().numpy() == ref).all()
     # the error bound, up to the float32 roundings of f / (2 eb) and q * 2 eb
     assert np.abs(rec.astype(np.float64) - f).max() <= eb + 4 * np.finfo(np.float32).eps * np.abs(f).max()
     ws.check()
