@@ -43,12 +43,13 @@ def check(f, eb, radius, ws, cap=None):
     rec = mans_amd.lorenzo_reconstruct(codes, idx, val, f.shape, eb, ws).cpu().numpy()
     ref = L.lorenzo_reconstruct(rc, ri, rv, f.shape, eb)
     assert (rec == ref).all()
-    # codes at an address that is 2 but not 4 bytes aligned take the
-    # one-element-a-thread path of the inverse: same field
-    odd = torch.empty(codes.numel() + 1, dtype=codes.dtype, device=codes.device)
-    odd[1:] = codes
-    rec2 = mans_amd.lorenzo_reconstruct(odd[1:], idx, val, f.shape, eb, ws).cpu().numpy()
-    assert (rec2 == ref).all()
+    # codes 2 and 4 bytes past an 8-byte boundary (2 bytes: the inverse's
+    # one-element-a-thread path): same field
+    odd = torch.empty(codes.numel() + 2, dtype=codes.dtype, device=codes.device)
+    for off in (1, 2):
+        odd[off:off + codes.numel()] = codes
+        rec2 = mans_amd.lorenzo_reconstruct(odd[off:off + codes.numel()], idx, val, f.shape, eb, ws).cpu().numpy()
+        assert (rec2 == ref).all(), off
     # the field written at an address that is 4 but not 8 bytes aligned:
     # same field
     buf = torch.empty(codes.numel() + 1, dtype=torch.float32, device=codes.device)
@@ -61,7 +62,7 @@ def check(f, eb, radius, ws, cap=None):
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (3, 5, 7), (17, 33, 300), (20, 16, 256), (5, 70, 1000),
-                                   (3, 19, 2500), (2, 4, 8192), (2, 9, 2049)])
+                                   (3, 19, 2500), (2, 4, 8192), (2, 9, 2049), (3, 9, 1026)])
 def test_lorenzo_matches_oracle(ws, shape):
     check(field(shape, sum(shape), spikes=5), 1e-3, 512, ws)
 
