@@ -26,7 +26,7 @@ constexpr int kMaxPlanes = 4;
 constexpr uint32_t kAdmLanes = 32;
 constexpr uint32_t kAdmChunk = 16;
 constexpr uint32_t kAdmGroup = kAdmLanes * kAdmChunk;  // 512 elements
-constexpr uint32_t kAdmTicketGroups = 32;              // groups per map ticket (one look-back entry; 16: 0.55 ms, 64: 0.53 ms on 512 MiB)
+constexpr uint32_t kAdmTicketGroups = 32;              // groups per map ticket (one look-back entry; on 512 MiB 16: 0.55 ms, 32 and 64: 0.46 ms)
 constexpr uint32_t kAdmMod = 126;
 constexpr uint32_t kAdmLaneBytes = 32;                 // 256-bit lane buffer
 
